@@ -1,0 +1,127 @@
+/*
+ * ozaie.h -- C ABI of the MI355X-native Ozaki-scheme DGEMM library (libozaie.so).
+ *
+ * Plain C types only (pointers, sizes, ints); usable from C, C++, ctypes, cgo, JNI ...
+ * The reference's C++ API (namespace oz_aie in include/oz_aie.h of atlarge-research/ozAIE)
+ * is provided on top of this ABI by include/oz_aie.h in this directory.
+ *
+ * Arithmetic: C = A * B with A (M x K) and B (K x N) FP64.  Every element is split into
+ * d signed b-bit slices relative to its row (A) / column (B) maximum exponent, the
+ * d(d+1)/2 slice products are exact int8 x int8 -> int32 MFMA GEMMs accumulated into d
+ * bins, the bins are summed exactly in 128-bit integers and converted to FP64.
+ * With rounding OZAIE_ROUND_TRUNCATE (the default) the result is bit-identical to the
+ * reference rule (include/oz_aie.h:92-150 of the reference).
+ */
+#ifndef OZAIE_H
+#define OZAIE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes.  0..4 keep the reference's meanings (include/oz_aie.h:187-248,
+ *      337-340 of the reference); 5.. are new.                                          */
+#define OZAIE_OK 0
+#define OZAIE_ERR_DIMS 1        /* dimensions rejected (reference: non-square / K range) */
+#define OZAIE_ERR_ENV 2         /* reference: environment variable missing; never returned */
+#define OZAIE_ERR_INCOMPLETE 3  /* reference: run did not complete; here: kernel failure  */
+#define OZAIE_ERR_BUILD 4       /* reference: xclbin build failed; never returned          */
+#define OZAIE_ERR_HIP 5         /* a HIP runtime call failed                               */
+#define OZAIE_ERR_NODEVICE 6    /* no usable GPU                                           */
+#define OZAIE_ERR_NOSETUP 7     /* DGEMM called before a successful setup                  */
+#define OZAIE_ERR_MISMATCH 8    /* DGEMM dimensions differ from the last setup             */
+#define OZAIE_ERR_NOMEM 9       /* device allocation failed                                */
+#define OZAIE_ERR_ARG 10        /* invalid argument (null pointer, bad bits/slices, ...)   */
+#define OZAIE_ERR_OVERFLOW 11   /* product too long for exact 128-bit accumulation         */
+
+/* ---- modes.  0 and 1 are the reference's (oz_aie::Mode, include/oz_aie.h:82):
+ *   ACCURATE  b = 3, d = 29/28/27/27/26/25/25 for log2 K = 7..13, b = 4, d = 18 for 14+
+ *   FAST      b = 7, d = 8
+ * 2 and 3 are the int8-rate presets of this library: b = 7 with 3 and 7 slices.        */
+#define OZAIE_MODE_ACCURATE 0
+#define OZAIE_MODE_FAST 1
+#define OZAIE_MODE_FAST3 2
+#define OZAIE_MODE_ACCURATE7 3
+
+/* ---- options (ozaie_set_option) */
+#define OZAIE_OPT_ROUNDING 1    /* OZAIE_ROUND_TRUNCATE (default) or OZAIE_ROUND_NEAREST   */
+#define OZAIE_OPT_STRICT 2      /* 1: ozAIE_setup rejects what the reference rejects        */
+#define OZAIE_OPT_KCHUNK 3      /* cap on contraction indices per pass (0 = automatic)      */
+#define OZAIE_OPT_KERNEL 4      /* 0 auto, 1 force the bin-group kernel                     */
+#define OZAIE_ROUND_TRUNCATE 0
+#define OZAIE_ROUND_NEAREST 1
+
+/* ---- reference-compatible entry points (process-global plan, blocking, device 0)     */
+
+/* Replaces oz_aie::ozAIE_setup (include/oz_aie.h:182-300 of the reference).
+ * Picks (b, d) for `mode` from K and prepares the device buffers.  Any M, N, K >= 1 is
+ * accepted unless OZAIE_OPT_STRICT is set (then only square power-of-two K in
+ * [128, 2048], like the reference).  Returns OZAIE_OK or an error code.               */
+int32_t ozAIE_setup(int M, int N, int K, int mode);
+
+/* Explicit slice configuration: b bits per slice (1..7), d slices (1..32), b*d <= 96. */
+int32_t ozAIE_setup_slices(int M, int N, int K, int bits, int slices);
+
+/* Replaces oz_aie::ozAIE_DGEMM (include/oz_aie.h:311-349 of the reference).
+ * C = A * B, row-major, contiguous, dimensions must equal the last setup.  A, B and C
+ * may be host or device pointers.  Blocking.                                          */
+int32_t ozAIE_DGEMM(int M, int N, int K, const double *A, const double *B, double *C);
+
+/* Set / get a library option (OZAIE_OPT_*).  Applies to plans set up afterwards.       */
+int32_t ozaie_set_option(int option, int value);
+int32_t ozaie_get_option(int option);
+
+/* Number of (b, d) and passes of the current global plan:
+ * out[0]=b out[1]=d out[2]=k-chunk out[3]=passes out[4]=kernel (0 fused, 1 group)       */
+int32_t ozaie_plan_info(int32_t *out5);
+
+/* Release every device buffer and plan (reference: cleanup_xrt, include/oz_aie.h:153). */
+int32_t ozaie_release(void);
+
+/* ---- device-pointer API (explicit plans, asynchronous on a HIP stream)               */
+typedef struct ozaie_plan_s *ozaie_plan;
+
+/* Create a plan for C (M x N) = op(A) (M x K) * op(B) (K x N) with b bits and d slices.
+ * flags: bit 0 -> nearest-even rounding (default truncation).                          */
+int32_t ozaie_plan_create(ozaie_plan *plan, int64_t M, int64_t N, int64_t K, int bits,
+                          int slices, int flags);
+int32_t ozaie_plan_destroy(ozaie_plan plan);
+int32_t ozaie_plan_query(ozaie_plan plan, int32_t *out5);
+
+/* Enqueue C = op(A) * op(B) on `stream` (a hipStream_t, 0 = default stream).  All
+ * pointers are device pointers.  transA: 0 -> A is M x K row-major with leading
+ * dimension lda; 1 -> A is stored K x M (op(A) = A^T).  Same for B (N vs K).           */
+int32_t ozaie_dgemm(ozaie_plan plan, int transA, int transB, const double *A, int64_t lda,
+                    const double *B, int64_t ldb, double *C, int64_t ldc, void *stream);
+
+/* ---- inspection (reference full_dump equivalent, whole_array_iron_ozaki.py:485-558)  */
+
+/* Exponents and slices of one operand, as computed on the GPU, in plain layout.
+ * X is a device pointer to a rows x K view: element (r, k) at X[r*sr + k*sk].
+ * E_out: int32[rows] (host), S_out: int8[d][rows][K] (host).                          */
+int32_t ozaie_debug_split(const double *X, int64_t rows, int64_t K, int64_t sr, int64_t sk,
+                          int bits, int slices, int32_t *E_out, int8_t *S_out);
+
+/* The d int32 bins of C = A * B (row-major device A, B) as computed by the slice GEMM
+ * kernels (bins_out: host int32[d][M][N]).  K must fit one pass.  kernel: 0 fused,
+ * 1 bin-group.                                                                        */
+int32_t ozaie_debug_bins(int64_t M, int64_t N, int64_t K, const double *A, const double *B,
+                         int bits, int slices, int kernel, int32_t *bins_out);
+
+/* Reference wire format: the paired int64 {L, H} per element that the reference's NPU
+ * returns (ozaki_base.cc:143-340), computed from GPU bins (host int64[M][N][2]).      */
+int32_t ozaie_paired_output(int64_t M, int64_t N, int64_t K, const double *A,
+                            const double *B, int bits, int slices, int64_t *out);
+
+/* Last HIP error string (for diagnostics). */
+const char *ozaie_last_error(void);
+
+/* Library version, e.g. 100 = 0.1.0 */
+int32_t ozaie_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OZAIE_H */

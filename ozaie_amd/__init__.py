@@ -1,0 +1,298 @@
+"""ozaie_amd -- MI355X-native Ozaki-scheme DGEMM emulation.
+
+Python host mirror of the reference's API (include/oz_aie.h of atlarge-research/ozAIE):
+``ozAIE_setup(M, N, K, mode)`` and ``ozAIE_DGEMM(M, N, K, A, B, C)`` with
+``Mode.accurate`` / ``Mode.fast`` and the same status codes, plus explicit plans on
+device tensors, inspection of slices/bins and the reference wire format.
+
+Everything calls libozaie.so (HIP kernels for gfx950 behind a C ABI, built from
+ozaie_amd/csrc by ``ozaie_amd.build``).  There is no CPU fallback: if the library is
+missing, every call raises.
+"""
+import ctypes
+import enum
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libozaie.so")
+
+OK = 0
+ERR_DIMS = 1
+ERR_ENV = 2
+ERR_INCOMPLETE = 3
+ERR_BUILD = 4
+ERR_HIP = 5
+ERR_NODEVICE = 6
+ERR_NOSETUP = 7
+ERR_MISMATCH = 8
+ERR_NOMEM = 9
+ERR_ARG = 10
+ERR_OVERFLOW = 11
+
+OPT_ROUNDING = 1
+OPT_STRICT = 2
+OPT_KCHUNK = 3
+OPT_KERNEL = 4
+ROUND_TRUNCATE = 0
+ROUND_NEAREST = 1
+
+
+class Mode(enum.IntEnum):
+    """oz_aie::Mode (include/oz_aie.h:82) plus the int8-rate presets of this library."""
+    accurate = 0   # reference table: b = 3, d = 29 .. 25 by log2 K (b = 4, d = 18 at 2^14+)
+    fast = 1       # reference: b = 7, d = 8
+    fast3 = 2      # b = 7, d = 3
+    accurate7 = 3  # b = 7, d = 7
+
+
+# C symbols declared in include/ozaie.h (the tests check that every one is exported).
+EXPORTS = [
+    "ozAIE_setup", "ozAIE_setup_slices", "ozAIE_DGEMM", "ozaie_set_option", "ozaie_get_option",
+    "ozaie_plan_info", "ozaie_release", "ozaie_plan_create", "ozaie_plan_destroy",
+    "ozaie_plan_query", "ozaie_dgemm", "ozaie_debug_split", "ozaie_debug_bins",
+    "ozaie_paired_output", "ozaie_last_error", "ozaie_version",
+]
+
+
+class OzaieError(RuntimeError):
+    def __init__(self, code, what):
+        self.code = code
+        msg = f"{what} failed with status {code}"
+        try:
+            last = _lib_handle.ozaie_last_error().decode() if _lib_handle is not None else ""
+        except Exception:
+            last = ""
+        if last:
+            msg += f": {last}"
+        super().__init__(msg)
+
+
+_lib_handle = None
+
+
+def lib():
+    """Load libozaie.so (raises if it has not been built)."""
+    global _lib_handle
+    if _lib_handle is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m ozaie_amd.build` "
+                               "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib_handle = L
+    return _lib_handle
+
+
+def _declare(L):
+    P = ctypes.c_void_p
+    i32 = ctypes.c_int32
+    i64 = ctypes.c_int64
+    c_int = ctypes.c_int
+    L.ozAIE_setup.argtypes = [c_int, c_int, c_int, c_int]
+    L.ozAIE_setup.restype = i32
+    L.ozAIE_setup_slices.argtypes = [c_int, c_int, c_int, c_int, c_int]
+    L.ozAIE_setup_slices.restype = i32
+    L.ozAIE_DGEMM.argtypes = [c_int, c_int, c_int, P, P, P]
+    L.ozAIE_DGEMM.restype = i32
+    L.ozaie_set_option.argtypes = [c_int, c_int]
+    L.ozaie_set_option.restype = i32
+    L.ozaie_get_option.argtypes = [c_int]
+    L.ozaie_get_option.restype = i32
+    L.ozaie_plan_info.argtypes = [P]
+    L.ozaie_plan_info.restype = i32
+    L.ozaie_release.argtypes = []
+    L.ozaie_release.restype = i32
+    L.ozaie_plan_create.argtypes = [ctypes.POINTER(P), i64, i64, i64, c_int, c_int, c_int]
+    L.ozaie_plan_create.restype = i32
+    L.ozaie_plan_destroy.argtypes = [P]
+    L.ozaie_plan_destroy.restype = i32
+    L.ozaie_plan_query.argtypes = [P, P]
+    L.ozaie_plan_query.restype = i32
+    L.ozaie_dgemm.argtypes = [P, c_int, c_int, P, i64, P, i64, P, i64, P]
+    L.ozaie_dgemm.restype = i32
+    L.ozaie_debug_split.argtypes = [P, i64, i64, i64, i64, c_int, c_int, P, P]
+    L.ozaie_debug_split.restype = i32
+    L.ozaie_debug_bins.argtypes = [i64, i64, i64, P, P, c_int, c_int, c_int, P]
+    L.ozaie_debug_bins.restype = i32
+    L.ozaie_paired_output.argtypes = [i64, i64, i64, P, P, c_int, c_int, P]
+    L.ozaie_paired_output.restype = i32
+    L.ozaie_last_error.argtypes = []
+    L.ozaie_last_error.restype = ctypes.c_char_p
+    L.ozaie_version.argtypes = []
+    L.ozaie_version.restype = i32
+
+
+def _check(rc, what):
+    if rc != OK:
+        raise OzaieError(rc, what)
+    return rc
+
+
+def _ptr(x):
+    """Raw pointer of a numpy array or a torch tensor."""
+    if isinstance(x, np.ndarray):
+        return ctypes.c_void_p(x.ctypes.data)
+    return ctypes.c_void_p(x.data_ptr())
+
+
+def mode_slices(mode, K):
+    """(b, d) that ozAIE_setup picks for `mode` (reference table include/oz_aie.h:60-80)."""
+    import math
+    mode = Mode(mode)
+    if mode == Mode.accurate:
+        table = {7: (3, 29), 8: (3, 28), 9: (3, 27), 10: (3, 27), 11: (3, 26), 12: (3, 25),
+                 13: (3, 25), 14: (4, 18)}
+        lk = min(max(int(math.log2(K)), 7), 14)
+        return table[lk]
+    return {Mode.fast: (7, 8), Mode.fast3: (7, 3), Mode.accurate7: (7, 7)}[mode]
+
+
+# ---------------------------------------------------------------- reference-style API
+def ozAIE_setup(M, N, K, mode=Mode.accurate):
+    """Status code, like oz_aie::ozAIE_setup (0 = OK)."""
+    return lib().ozAIE_setup(int(M), int(N), int(K), int(mode))
+
+
+def ozAIE_setup_slices(M, N, K, bits, slices):
+    return lib().ozAIE_setup_slices(int(M), int(N), int(K), int(bits), int(slices))
+
+
+def ozAIE_DGEMM(M, N, K, A, B, C):
+    """C = A @ B with the configuration of the last setup; A, B, C are contiguous float64
+    numpy arrays (host) or torch tensors (host or device).  Returns the status code."""
+    for x in (A, B, C):
+        if isinstance(x, np.ndarray):
+            if x.dtype != np.float64 or not x.flags["C_CONTIGUOUS"]:
+                raise ValueError("numpy operands must be C-contiguous float64")
+        else:
+            if str(x.dtype) != "torch.float64" or not x.is_contiguous():
+                raise ValueError("torch operands must be contiguous float64")
+    return lib().ozAIE_DGEMM(int(M), int(N), int(K), _ptr(A), _ptr(B), _ptr(C))
+
+
+def set_option(option, value):
+    return _check(lib().ozaie_set_option(int(option), int(value)), "ozaie_set_option")
+
+
+def get_option(option):
+    return lib().ozaie_get_option(int(option))
+
+
+def plan_info():
+    out = np.zeros(5, dtype=np.int32)
+    _check(lib().ozaie_plan_info(_ptr(out)), "ozaie_plan_info")
+    return dict(bits=int(out[0]), slices=int(out[1]), kchunk=int(out[2]), passes=int(out[3]),
+                kernel=("fused", "group")[int(out[4])])
+
+
+def release():
+    return lib().ozaie_release()
+
+
+# ---------------------------------------------------------------- explicit plans (torch)
+def _stream_handle(stream):
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class Plan:
+    """A device plan for C (M x N) = op(A) (M x K) @ op(B) (K x N) with b bits, d slices."""
+
+    def __init__(self, M, N, K, bits=7, slices=8, rounding=ROUND_TRUNCATE):
+        self.M, self.N, self.K, self.bits, self.slices = int(M), int(N), int(K), int(bits), int(slices)
+        h = ctypes.c_void_p()
+        _check(lib().ozaie_plan_create(ctypes.byref(h), self.M, self.N, self.K, self.bits,
+                                       self.slices, 1 if rounding == ROUND_NEAREST else 0),
+               "ozaie_plan_create")
+        self._h = h
+
+    def info(self):
+        out = np.zeros(5, dtype=np.int32)
+        _check(lib().ozaie_plan_query(self._h, _ptr(out)), "ozaie_plan_query")
+        return dict(bits=int(out[0]), slices=int(out[1]), kchunk=int(out[2]), passes=int(out[3]),
+                    kernel=("fused", "group")[int(out[4])])
+
+    def __call__(self, A, B, C=None, transA=False, transB=False, stream=None):
+        """C = op(A) @ op(B) on device float64 tensors (asynchronous on `stream`)."""
+        import torch
+        if C is None:
+            C = torch.empty((self.M, self.N), dtype=torch.float64, device=A.device)
+        for x in (A, B, C):
+            if x.dtype != torch.float64 or not x.is_cuda:
+                raise ValueError("operands must be float64 device tensors")
+            if x.stride(1) != 1:
+                raise ValueError("operands must have unit stride along their last axis")
+        ea = (self.K, self.M) if transA else (self.M, self.K)
+        eb = (self.N, self.K) if transB else (self.K, self.N)
+        if tuple(A.shape) != ea or tuple(B.shape) != eb or tuple(C.shape) != (self.M, self.N):
+            raise ValueError(f"shape mismatch: A {tuple(A.shape)} (want {ea}), B {tuple(B.shape)} "
+                             f"(want {eb}), C {tuple(C.shape)}")
+        _check(lib().ozaie_dgemm(self._h, int(transA), int(transB), _ptr(A), A.stride(0), _ptr(B),
+                                 B.stride(0), _ptr(C), C.stride(0), _stream_handle(stream)),
+               "ozaie_dgemm")
+        return C
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().ozaie_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def matmul(A, B, bits=7, slices=8, rounding=ROUND_TRUNCATE):
+    """One-shot emulated C = A @ B on device float64 tensors (creates a plan)."""
+    p = Plan(A.shape[0], B.shape[1], A.shape[1], bits, slices, rounding)
+    try:
+        return p(A, B)
+    finally:
+        import torch
+        torch.cuda.current_stream().synchronize()
+        p.close()
+
+
+# ---------------------------------------------------------------- inspection
+def debug_split(X, bits, slices, contract_axis=1):
+    """(E int32[rows], S int8[d][rows][K]) of a device float64 matrix as split on the GPU.
+    contract_axis=1: rows of X are operand rows (A); 0: columns of X are (B)."""
+    if contract_axis == 1:
+        rows, K, sr, sk = X.shape[0], X.shape[1], X.stride(0), X.stride(1)
+    else:
+        rows, K, sr, sk = X.shape[1], X.shape[0], X.stride(1), X.stride(0)
+    E = np.zeros(rows, dtype=np.int32)
+    S = np.zeros((slices, rows, K), dtype=np.int8)
+    _check(lib().ozaie_debug_split(_ptr(X), rows, K, sr, sk, bits, slices, _ptr(E), _ptr(S)),
+           "ozaie_debug_split")
+    return E, S
+
+
+def debug_bins(A, B, bits, slices, kernel=0):
+    """int32 bins [d][M][N] of A @ B (device float64, row-major) from the slice GEMM kernels."""
+    M, K = A.shape
+    N = B.shape[1]
+    out = np.zeros((slices, M, N), dtype=np.int32)
+    _check(lib().ozaie_debug_bins(M, N, K, _ptr(A), _ptr(B), bits, slices, kernel, _ptr(out)),
+           "ozaie_debug_bins")
+    return out
+
+
+def paired_output(A, B, bits, slices):
+    """Reference wire format: int64 [M][N][2] = {L, H} per element (ozaki_base.cc:143-340)."""
+    M, K = A.shape
+    N = B.shape[1]
+    out = np.zeros((M, N, 2), dtype=np.int64)
+    _check(lib().ozaie_paired_output(M, N, K, _ptr(A), _ptr(B), bits, slices, _ptr(out)),
+           "ozaie_paired_output")
+    return out
+
+
+def version():
+    return lib().ozaie_version()

@@ -1,0 +1,61 @@
+"""Build libozaie.so (HIP kernels + C ABI) for gfx950 with hipcc, in-tree."""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libozaie.so")
+ARCH = os.environ.get("OZAIE_OFFLOAD_ARCH", "gfx950")
+SOURCES = ["oz_kernels.hip", "oz_api.cpp"]
+
+
+def hipcc():
+    for c in ("/opt/rocm/bin/hipcc", "hipcc"):
+        if os.path.sep not in c or os.path.exists(c):
+            return c
+    return "hipcc"
+
+
+def build(verbose=False, jobs=2):
+    """Compile every source to an object (in parallel), then link the shared library."""
+    common = [hipcc(), "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
+              # packed FP32 instructions are disabled for all device code
+              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+              "-I" + os.path.join(HERE, "..", "include")]
+    procs = []
+    objs = []
+    for src in SOURCES:
+        obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
+        cmd = common + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if src.endswith(".cpp"):
+            cmd.insert(1, "-xhip")
+        objs.append(obj)
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT))
+    errs = []
+    for p, src in zip(procs, SOURCES):
+        out, _ = p.communicate()
+        text = "\n".join(l for l in out.decode(errors="replace").splitlines()
+                         if "packed-fp32-ops" not in l)
+        if p.returncode != 0:
+            errs.append(f"{src}:\n{text}")
+        elif verbose and text.strip():
+            print(text)
+    if errs:
+        raise RuntimeError("hipcc failed:\n" + "\n".join(errs))
+    tmp = LIB + ".tmp"
+    link = [hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    if verbose:
+        print(" ".join(link), flush=True)
+    subprocess.check_call(link)
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,504 @@
+// oz_api.cpp -- C ABI of libozaie.so: planner, device buffers, entry points.
+//
+// The reference's host layer (include/oz_aie.h:31-349 of atlarge-research/ozAIE) keeps one
+// process-global configuration, picks (b, d) from a table indexed by log2 K, builds or
+// loads an NPU binary, allocates XRT buffers and converts the NPU's paired-int64 output
+// on the host.  Here the same public entry points sit on a planner that picks (b, d),
+// the contraction chunk that keeps every int32 bin exact, the kernel family and the
+// device buffers; there is no compile step and the conversion runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ozaie.h"
+#include "oz_internal.h"
+
+using namespace oz;
+
+namespace {
+
+std::mutex g_mu;
+std::string g_last_error;
+int g_opt_rounding = OZAIE_ROUND_TRUNCATE;
+int g_opt_strict = 0;
+int g_opt_kchunk = 0;
+int g_opt_kernel = 0;
+
+int set_hip_error(hipError_t e, const char *where) {
+    g_last_error = std::string(where) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
+    if (e == hipErrorOutOfMemory) return OZAIE_ERR_NOMEM;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return OZAIE_ERR_NODEVICE;
+    return OZAIE_ERR_HIP;
+}
+
+#define HIPCHK(expr)                                   \
+    do {                                               \
+        hipError_t e_ = (expr);                        \
+        if (e_ != hipSuccess) return set_hip_error(e_, #expr); \
+    } while (0)
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// Reference mode table, include/oz_aie.h:60-80 (log2 K -> (b, d)).
+const int kAccurate[8][2] = {{3, 29}, {3, 28}, {3, 27}, {3, 27}, {3, 26}, {3, 25}, {3, 25}, {4, 18}};
+
+int ensure_device() {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        g_last_error = "no HIP device available";
+        return OZAIE_ERR_NODEVICE;
+    }
+    return OZAIE_OK;
+}
+
+}  // namespace
+
+struct ozaie_plan_s {
+    int64_t M = 0, N = 0, K = 0;
+    int b = 0, d = 0, rounding = 0;
+    int64_t Mp = 0, Np = 0, Kp = 0, Kb = 0;
+    int64_t kcb = 0;      // contraction blocks per pass
+    int nchunks = 0, ngroups = 1, kernel = 0;
+    int device = 0;
+    int8_t *SA = nullptr, *SB = nullptr;
+    int32_t *Er = nullptr, *Ec = nullptr;
+    __int128 *T = nullptr;
+    // staging for host pointers (reference-compatible entry point)
+    double *dA = nullptr, *dB = nullptr, *dC = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+
+void plan_free(ozaie_plan p) {
+    if (!p) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(p->device);
+    if (p->SA) (void)hipFree(p->SA);
+    if (p->SB) (void)hipFree(p->SB);
+    if (p->Er) (void)hipFree(p->Er);
+    if (p->Ec) (void)hipFree(p->Ec);
+    if (p->T) (void)hipFree(p->T);
+    if (p->dA) (void)hipFree(p->dA);
+    if (p->dB) (void)hipFree(p->dB);
+    if (p->dC) (void)hipFree(p->dC);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    (void)hipSetDevice(cur);
+    delete p;
+}
+
+// Contraction indices per pass so that no int32 bin can wrap: the largest bin sums
+// d slice products of magnitude <= (2^b - 1)^2 over the pass.
+int64_t max_kchunk(int b, int d) {
+    const int64_t m = (1 << b) - 1;
+    int64_t kc = (int64_t)INT32_MAX / ((int64_t)d * m * m);
+    return kc / kBlk * kBlk;
+}
+
+int plan_configure(ozaie_plan p, int64_t M, int64_t N, int64_t K, int b, int d, int rounding) {
+    if (M < 1 || N < 1 || K < 1) { g_last_error = "dimensions must be >= 1"; return OZAIE_ERR_DIMS; }
+    if (M > (1ll << 31) - 256 || N > (1ll << 31) - 256 || K > (1ll << 40)) {
+        g_last_error = "dimensions too large"; return OZAIE_ERR_DIMS;
+    }
+    if (b < 1 || b > kMaxBits || d < 1 || d > kMaxSlices || b * d > 96) {
+        g_last_error = "need 1 <= bits <= 7, 1 <= slices <= 32, bits*slices <= 96";
+        return OZAIE_ERR_ARG;
+    }
+    // |T| <= 4 K (2^b - 1)^2 2^(b(d-1)) must stay below 2^127.
+    if (2.0 + std::log2((double)K) + 2.0 * b + (double)b * (d - 1) > 126.0) {
+        g_last_error = "product too long for exact 128-bit accumulation";
+        return OZAIE_ERR_OVERFLOW;
+    }
+    p->M = M; p->N = N; p->K = K; p->b = b; p->d = d; p->rounding = rounding;
+    p->Mp = round_up(M, kRowPad);
+    p->Np = round_up(N, kRowPad);
+    p->Kp = round_up(K, kBlk);
+    p->Kb = p->Kp / kBlk;
+    int64_t kc = max_kchunk(b, d);
+    if (g_opt_kchunk > 0) kc = std::min<int64_t>(kc, std::max<int64_t>(kBlk, g_opt_kchunk / kBlk * kBlk));
+    p->kcb = std::max<int64_t>(1, kc / kBlk);
+    p->nchunks = (int)((p->Kb + p->kcb - 1) / p->kcb);
+    p->kernel = (fused_supported(d) && g_opt_kernel != 1) ? 0 : 1;
+    p->ngroups = p->kernel == 0 ? 1 : (d + kGroupBins - 1) / kGroupBins;
+    if (p->Mp / kBlk > 65535 || p->Np / kBlk > 65535) {
+        g_last_error = "too many rows for the split grid"; return OZAIE_ERR_DIMS;
+    }
+    return OZAIE_OK;
+}
+
+int plan_alloc(ozaie_plan p, bool staging) {
+    HIPCHK(hipGetDevice(&p->device));
+    const size_t sa = (size_t)p->d * p->Mp * p->Kp, sb = (size_t)p->d * p->Np * p->Kp;
+    HIPCHK(hipMalloc(&p->SA, sa));
+    HIPCHK(hipMalloc(&p->SB, sb));
+    HIPCHK(hipMalloc(&p->Er, sizeof(int32_t) * p->Mp));
+    HIPCHK(hipMalloc(&p->Ec, sizeof(int32_t) * p->Np));
+    if ((int64_t)p->nchunks * p->ngroups > 1)
+        HIPCHK(hipMalloc(&p->T, sizeof(__int128) * (size_t)p->M * p->N));
+    if (staging) {
+        HIPCHK(hipMalloc(&p->dA, sizeof(double) * (size_t)p->M * p->K));
+        HIPCHK(hipMalloc(&p->dB, sizeof(double) * (size_t)p->K * p->N));
+        HIPCHK(hipMalloc(&p->dC, sizeof(double) * (size_t)p->M * p->N));
+        HIPCHK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    }
+    return OZAIE_OK;
+}
+
+int plan_new(ozaie_plan *out, int64_t M, int64_t N, int64_t K, int b, int d, int rounding,
+             bool staging) {
+    *out = nullptr;
+    int rc = ensure_device();
+    if (rc) return rc;
+    ozaie_plan p = new ozaie_plan_s();
+    rc = plan_configure(p, M, N, K, b, d, rounding);
+    if (!rc) rc = plan_alloc(p, staging);
+    if (rc) { plan_free(p); return rc; }
+    *out = p;
+    return OZAIE_OK;
+}
+
+OperandView view_A(const ozaie_plan p, int trans, const double *A, int64_t lda) {
+    OperandView v;
+    v.X = A; v.R = p->M; v.K = p->K;
+    if (!trans) { v.sr = lda; v.sk = 1; } else { v.sr = 1; v.sk = lda; }
+    return v;
+}
+
+OperandView view_B(const ozaie_plan p, int trans, const double *B, int64_t ldb) {
+    OperandView v;
+    v.X = B; v.R = p->N; v.K = p->K;
+    if (!trans) { v.sr = 1; v.sk = ldb; } else { v.sr = ldb; v.sk = 1; }
+    return v;
+}
+
+GemmLaunch base_launch(const ozaie_plan p) {
+    GemmLaunch g;
+    std::memset(&g, 0, sizeof(g));
+    g.SA = p->SA; g.SB = p->SB; g.Er = p->Er; g.Ec = p->Ec;
+    g.RbA = p->Mp / kBlk; g.RbB = p->Np / kBlk; g.Kb = p->Kb;
+    g.M = (int)p->M; g.N = (int)p->N; g.b = p->b; g.d = p->d;
+    g.rounding = p->rounding;
+    g.T = p->T; g.ldT = p->N;
+    g.K_true = (int)std::min<int64_t>(p->K, INT32_MAX);
+    return g;
+}
+
+// Exponents + slices of both operands.
+int run_split(ozaie_plan p, const OperandView &va, const OperandView &vb, hipStream_t st) {
+    HIPCHK(launch_maxexp(va, p->Er, p->Mp, st));
+    HIPCHK(launch_maxexp(vb, p->Ec, p->Np, st));
+    HIPCHK(launch_split(va, p->Er, p->b, p->d, p->SA, p->Mp, p->Kp, st));
+    HIPCHK(launch_split(vb, p->Ec, p->b, p->d, p->SB, p->Np, p->Kp, st));
+    return OZAIE_OK;
+}
+
+int run_passes(ozaie_plan p, double *C, int64_t ldc, int epi, int32_t *bins, hipStream_t st) {
+    GemmLaunch g = base_launch(p);
+    g.C = C; g.ldc = ldc; g.bins = bins;
+    const int npass = p->nchunks * p->ngroups;
+    int pass = 0;
+    for (int c = 0; c < p->nchunks; c++) {
+        g.kb0 = (int64_t)c * p->kcb;
+        g.kb1 = std::min<int64_t>(p->Kb, g.kb0 + p->kcb);
+        for (int gi = 0; gi < p->ngroups; gi++, pass++) {
+            g.k0 = gi * kGroupBins;
+            g.first = pass == 0;
+            g.last = pass == npass - 1;
+            if (p->kernel == 0) HIPCHK(launch_gemm_fused(g, epi, st));
+            else HIPCHK(launch_gemm_group(g, epi, st));
+        }
+    }
+    return OZAIE_OK;
+}
+
+int plan_execute(ozaie_plan p, int transA, int transB, const double *A, int64_t lda,
+                 const double *B, int64_t ldb, double *C, int64_t ldc, hipStream_t st) {
+    if (!A || !B || !C) { g_last_error = "null pointer"; return OZAIE_ERR_ARG; }
+    const int64_t minlda = transA ? p->M : p->K, minldb = transB ? p->K : p->N;
+    if (lda < minlda || ldb < minldb || ldc < p->N) { g_last_error = "leading dimension too small"; return OZAIE_ERR_ARG; }
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != p->device) HIPCHK(hipSetDevice(p->device));
+    int rc = run_split(p, view_A(p, transA, A, lda), view_B(p, transB, B, ldb), st);
+    if (!rc) rc = run_passes(p, C, ldc, EPI_C, nullptr, st);
+    if (cur != p->device) (void)hipSetDevice(cur);
+    return rc;
+}
+
+bool is_device_ptr(const void *ptr) {
+    hipPointerAttribute_t attr;
+    std::memset(&attr, 0, sizeof(attr));
+    hipError_t e = hipPointerGetAttributes(&attr, ptr);
+    if (e != hipSuccess) { (void)hipGetLastError(); return false; }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+ozaie_plan g_plan = nullptr;
+
+int setup_global(int64_t M, int64_t N, int64_t K, int b, int d) {
+    if (g_plan) { plan_free(g_plan); g_plan = nullptr; }
+    return plan_new(&g_plan, M, N, K, b, d, g_opt_rounding, true);
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ozAIE_setup(int M, int N, int K, int mode) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (M < 1 || N < 1 || K < 1) {
+        std::fprintf(stderr, "ozAIE_setup: dimensions must be >= 1\n");
+        return OZAIE_ERR_DIMS;
+    }
+    int log2k = (int)std::log2((double)K);
+    if (g_opt_strict && (log2k < 7 || log2k > 11 || M != K || N != K || (K & (K - 1)))) {
+        std::fprintf(stderr, "Only square matrices are currently supported\n");
+        std::fprintf(stderr, "Single side dimension : power of 2 between 128 and 2048 (inclusive)\n");
+        return OZAIE_ERR_DIMS;
+    }
+    int b, d;
+    switch (mode) {
+        case OZAIE_MODE_ACCURATE: {
+            const int idx = std::min(std::max(log2k, 7), 14) - 7;
+            b = kAccurate[idx][0]; d = kAccurate[idx][1];
+            break;
+        }
+        case OZAIE_MODE_FAST: b = 7; d = 8; break;
+        case OZAIE_MODE_FAST3: b = 7; d = 3; break;
+        case OZAIE_MODE_ACCURATE7: b = 7; d = 7; break;
+        default:
+            g_last_error = "unknown mode";
+            return OZAIE_ERR_ARG;
+    }
+    return setup_global(M, N, K, b, d);
+}
+
+int32_t ozAIE_setup_slices(int M, int N, int K, int bits, int slices) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (M < 1 || N < 1 || K < 1) return OZAIE_ERR_DIMS;
+    return setup_global(M, N, K, bits, slices);
+}
+
+int32_t ozAIE_DGEMM(int M, int N, int K, const double *A, const double *B, double *C) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    ozaie_plan p = g_plan;
+    if (!p) { g_last_error = "ozAIE_DGEMM before ozAIE_setup"; return OZAIE_ERR_NOSETUP; }
+    if (M != p->M || N != p->N || K != p->K) { g_last_error = "dimensions differ from setup"; return OZAIE_ERR_MISMATCH; }
+    if (!A || !B || !C) { g_last_error = "null pointer"; return OZAIE_ERR_ARG; }
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != p->device) HIPCHK(hipSetDevice(p->device));
+    hipStream_t st = p->stream;
+    const bool devA = is_device_ptr(A), devB = is_device_ptr(B), devC = is_device_ptr(C);
+    const double *a = A, *bb = B;
+    double *c = devC ? C : p->dC;
+    if (!devA) { HIPCHK(hipMemcpyAsync(p->dA, A, sizeof(double) * (size_t)M * K, hipMemcpyHostToDevice, st)); a = p->dA; }
+    if (!devB) { HIPCHK(hipMemcpyAsync(p->dB, B, sizeof(double) * (size_t)K * N, hipMemcpyHostToDevice, st)); bb = p->dB; }
+    int rc = plan_execute(p, 0, 0, a, K, bb, N, c, N, st);
+    if (rc) return rc == OZAIE_ERR_HIP ? OZAIE_ERR_INCOMPLETE : rc;
+    if (!devC) HIPCHK(hipMemcpyAsync(C, p->dC, sizeof(double) * (size_t)M * N, hipMemcpyDeviceToHost, st));
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { set_hip_error(e, "ozAIE_DGEMM"); return OZAIE_ERR_INCOMPLETE; }
+    if (cur != p->device) (void)hipSetDevice(cur);
+    return OZAIE_OK;
+}
+
+int32_t ozaie_set_option(int option, int value) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    switch (option) {
+        case OZAIE_OPT_ROUNDING:
+            if (value != OZAIE_ROUND_TRUNCATE && value != OZAIE_ROUND_NEAREST) return OZAIE_ERR_ARG;
+            g_opt_rounding = value; return OZAIE_OK;
+        case OZAIE_OPT_STRICT: g_opt_strict = value ? 1 : 0; return OZAIE_OK;
+        case OZAIE_OPT_KCHUNK:
+            if (value < 0) return OZAIE_ERR_ARG;
+            g_opt_kchunk = value; return OZAIE_OK;
+        case OZAIE_OPT_KERNEL:
+            if (value < 0 || value > 1) return OZAIE_ERR_ARG;
+            g_opt_kernel = value; return OZAIE_OK;
+        default: return OZAIE_ERR_ARG;
+    }
+}
+
+int32_t ozaie_get_option(int option) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    switch (option) {
+        case OZAIE_OPT_ROUNDING: return g_opt_rounding;
+        case OZAIE_OPT_STRICT: return g_opt_strict;
+        case OZAIE_OPT_KCHUNK: return g_opt_kchunk;
+        case OZAIE_OPT_KERNEL: return g_opt_kernel;
+        default: return -OZAIE_ERR_ARG;
+    }
+}
+
+static void fill_info(const ozaie_plan p, int32_t *out) {
+    out[0] = p->b; out[1] = p->d;
+    out[2] = (int32_t)std::min<int64_t>(p->kcb * kBlk, INT32_MAX);
+    out[3] = p->nchunks * p->ngroups;
+    out[4] = p->kernel;
+}
+
+int32_t ozaie_plan_info(int32_t *out5) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_plan) return OZAIE_ERR_NOSETUP;
+    if (!out5) return OZAIE_ERR_ARG;
+    fill_info(g_plan, out5);
+    return OZAIE_OK;
+}
+
+int32_t ozaie_release(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_plan) { plan_free(g_plan); g_plan = nullptr; }
+    return OZAIE_OK;
+}
+
+int32_t ozaie_plan_create(ozaie_plan *plan, int64_t M, int64_t N, int64_t K, int bits, int slices,
+                          int flags) {
+    if (!plan) return OZAIE_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_mu);
+    return plan_new(plan, M, N, K, bits, slices, (flags & 1) ? OZAIE_ROUND_NEAREST : OZAIE_ROUND_TRUNCATE, false);
+}
+
+int32_t ozaie_plan_destroy(ozaie_plan plan) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    plan_free(plan);
+    return OZAIE_OK;
+}
+
+int32_t ozaie_plan_query(ozaie_plan plan, int32_t *out5) {
+    if (!plan || !out5) return OZAIE_ERR_ARG;
+    fill_info(plan, out5);
+    return OZAIE_OK;
+}
+
+int32_t ozaie_dgemm(ozaie_plan plan, int transA, int transB, const double *A, int64_t lda,
+                    const double *B, int64_t ldb, double *C, int64_t ldc, void *stream) {
+    if (!plan) return OZAIE_ERR_ARG;
+    return plan_execute(plan, transA, transB, A, lda, B, ldb, C, ldc, (hipStream_t)stream);
+}
+
+int32_t ozaie_debug_split(const double *X, int64_t rows, int64_t K, int64_t sr, int64_t sk,
+                          int bits, int slices, int32_t *E_out, int8_t *S_out) {
+    if (!X || !E_out || !S_out) return OZAIE_ERR_ARG;
+    if (rows < 1 || K < 1) return OZAIE_ERR_DIMS;
+    if (bits < 1 || bits > kMaxBits || slices < 1 || slices > kMaxSlices || bits * slices > 96) return OZAIE_ERR_ARG;
+    if (sr != 1 && sk != 1) return OZAIE_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc = ensure_device();
+    if (rc) return rc;
+    const int64_t Rp = round_up(rows, kRowPad), Kp = round_up(K, kBlk);
+    int32_t *E = nullptr;
+    int8_t *S = nullptr;
+    HIPCHK(hipMalloc(&E, sizeof(int32_t) * Rp));
+    hipError_t e = hipMalloc(&S, (size_t)slices * Rp * Kp);
+    if (e != hipSuccess) { (void)hipFree(E); return set_hip_error(e, "hipMalloc"); }
+    OperandView v;
+    v.X = X; v.sr = sr; v.sk = sk; v.R = rows; v.K = K;
+    e = launch_maxexp(v, E, Rp, 0);
+    if (e == hipSuccess) e = launch_split(v, E, bits, slices, S, Rp, Kp, 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    std::vector<int8_t> hs;
+    if (e == hipSuccess) e = hipMemcpy(E_out, E, sizeof(int32_t) * rows, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) {
+        hs.resize((size_t)slices * Rp * Kp);
+        e = hipMemcpy(hs.data(), S, hs.size(), hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(E);
+    (void)hipFree(S);
+    if (e != hipSuccess) return set_hip_error(e, "ozaie_debug_split");
+    // Undo the block layout: S[j][rb][kb][ (k%32/16)*512 + (r%32)*16 + k%16 ].
+    const int64_t Kb = Kp / kBlk;
+    for (int j = 0; j < slices; j++)
+        for (int64_t r = 0; r < rows; r++)
+            for (int64_t k = 0; k < K; k++) {
+                const int64_t off = (((int64_t)j * (Rp / kBlk) + r / kBlk) * Kb + k / kBlk) * kBlkBytes +
+                                    ((k % kBlk) / 16) * 512 + (r % kBlk) * 16 + (k % 16);
+                S_out[((int64_t)j * rows + r) * K + k] = hs[off];
+            }
+    return OZAIE_OK;
+}
+
+static int debug_bins_impl(int64_t M, int64_t N, int64_t K, const double *A, const double *B, int bits,
+                           int slices, int kernel, int32_t *dbins, ozaie_plan *pout) {
+    ozaie_plan p = nullptr;
+    const int save_kernel = g_opt_kernel;
+    g_opt_kernel = kernel;
+    int rc = plan_new(&p, M, N, K, bits, slices, OZAIE_ROUND_TRUNCATE, false);
+    g_opt_kernel = save_kernel;
+    if (rc) return rc;
+    if (p->nchunks != 1) { plan_free(p); g_last_error = "K needs more than one pass"; return OZAIE_ERR_DIMS; }
+    rc = run_split(p, view_A(p, 0, A, K), view_B(p, 0, B, N), 0);
+    if (!rc) rc = run_passes(p, nullptr, N, EPI_BINS, dbins, 0);
+    if (rc) { plan_free(p); return rc; }
+    *pout = p;
+    return OZAIE_OK;
+}
+
+int32_t ozaie_debug_bins(int64_t M, int64_t N, int64_t K, const double *A, const double *B, int bits,
+                         int slices, int kernel, int32_t *bins_out) {
+    if (!A || !B || !bins_out) return OZAIE_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc = ensure_device();
+    if (rc) return rc;
+    if (M < 1 || N < 1 || K < 1) return OZAIE_ERR_DIMS;
+    int32_t *dbins = nullptr;
+    HIPCHK(hipMalloc(&dbins, sizeof(int32_t) * (size_t)std::max(slices, 1) * M * N));
+    ozaie_plan p = nullptr;
+    rc = debug_bins_impl(M, N, K, A, B, bits, slices, kernel, dbins, &p);
+    if (!rc) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e == hipSuccess) e = hipMemcpy(bins_out, dbins, sizeof(int32_t) * (size_t)slices * M * N, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = set_hip_error(e, "ozaie_debug_bins");
+    }
+    if (p) plan_free(p);
+    (void)hipFree(dbins);
+    return rc;
+}
+
+int32_t ozaie_paired_output(int64_t M, int64_t N, int64_t K, const double *A, const double *B, int bits,
+                            int slices, int64_t *out) {
+    if (!A || !B || !out) return OZAIE_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc = ensure_device();
+    if (rc) return rc;
+    if (M < 1 || N < 1 || K < 1 || K > INT32_MAX) return OZAIE_ERR_DIMS;
+    int32_t *dbins = nullptr;
+    int64_t *dout = nullptr;
+    HIPCHK(hipMalloc(&dbins, sizeof(int32_t) * (size_t)std::max(slices, 1) * M * N));
+    hipError_t e = hipMalloc(&dout, sizeof(int64_t) * 2 * (size_t)M * N);
+    if (e != hipSuccess) { (void)hipFree(dbins); return set_hip_error(e, "hipMalloc"); }
+    ozaie_plan p = nullptr;
+    rc = debug_bins_impl(M, N, K, A, B, bits, slices, 0, dbins, &p);
+    if (!rc) {
+        // group sizes n0/n1 of the reference format, ozaki_base.cc:151-161
+        const int log2k = __builtin_ctz((unsigned)K);
+        const int alignable = 31 / bits + 1;
+        const int numer = bits * (slices + 1) - 51 + log2k;
+        const int minpos = numer < 0 ? 1 : numer / bits + 1;
+        const int n0 = std::min(slices, std::max(minpos, alignable));
+        e = launch_pack_paired(dbins, M, N, bits, slices, n0, p->Er, p->Ec, dout, 0);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e == hipSuccess) e = hipMemcpy(out, dout, sizeof(int64_t) * 2 * (size_t)M * N, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = set_hip_error(e, "ozaie_paired_output");
+    }
+    if (p) plan_free(p);
+    (void)hipFree(dbins);
+    (void)hipFree(dout);
+    return rc;
+}
+
+const char *ozaie_last_error(void) { return g_last_error.c_str(); }
+
+int32_t ozaie_version(void) { return 100; }
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// oz_internal.h -- shared declarations between the HIP kernels and the C-ABI layer.
+//
+// Data layout in HBM (all device buffers are owned by a plan):
+//   slices  S[j][rb][kb][1024] int8 for j < d, rb < Rp/32, kb < Kp/32.  One 1 KiB block
+//           holds 32 rows x 32 contraction indices, stored so that lane l of a wave
+//           reads bytes [16 l, 16 l + 16): row = l % 32, k = 16 (l / 32) + [0, 16).
+//           That is exactly the operand register of v_mfma_i32_32x32x32_i8, so a block
+//           moves HBM -> LDS -> VGPR with one 16-byte access per lane at each step.
+//   E       int32 per row of A (Mp entries) and per column of B (Np entries): the
+//           maximum biased exponent field along the contraction axis.
+//   T       int128 per element of C (Mp x Np), only when a product needs several
+//           passes (K chunks or bin groups).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace oz {
+
+constexpr int kBlk = 32;          // rows / contraction indices per slice block
+constexpr int kBlkBytes = 1024;   // bytes per slice block
+constexpr int kRowPad = 128;      // rows of each operand are padded to this
+constexpr int kMaxSlices = 32;
+constexpr int kMaxBits = 7;
+
+// One operand viewed as R rows x K contraction indices: element (r, k) at X[r*sr + k*sk].
+struct OperandView {
+    const double *X;
+    int64_t sr, sk;
+    int64_t R, K;
+};
+
+struct SliceBuf {
+    int8_t *S;       // d planes
+    int32_t *E;      // Rp exponents
+    int64_t Rp, Kp;  // padded sizes
+};
+
+struct GemmLaunch {
+    const int8_t *SA, *SB;
+    const int32_t *Er, *Ec;
+    int64_t RbA, RbB, Kb;   // blocks per plane (rows/32) and contraction blocks
+    int64_t kb0, kb1;       // contraction block range of this pass
+    int M, N;
+    int b, d;
+    int k0;                 // first bin of this pass (group kernel)
+    int rounding;           // 0 truncate (reference), 1 nearest-even
+    int first, last;        // pass flags: first writes T, last converts to C
+    double *C;
+    int64_t ldc;
+    __int128 *T;            // Mp x Np (row-major, stride Np) when passes > 1
+    int64_t ldT;
+    int32_t *bins;          // debug: [d][M][N] int32 instead of C
+    int64_t *paired;        // reference wire format: [M][N][2] int64 (optional)
+    int K_true;             // K of the product (paired-format group sizes)
+};
+
+// Kernel launchers (oz_kernels.hip).  All enqueue on `st` and return hipError_t.
+hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
+hipError_t launch_maxexp(const OperandView &op, int32_t *E, int64_t Rp, hipStream_t st);
+hipError_t launch_split(const OperandView &op, const int32_t *E, int b, int d, int8_t *S,
+                        int64_t Rp, int64_t Kp, hipStream_t st);
+// Fused kernel (all d bins in registers); returns hipErrorInvalidValue if d has no
+// instantiation.  `wn` picks the wave tile (1: 32x32, 2: 32x64 per wave).
+bool fused_supported(int d);
+int fused_wn(int d);
+hipError_t launch_gemm_fused(const GemmLaunch &g, int epi, hipStream_t st);
+// Bin-group kernel for any d <= kMaxSlices: bins [g.k0, g.k0 + 8).
+constexpr int kGroupBins = 8;
+hipError_t launch_gemm_group(const GemmLaunch &g, int epi, hipStream_t st);
+
+enum Epi { EPI_C = 0, EPI_BINS = 1 };
+
+// Reference paired-int64 wire format from int32 bins [d][M][N].
+hipError_t launch_pack_paired(const int32_t *bins, int64_t M, int64_t N, int b, int d, int n0,
+                              const int32_t *Er, const int32_t *Ec, int64_t *out, hipStream_t st);
+
+}  // namespace oz

@@ -1,0 +1,559 @@
+// oz_kernels.hip -- the CDNA4 (gfx950) kernels of the Ozaki-scheme DGEMM.
+//
+//   k_maxexp_*   per-row (A) / per-column (B) maximum exponent field along the
+//                contraction axis (reference: ozaki_base.cc:342-440).
+//   k_split      sign-magnitude split of every FP64 element into d b-bit digits relative
+//                to its row/column maximum exponent, written as int8 slice blocks in the
+//                MFMA operand order (reference rule: ozaki_base.cc:449-771; closed form
+//                Q = floor(M_x * 2^(b*d - 53 - bias)), s_j = sign*((Q >> b(d-1-j)) & (2^b-1))).
+//   k_gemm_fused the triangular d(d+1)/2 int8 x int8 -> int32 slice products on
+//                v_mfma_i32_32x32x32_i8 with every bin of the output tile held in
+//                registers, and the bin summation fused into the epilogue: exact 128-bit
+//                T = sum_k bin_k 2^(b(d-1-k)), then FP64 C (reference: mm_ozaki.cc:304-357,
+//                ozaki_base.cc:143-340, include/oz_aie.h:92-150).
+//   k_gemm_group the same products for any d <= 32, eight bins per pass, for the
+//                reference's b=3 presets whose 25..29 bins do not fit in registers.
+#include "oz_internal.h"
+
+namespace oz {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+typedef __int128 i128;
+typedef unsigned __int128 u128;
+typedef __attribute__((address_space(3))) void lds_void;
+
+// ------------------------------------------------------------------ small helpers
+__global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)nb), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+__device__ __forceinline__ int field_of(double x) {
+    const uint32_t hi = (uint32_t)__double2hiint(x);
+    return (int)((hi << 1) >> 21);
+}
+
+// ------------------------------------------------------------------ max exponent
+// Contraction axis contiguous (sk == 1): one wave per row, coalesced along k.
+__global__ __launch_bounds__(256) void k_maxexp_kc(const double *__restrict__ X, int64_t sr,
+                                                   int64_t R, int64_t K, int64_t Rp,
+                                                   int32_t *__restrict__ E) {
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r >= Rp) return;  // whole wave leaves together
+    int m = 0;
+    if (r < R) {
+        const double *row = X + r * sr;
+        int64_t k = lane;
+        for (; k + 192 < K; k += 256) {
+            const double a0 = row[k], a1 = row[k + 64], a2 = row[k + 128], a3 = row[k + 192];
+            m = max(m, max(max(field_of(a0), field_of(a1)), max(field_of(a2), field_of(a3))));
+        }
+        for (; k < K; k += 64) m = max(m, field_of(row[k]));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+    if (lane == 0) E[r] = m;
+}
+
+// Row index contiguous (sr == 1): thread per row, a slab of k per block row, combined
+// with atomicMax into E (zero-filled beforehand).
+__global__ __launch_bounds__(256) void k_maxexp_rc(const double *__restrict__ X, int64_t sk,
+                                                   int64_t R, int64_t K, int64_t kper,
+                                                   int32_t *__restrict__ E) {
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= R) return;
+    const int64_t k0 = (int64_t)blockIdx.y * kper;
+    const int64_t k1 = min(K, k0 + kper);
+    int m = 0;
+    int64_t k = k0;
+    for (; k + 3 < k1; k += 4) {
+        const double a0 = X[r + k * sk], a1 = X[r + (k + 1) * sk];
+        const double a2 = X[r + (k + 2) * sk], a3 = X[r + (k + 3) * sk];
+        m = max(m, max(max(field_of(a0), field_of(a1)), max(field_of(a2), field_of(a3))));
+    }
+    for (; k < k1; k++) m = max(m, field_of(X[r + k * sk]));
+    if (m > 0) atomicMax(E + r, m);
+}
+
+hipError_t launch_maxexp(const OperandView &op, int32_t *E, int64_t Rp, hipStream_t st) {
+    if (op.sk == 1) {
+        const int64_t nb = (Rp + 3) / 4;
+        hipLaunchKernelGGL(k_maxexp_kc, dim3((unsigned)nb), dim3(256), 0, st, op.X, op.sr, op.R,
+                           op.K, Rp, E);
+        return hipGetLastError();
+    }
+    hipError_t e = launch_fill_i32(E, Rp, 0, st);
+    if (e != hipSuccess) return e;
+    if (op.R <= 0 || op.K <= 0) return hipSuccess;
+    const int64_t nbx = (op.R + 255) / 256;
+    int64_t nby = 2048 / nbx;
+    const int64_t kmax = (op.K + 63) / 64;
+    if (nby > kmax) nby = kmax;
+    if (nby < 1) nby = 1;
+    if (nby > 65535) nby = 65535;
+    const int64_t kper = (op.K + nby - 1) / nby;
+    nby = (op.K + kper - 1) / kper;
+    hipLaunchKernelGGL(k_maxexp_rc, dim3((unsigned)nbx, (unsigned)nby), dim3(256), 0, st, op.X,
+                       op.sk, op.R, op.K, kper, E);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ split
+// One block handles 32 rows x 128 contraction indices (four slice blocks).  The FP64
+// tile is staged through LDS so that the global reads are coalesced along whichever
+// axis is contiguous; each thread then owns 16 consecutive k of one row and writes one
+// 16-byte lane chunk per slice plane (a wave writes one whole 1 KiB block per plane).
+template <bool KC>
+__global__ __launch_bounds__(256) void k_split(const double *__restrict__ X, int64_t sr,
+                                               int64_t sk, int64_t R, int64_t K,
+                                               const int32_t *__restrict__ E, int b, int d,
+                                               int8_t *__restrict__ S, int64_t Rb, int64_t Kb) {
+    constexpr int TK = 128;
+    constexpr int LDW = KC ? (TK + 1) : (32 + 1);
+    __shared__ double tile[KC ? 32 * LDW : TK * LDW];
+    const int64_t rb = blockIdx.y, kt = blockIdx.x;
+    const int64_t r0 = rb * 32, k0 = kt * TK;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const int c = t + 256 * i;
+        int row, kk;
+        if (KC) { row = c >> 7; kk = c & 127; }
+        else { kk = c >> 5; row = c & 31; }
+        const int64_t r = r0 + row, k = k0 + kk;
+        double v = 0.0;
+        if (r < R && k < K) v = KC ? X[r * sr + k] : X[k * sk + r];
+        if (KC) tile[row * LDW + kk] = v;
+        else tile[kk * LDW + row] = v;
+    }
+    __syncthreads();
+    const int row = t & 31, q = t >> 5;  // k in [16q, 16q + 16) of the tile
+    const int64_t kb = kt * 4 + (q >> 1);
+    if (kb >= Kb) return;  // no barrier below
+    const int emax = (r0 + row < R) ? E[r0 + row] : 0;
+    uint64_t mx[16];
+    int bias[16];
+    uint32_t negm = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const double v = KC ? tile[row * LDW + 16 * q + i] : tile[(16 * q + i) * LDW + row];
+        const uint64_t u = (uint64_t)__double_as_longlong(v);
+        const int e = (int)((u >> 52) & 0x7FF);
+        mx[i] = (u & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
+        bias[i] = emax - e;
+        negm |= (uint32_t)(u >> 63) << i;
+    }
+    const uint64_t mask = (1ull << b) - 1ull;
+    const int64_t plane = Rb * Kb * (int64_t)kBlkBytes;
+    int8_t *dst = S + (rb * Kb + kb) * (int64_t)kBlkBytes + (q & 1) * 512 + row * 16;
+    for (int j = 0; j < d; j++) {
+        uint32_t w[4];
+#pragma unroll
+        for (int gi = 0; gi < 4; gi++) {
+            uint32_t acc = 0;
+#pragma unroll
+            for (int h = 0; h < 4; h++) {
+                const int i = 4 * gi + h;
+                const int sh = 53 + bias[i] - b * (j + 1);
+                uint64_t v;
+                if (sh >= 64) v = 0;
+                else if (sh >= 0) v = mx[i] >> sh;
+                else v = (sh <= -64) ? 0ull : (mx[i] << (-sh));
+                int dig = (int)(v & mask);
+                dig = ((negm >> i) & 1u) ? -dig : dig;
+                acc |= (uint32_t)(dig & 0xFF) << (8 * h);
+            }
+            w[gi] = acc;
+        }
+        *reinterpret_cast<uint4 *>(dst + (int64_t)j * plane) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
+hipError_t launch_split(const OperandView &op, const int32_t *E, int b, int d, int8_t *S,
+                        int64_t Rp, int64_t Kp, hipStream_t st) {
+    const int64_t Rb = Rp / kBlk, Kb = Kp / kBlk;
+    const int64_t nkt = (Kb + 3) / 4;
+    if (Rb > 65535 || nkt > 0x7fffffff) return hipErrorInvalidValue;
+    if (op.sk == 1)
+        hipLaunchKernelGGL(k_split<true>, dim3((unsigned)nkt, (unsigned)Rb), dim3(256), 0, st, op.X,
+                           op.sr, op.sk, op.R, op.K, E, b, d, S, Rb, Kb);
+    else
+        hipLaunchKernelGGL(k_split<false>, dim3((unsigned)nkt, (unsigned)Rb), dim3(256), 0, st,
+                           op.X, op.sr, op.sk, op.R, op.K, E, b, d, S, Rb, Kb);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ epilogue math
+__device__ __forceinline__ int clz128(u128 a) {
+    const uint64_t h = (uint64_t)(a >> 64), l = (uint64_t)a;
+    if (h) return __builtin_clzll(h);
+    if (l) return 64 + __builtin_clzll(l);
+    return 128;
+}
+
+// Exact integer T (scaled by 2^(es - 2044 - b(d+1))) to FP64.
+// rounding 0: sign * trunc53(|T|) via the bit arithmetic of include/oz_aie.h:124-146
+// (no over/underflow handling, as in the reference); rounding 1: nearest-even.
+__device__ __forceinline__ double t_to_double(i128 t, int es, int b, int d, int rounding) {
+    if (t == 0) return 0.0;
+    const uint64_t sgn = t < 0 ? (1ull << 63) : 0ull;
+    const u128 a = t < 0 ? (u128)(-t) : (u128)t;
+    const int nb = 128 - clz128(a);
+    if (rounding == 0) {
+        const int shift = nb - 53;
+        uint64_t lu = shift > 0 ? (uint64_t)(a >> shift) : (uint64_t)(a << (-shift));
+        const int64_t ex = nb - b * (d + 1) - 1;
+        lu -= 0x0010000000000000ull;
+        lu += sgn + ((uint64_t)(ex + 1023) << 52) + ((uint64_t)(int64_t)(es - 2044) << 52);
+        return __longlong_as_double((long long)lu);
+    }
+    uint64_t m;
+    int e2;
+    if (nb <= 53) {
+        m = (uint64_t)a;
+        e2 = 0;
+    } else {
+        const int sh = nb - 53;
+        u128 q = a >> sh;
+        const u128 rem = a - (q << sh);
+        const u128 half = (u128)1 << (sh - 1);
+        if (rem > half || (rem == half && (q & 1))) q += 1;
+        m = (uint64_t)q;
+        e2 = sh;
+    }
+    const double v = ldexp((double)m, e2 + es - 2044 - b * (d + 1));
+    return sgn ? -v : v;
+}
+
+// Store one element: bins k0 .. k0+NB-1 of this pass (values vals[kk]).
+template <int NB>
+__device__ __forceinline__ void epilogue_elem(const GemmLaunch &g, int epi, int row, int col,
+                                              const int32_t (&vals)[NB], int k0) {
+    if (row >= g.M || col >= g.N) return;
+    if (epi == EPI_BINS) {
+#pragma unroll
+        for (int kk = 0; kk < NB; kk++)
+            if (k0 + kk < g.d) g.bins[((int64_t)(k0 + kk) * g.M + row) * g.N + col] = vals[kk];
+        return;
+    }
+    i128 T = 0;
+#pragma unroll
+    for (int kk = 0; kk < NB; kk++) {
+        const int k = k0 + kk;
+        if (k < g.d) T += (i128)vals[kk] << (g.b * (g.d - 1 - k));
+    }
+    i128 *tp = g.T + (int64_t)row * g.ldT + col;
+    if (!g.first) T += *tp;
+    if (g.last)
+        g.C[(int64_t)row * g.ldc + col] = t_to_double(T, g.Er[row] + g.Ec[col], g.b, g.d, g.rounding);
+    else
+        *tp = T;
+}
+
+// Block -> tile mapping.  Workgroups are dispatched round-robin over the 8 XCDs; give
+// each XCD a contiguous range of tiles, and walk that range in bands of 8 tile-rows so
+// the concurrently running tiles of one XCD share A rows and B columns in its L2.
+__device__ __forceinline__ void tile_map(int bid, int ntm, int ntn, int &tm, int &tn) {
+    const int total = ntm * ntn;
+    const int q = total / 8, rem = total % 8;
+    const int xcd = bid % 8, idx = bid / 8;
+    const int lin = (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + idx;
+    constexpr int G = 8;
+    const int band = lin / (G * ntn);
+    const int inb = lin % (G * ntn);
+    const int gh = min(G, ntm - band * G);
+    tm = band * G + inb % gh;
+    tn = inb / gh;
+}
+
+
+template <int N>
+__device__ __forceinline__ void s_waitcnt_vm() {
+    // gfx9 encoding: vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt_hi[15:14]
+    constexpr int imm = (N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8);
+    __builtin_amdgcn_s_waitcnt(imm);
+}
+
+// ------------------------------------------------------------------ fused slice GEMM
+// Workgroup: 4 waves as 2 x 2, wave tile (32 WM) x (32 WN), workgroup tile
+// (64 WM) x (64 WN).  Per contraction block (32) every slice block of the tile is
+// copied HBM -> LDS with global_load_lds_dwordx4 (one 1 KiB block per wave
+// instruction) into one of two stages.  One barrier per block: after it, each wave
+// reads its d A-fragments and d*WN B-fragments of the current stage, then issues the
+// copies of the next block into the other stage (free: every wave has passed the
+// barrier, so all reads of that stage from the previous block have completed), then
+// runs the d(d+1)/2 * WM * WN MFMAs of the triangle into its d bin accumulators while
+// the copies are in flight.
+template <int D, int WM, int WN>
+struct FusedCfg {
+    static constexpr int BA = 2 * WM, BB = 2 * WN;
+    static constexpr int NBLK = D * (BA + BB);
+    static constexpr int STAGE = NBLK * kBlkBytes;
+    static constexpr int NLD = (NBLK + 3) / 4;  // load slots per wave per stage
+    static constexpr int NMIN = NBLK / 4;       // loads every wave issues per stage
+};
+
+// Epilogue of one wave: bins leave the accumulators four elements at a time through a
+// per-wave LDS scratch (each lane reads back only its own values), and a rolled loop
+// turns each element's bins into T and FP64 C, keeping register pressure flat.
+template <int NB, int WM, int WN>
+__device__ __forceinline__ void wave_epilogue(const GemmLaunch &g, int epi, int k0, v16i (&acc)[NB][WM][WN],
+                                              int32_t *scratch, int lane, int rbase, int cbase) {
+#pragma unroll
+    for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+        for (int ni = 0; ni < WN; ni++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+#pragma unroll
+                for (int k = 0; k < NB; k++)
+#pragma unroll
+                    for (int e = 0; e < 4; e++) scratch[(k * 4 + e) * 64 + lane] = acc[k][mi][ni][4 * q + e];
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll 1
+                for (int e = 0; e < 4; e++) {
+                    const int row = rbase + mi * 32 + q * 8 + (lane >> 5) * 4 + e;
+                    const int col = cbase + ni * 32 + (lane & 31);
+                    int32_t vals[NB];
+#pragma unroll
+                    for (int k = 0; k < NB; k++) vals[k] = scratch[(k * 4 + e) * 64 + lane];
+                    epilogue_elem<NB>(g, epi, row, col, vals, k0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+}
+
+template <int D, int WM, int WN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_gemm_fused(GemmLaunch g, int epi) {
+    using F = FusedCfg<D, WM, WN>;
+    constexpr int BA = F::BA, BB = F::BB, NBLK = F::NBLK, NLD = F::NLD, NMIN = F::NMIN;
+    __shared__ __attribute__((aligned(1024))) int8_t smem[2 * F::STAGE];
+
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = w & 1, wn = w >> 1;
+    const int ntm = (g.M + 64 * WM - 1) / (64 * WM), ntn = (g.N + 64 * WN - 1) / (64 * WN);
+    int tm, tn;
+    tile_map(blockIdx.x, ntm, ntn, tm, tn);
+    const int64_t rbA0 = (int64_t)tm * BA, rbB0 = (int64_t)tn * BB;
+    const int64_t planeA = g.RbA * g.Kb * kBlkBytes, planeB = g.RbB * g.Kb * kBlkBytes;
+
+    const int8_t *src[NLD];
+    int ldsoff[NLD];
+#pragma unroll
+    for (int it = 0; it < NLD; it++) {
+        const int blk = w + 4 * it;
+        const int bc = blk < NBLK ? blk : NBLK - 1;
+        if (bc < D * BA) {
+            const int j = bc / BA, mb = bc % BA;
+            src[it] = g.SA + j * planeA + (rbA0 + mb) * g.Kb * kBlkBytes;
+        } else {
+            const int bb = bc - D * BA;
+            const int j = bb / BB, nb = bb % BB;
+            src[it] = g.SB + j * planeB + (rbB0 + nb) * g.Kb * kBlkBytes;
+        }
+        src[it] += lane * 16;
+        ldsoff[it] = bc * kBlkBytes;
+    }
+    auto issue = [&](int stage, int64_t kb) {
+#pragma unroll
+        for (int it = 0; it < NLD; it++) {
+            if (it < NMIN || w + 4 * it < NBLK)
+                __builtin_amdgcn_global_load_lds((const void *)(src[it] + kb * kBlkBytes),
+                                                 (lds_void *)(smem + stage * F::STAGE + ldsoff[it]), 16, 0, 0);
+        }
+    };
+
+    v16i acc[D][WM][WN];
+#pragma unroll
+    for (int k = 0; k < D; k++)
+#pragma unroll
+        for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++) acc[k][mi][ni] = v16i{};
+
+    const int64_t kb0 = g.kb0, kb1 = g.kb1;
+    if (kb0 < kb1) issue(0, kb0);
+    for (int64_t kb = kb0; kb < kb1; kb++) {
+        const int s = (int)((kb - kb0) & 1);
+        s_waitcnt_vm<0>();
+        __syncthreads();
+        const int8_t *st = smem + s * F::STAGE;
+        v4i af[D][WM], bf[D][WN];
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+#pragma unroll
+            for (int mi = 0; mi < WM; mi++)
+                af[j][mi] = *reinterpret_cast<const v4i *>(st + (j * BA + wm * WM + mi) * kBlkBytes + lane * 16);
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++)
+                bf[j][ni] = *reinterpret_cast<const v4i *>(st + (D * BA + j * BB + wn * WN + ni) * kBlkBytes + lane * 16);
+        }
+        if (kb + 1 < kb1) issue(s ^ 1, kb + 1);
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; i + j < D; j++)
+#pragma unroll
+                for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+                    for (int ni = 0; ni < WN; ni++)
+                        acc[i + j][mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i][mi], bf[j][ni],
+                                                                                   acc[i + j][mi][ni], 0, 0, 0);
+    }
+    __syncthreads();
+
+    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * (D * 256);
+    wave_epilogue<D, WM, WN>(g, epi, 0, acc, scratch, lane, tm * 64 * WM + wm * 32 * WM,
+                             tn * 64 * WN + wn * 32 * WN);
+}
+
+bool fused_supported(int d) { return d >= 1 && d <= 16; }
+int fused_wn(int d) { return d <= 4 ? 2 : (d <= 9 ? 2 : 1); }
+static int fused_wm(int d) { return d <= 4 ? 2 : 1; }
+
+template <int D, int WM, int WN>
+static hipError_t launch_fused_t(const GemmLaunch &g, int epi, hipStream_t st) {
+    const int ntm = (g.M + 64 * WM - 1) / (64 * WM), ntn = (g.N + 64 * WN - 1) / (64 * WN);
+    const int64_t tiles = (int64_t)ntm * ntn;
+    if (tiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_gemm_fused<D, WM, WN>), dim3((unsigned)tiles), dim3(256), 0, st, g, epi);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_fused(const GemmLaunch &g, int epi, hipStream_t st) {
+    switch (g.d) {
+        case 1: return launch_fused_t<1, 2, 2>(g, epi, st);
+        case 2: return launch_fused_t<2, 2, 2>(g, epi, st);
+        case 3: return launch_fused_t<3, 2, 2>(g, epi, st);
+        case 4: return launch_fused_t<4, 2, 2>(g, epi, st);
+        case 5: return launch_fused_t<5, 1, 2>(g, epi, st);
+        case 6: return launch_fused_t<6, 1, 2>(g, epi, st);
+        case 7: return launch_fused_t<7, 1, 2>(g, epi, st);
+        case 8: return launch_fused_t<8, 1, 2>(g, epi, st);
+        case 9: return launch_fused_t<9, 1, 2>(g, epi, st);
+        case 10: return launch_fused_t<10, 1, 1>(g, epi, st);
+        case 11: return launch_fused_t<11, 1, 1>(g, epi, st);
+        case 12: return launch_fused_t<12, 1, 1>(g, epi, st);
+        case 13: return launch_fused_t<13, 1, 1>(g, epi, st);
+        case 14: return launch_fused_t<14, 1, 1>(g, epi, st);
+        case 15: return launch_fused_t<15, 1, 1>(g, epi, st);
+        case 16: return launch_fused_t<16, 1, 1>(g, epi, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// ------------------------------------------------------------------ bin-group GEMM
+// Any d <= 32: one pass computes bins [k0, k0 + 8) from the slices 0 .. nsl-1,
+// nsl = min(d, k0 + 8).  Wave tile 32 x 32, workgroup 64 x 64, one LDS stage holding
+// nsl A blocks and nsl B blocks per wave row/column.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_gemm_group(GemmLaunch g, int epi) {
+    constexpr int NB = kGroupBins;
+    constexpr int BA = 2, BB = 2;
+    __shared__ __attribute__((aligned(1024))) int8_t smem[kMaxSlices * (BA + BB) * kBlkBytes];
+
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = w & 1, wn = w >> 1;
+    const int ntm = (g.M + 63) / 64, ntn = (g.N + 63) / 64;
+    int tm, tn;
+    tile_map(blockIdx.x, ntm, ntn, tm, tn);
+    const int k0 = g.k0;
+    const int nsl = min(g.d, k0 + NB);
+    const int nblk = nsl * (BA + BB);
+    const int64_t planeA = g.RbA * g.Kb * kBlkBytes, planeB = g.RbB * g.Kb * kBlkBytes;
+    const int64_t rbA0 = (int64_t)tm * BA, rbB0 = (int64_t)tn * BB;
+
+    v16i acc[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) acc[k] = v16i{};
+
+    for (int64_t kb = g.kb0; kb < g.kb1; kb++) {
+        for (int blk = w; blk < nblk; blk += 4) {
+            const int8_t *s;
+            if (blk < nsl * BA) {
+                const int j = blk / BA, mb = blk % BA;
+                s = g.SA + j * planeA + ((rbA0 + mb) * g.Kb + kb) * kBlkBytes;
+            } else {
+                const int bb = blk - nsl * BA;
+                const int j = bb / BB, nb = bb % BB;
+                s = g.SB + j * planeB + ((rbB0 + nb) * g.Kb + kb) * kBlkBytes;
+            }
+            __builtin_amdgcn_global_load_lds((const void *)(s + lane * 16),
+                                             (lds_void *)(smem + blk * kBlkBytes), 16, 0, 0);
+        }
+        s_waitcnt_vm<0>();
+        __syncthreads();
+        for (int i = 0; i < nsl; i++) {
+            const v4i a = *reinterpret_cast<const v4i *>(smem + (i * BA + wm) * kBlkBytes + lane * 16);
+#pragma unroll
+            for (int kk = 0; kk < NB; kk++) {
+                const int j = k0 + kk - i;
+                if (j >= 0 && j < nsl) {
+                    const v4i bv = *reinterpret_cast<const v4i *>(
+                        smem + (nsl * BA + j * BB + wn) * kBlkBytes + lane * 16);
+                    acc[kk] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bv, acc[kk], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    v16i acc3[NB][1][1];
+#pragma unroll
+    for (int k = 0; k < NB; k++) acc3[k][0][0] = acc[k];
+    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * (NB * 256);
+    wave_epilogue<NB, 1, 1>(g, epi, k0, acc3, scratch, lane, tm * 64 + wm * 32, tn * 64 + wn * 32);
+}
+
+hipError_t launch_gemm_group(const GemmLaunch &g, int epi, hipStream_t st) {
+    const int ntm = (g.M + 63) / 64, ntn = (g.N + 63) / 64;
+    const int64_t tiles = (int64_t)ntm * ntn;
+    if (tiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gemm_group, dim3((unsigned)tiles), dim3(256), 0, st, g, epi);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ wire format
+// Paired int64 {L, H} of the reference NPU output (ozaki_base.cc:143-340): L sums the
+// n0 least significant bins, H the next n1 with bits 51..62 replaced by E_r + E_c and
+// bit 63 kept as the sign of the sum; every shift-add wraps modulo 2^64 like acc64.
+__global__ void k_pack_paired(const int32_t *__restrict__ bins, int64_t M, int64_t N, int b, int d,
+                              int n0, const int32_t *__restrict__ Er, const int32_t *__restrict__ Ec,
+                              int64_t *__restrict__ out) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= M * N) return;
+    const int64_t r = idx / N, c = idx % N;
+    const int64_t plane = M * N;
+    const int n1 = d - n0;
+    uint64_t L = 0, H = 0;
+    for (int i = 0; i < n0; i++) L += (uint64_t)(int64_t)bins[(int64_t)(d - 1 - i) * plane + idx] << (b * i);
+    for (int i = 0; i < n1; i++) H += (uint64_t)(int64_t)bins[(int64_t)(d - 1 - n0 - i) * plane + idx] << (b * i);
+    const uint32_t h32 = (uint32_t)(H >> 32);
+    uint32_t emb = ((h32 << 13) >> 13) + ((h32 >> 31) << 31);
+    emb += (uint32_t)(int32_t)(int16_t)(Er[r] + Ec[c]) << 19;
+    out[idx * 2] = (int64_t)L;
+    out[idx * 2 + 1] = (int64_t)(((uint64_t)emb << 32) | (uint32_t)H);
+}
+
+hipError_t launch_pack_paired(const int32_t *bins, int64_t M, int64_t N, int b, int d, int n0,
+                              const int32_t *Er, const int32_t *Ec, int64_t *out, hipStream_t st) {
+    const int64_t n = M * N;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_paired, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, bins, M, N,
+                       b, d, n0, Er, Ec, out);
+    return hipGetLastError();
+}
+
+}  // namespace oz

@@ -1,0 +1,234 @@
+"""Parity of the HIP path (through the C ABI) with the oracle on the same inputs.
+
+Bar: bit-exact exponents, slices, int32 bins, paired-int64 wire format and FP64 C
+(truncating conversion, the reference rule); the nearest-even conversion is bit-exact
+against the oracle's nearest-even closed form and within 1 ulp of the truncated one.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import ozaie_amd as oz
+from conftest import gen_inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).to("cuda")
+
+
+def edge_matrix(rng, m, n, phi=2.0):
+    X = gen_inputs(rng, m, n, phi)
+    X[0, :6] = [0.0, -0.0, 1.0, 5e-324, 2.2250738585072014e-308, -2.2250738585072009e-308]
+    X[1, :] = 0.0
+    X[2, 3] = 1e-300
+    X[3, :] = -1.0
+    X[5, 1] = np.nextafter(1.0, 2.0)
+    return X
+
+
+def bits(x):
+    return np.ascontiguousarray(x).view(np.int64)
+
+
+SPLIT_CASES = [(3, 29), (3, 26), (3, 25), (4, 18), (7, 8), (7, 3), (7, 7), (7, 9), (7, 13), (1, 32)]
+
+
+@pytest.mark.parametrize("b,d", SPLIT_CASES)
+def test_split_bit_exact(gpu, b, d):
+    rng = np.random.default_rng(b * 1000 + d)
+    A = edge_matrix(rng, 70, 300)
+    E, S = oz.debug_split(dev(A), b, d, 1)
+    Eo = oracle.max_exp(A, 1)
+    assert np.array_equal(E, Eo.astype(np.int32))
+    assert np.array_equal(S, oracle.split(A, Eo, b, d, 1))
+    B = edge_matrix(rng, 260, 90).T.copy().T   # K x N, row-major
+    B = np.ascontiguousarray(B)
+    E, S = oz.debug_split(dev(B), b, d, 0)
+    Eo = oracle.max_exp(B, 0)
+    assert np.array_equal(E, Eo.astype(np.int32))
+    assert np.array_equal(S, oracle.split(B, Eo, b, d, 0))
+
+
+BIN_CASES = [(7, 8, 0), (7, 3, 0), (7, 7, 0), (7, 9, 0), (7, 1, 0), (7, 4, 0), (7, 10, 0), (6, 16, 0),
+             (3, 29, 1), (3, 26, 1), (7, 8, 1), (4, 18, 1)]
+
+
+@pytest.mark.parametrize("b,d,kernel", BIN_CASES)
+def test_bins_bit_exact(gpu, b, d, kernel):
+    rng = np.random.default_rng(d * 31 + kernel)
+    M, N, K = 100, 150, 300
+    A = gen_inputs(rng, M, K, 1.0)
+    B = gen_inputs(rng, K, N, 1.0)
+    got = oz.debug_bins(dev(A), dev(B), b, d, kernel)
+    Er, Ec = oracle.max_exp(A, 1), oracle.max_exp(B, 0)
+    ref = oracle.bins(oracle.split(A, Er, b, d, 1), oracle.split(B, Ec, b, d, 0))
+    assert np.abs(ref).max() < 2 ** 31
+    assert np.array_equal(got.astype(np.int64), ref)
+
+
+C_CASES = [(7, 8), (7, 3), (7, 7), (7, 9), (3, 29), (3, 26), (4, 18), (7, 12), (5, 19)]
+
+
+@pytest.mark.parametrize("b,d", C_CASES)
+def test_C_bit_exact_truncating(gpu, b, d):
+    import torch
+    rng = np.random.default_rng(d * 7 + b)
+    M, N, K = 130, 70, 257
+    A = gen_inputs(rng, M, K, 1.0)
+    B = gen_inputs(rng, K, N, 1.0)
+    plan = oz.Plan(M, N, K, b, d)
+    C = plan(dev(A), dev(B))
+    torch.cuda.synchronize()
+    ref, ov = oracle.ozaki_dgemm(A, B, b, d)
+    assert ov == 0
+    assert np.array_equal(bits(C.cpu().numpy()), bits(ref))
+
+
+def test_C_nearest_even(gpu):
+    import torch
+    rng = np.random.default_rng(3)
+    M, N, K = 64, 96, 200
+    A = gen_inputs(rng, M, K, 1.0)
+    B = gen_inputs(rng, K, N, 1.0)
+    for b, d in [(7, 8), (3, 29)]:
+        C = oz.Plan(M, N, K, b, d, rounding=oz.ROUND_NEAREST)(dev(A), dev(B))
+        torch.cuda.synchronize()
+        ref, _ = oracle.ozaki_dgemm(A, B, b, d, 0, 1)
+        assert np.array_equal(bits(C.cpu().numpy()), bits(ref))
+        tr, _ = oracle.ozaki_dgemm(A, B, b, d, 0, 0)
+        assert np.max(np.abs(bits(C.cpu().numpy()) - bits(tr))) <= 1
+
+
+def test_example_ones_via_reference_api(gpu):
+    """example/oztest.cpp of the reference: accurate mode, 256 x 512 x 128, ones -> 128."""
+    M, N, K = 256, 512, 128
+    assert oz.ozAIE_setup(M, N, K, oz.Mode.accurate) == oz.OK
+    assert oz.plan_info()["slices"] == 29
+    A = np.ones((M, K))
+    B = np.ones((K, N))
+    C = np.zeros((M, N))
+    assert oz.ozAIE_DGEMM(M, N, K, A, B, C) == oz.OK
+    assert np.all(C == 128.0)
+
+
+def test_config1_reference_api_vs_oracle_and_mpfr(gpu):
+    """256 x 512 x 128, phi = 0.5, accurate mode, host pointers; C bit-exact against the
+    oracle; max relative error against MPFR-106 printed and bounded."""
+    rng = np.random.default_rng(12445)
+    M, N, K = 256, 512, 128
+    A = gen_inputs(rng, M, K, 0.5)
+    B = gen_inputs(rng, K, N, 0.5)
+    assert oz.ozAIE_setup(M, N, K, oz.Mode.accurate) == oz.OK
+    C = np.zeros((M, N))
+    assert oz.ozAIE_DGEMM(M, N, K, A, B, C) == oz.OK
+    ref, _ = oracle.ozaki_dgemm(A, B, 3, 29)
+    assert np.array_equal(bits(C), bits(ref))
+    assert oz.ozAIE_DGEMM(M, N + 1, K, A, B, C) == oz.ERR_MISMATCH
+    if oracle.have_mpfr():
+        rows = np.repeat(np.arange(0, M, 8), N // 16)
+        cols = np.tile(np.arange(0, N, 16), M // 8)
+        mp = oracle.mpfr_sample(A, B, rows, cols)
+        rel = np.abs(C[rows, cols] - mp) / np.abs(mp)
+        print("config1 accurate max rel err vs MPFR:", rel.max())
+        assert rel.max() < 1e-10
+
+
+def test_reference_api_device_pointers(gpu):
+    import torch
+    rng = np.random.default_rng(8)
+    M, N, K = 96, 64, 512
+    A = gen_inputs(rng, M, K, 1.0)
+    B = gen_inputs(rng, K, N, 1.0)
+    assert oz.ozAIE_setup(M, N, K, oz.Mode.fast) == oz.OK
+    dA, dB = dev(A), dev(B)
+    dC = torch.zeros((M, N), dtype=torch.float64, device="cuda")
+    assert oz.ozAIE_DGEMM(M, N, K, dA, dB, dC) == oz.OK
+    ref, _ = oracle.ozaki_dgemm(A, B, 7, 8)
+    assert np.array_equal(bits(dC.cpu().numpy()), bits(ref))
+
+
+def test_multi_pass_kchunk_and_groups(gpu):
+    """Forcing several contraction chunks (and bin groups) gives the single-pass result."""
+    import torch
+    rng = np.random.default_rng(21)
+    M, N, K = 70, 90, 1000
+    A = gen_inputs(rng, M, K, 1.0)
+    B = gen_inputs(rng, K, N, 1.0)
+    for b, d, kernel in [(7, 8, 0), (3, 29, 1), (7, 8, 1)]:
+        oz.set_option(oz.OPT_KCHUNK, 128)
+        oz.set_option(oz.OPT_KERNEL, kernel)
+        try:
+            assert oz.ozAIE_setup_slices(M, N, K, b, d) == oz.OK
+            info = oz.plan_info()
+            assert info["passes"] > 1
+            C = torch.zeros((M, N), dtype=torch.float64, device="cuda")
+            assert oz.ozAIE_DGEMM(M, N, K, dev(A), dev(B), C) == oz.OK
+        finally:
+            oz.set_option(oz.OPT_KCHUNK, 0)
+            oz.set_option(oz.OPT_KERNEL, 0)
+        ref, _ = oracle.ozaki_dgemm(A, B, b, d)
+        assert np.array_equal(bits(C.cpu().numpy()), bits(ref))
+
+
+def test_long_K_needs_chunks(gpu):
+    """d = 9 bins overflow int32 beyond K = 14784: the planner chunks K automatically."""
+    import torch
+    rng = np.random.default_rng(22)
+    M, N, K = 40, 36, 17000
+    A = gen_inputs(rng, M, K, 2.0)
+    B = gen_inputs(rng, K, N, 2.0)
+    p = oz.Plan(M, N, K, 7, 9)
+    assert p.info()["passes"] == 2
+    C = p(dev(A), dev(B))
+    torch.cuda.synchronize()
+    ref, ov = oracle.ozaki_dgemm(A, B, 7, 9, kchunk=14784)
+    assert ov == 0
+    assert np.array_equal(bits(C.cpu().numpy()), bits(ref))
+
+
+def test_transposed_operands(gpu):
+    import torch
+    rng = np.random.default_rng(23)
+    M, N, K = 50, 60, 200
+    A = gen_inputs(rng, M, K, 1.0)
+    B = gen_inputs(rng, K, N, 1.0)
+    ref, _ = oracle.ozaki_dgemm(A, B, 7, 8)
+    p = oz.Plan(M, N, K, 7, 8)
+    for ta in (False, True):
+        for tb in (False, True):
+            a = dev(A.T.copy()) if ta else dev(A)
+            b = dev(B.T.copy()) if tb else dev(B)
+            C = p(a, b, transA=ta, transB=tb)
+            torch.cuda.synchronize()
+            assert np.array_equal(bits(C.cpu().numpy()), bits(ref)), (ta, tb)
+
+
+def test_paired_wire_format(gpu):
+    rng = np.random.default_rng(24)
+    for K, b, d in [(128, 3, 29), (256, 7, 8), (2048, 3, 26)]:
+        M, N = 40, 70
+        A = gen_inputs(rng, M, K, 1.0)
+        B = gen_inputs(rng, K, N, 1.0)
+        got = oz.paired_output(dev(A), dev(B), b, d)
+        Er, Ec = oracle.max_exp(A, 1), oracle.max_exp(B, 0)
+        bins = oracle.bins(oracle.split(A, Er, b, d, 1), oracle.split(B, Ec, b, d, 0))
+        ref = oracle.pack_paired(bins, K, b, d, Er, Ec)
+        assert np.array_equal(got, ref)
+        # and the reference host conversion of it gives the library's C
+        C = oracle.convert_paired(got, K, b, d)
+        C2, _ = oracle.ozaki_dgemm(A, B, b, d)
+        assert np.array_equal(bits(C), bits(C2))
+
+
+def test_small_integers_exact_gpu(gpu):
+    import torch
+    rng = np.random.default_rng(25)
+    A = rng.integers(-127, 128, size=(33, 70)).astype(np.float64)
+    B = rng.integers(-127, 128, size=(70, 45)).astype(np.float64)
+    for b, d in [(7, 3), (7, 8), (3, 29)]:
+        C = oz.Plan(33, 45, 70, b, d)(dev(A), dev(B))
+        torch.cuda.synchronize()
+        assert np.array_equal(C.cpu().numpy(), A @ B)

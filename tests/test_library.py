@@ -1,0 +1,113 @@
+"""Host-side checks of libozaie.so that need no GPU: it loads, exports every function
+include/ozaie.h declares, validates arguments and reports the reference's status codes
+(include/oz_aie.h:187-191 of the reference) before touching the device."""
+import os
+import re
+import subprocess
+
+import pytest
+
+import ozaie_amd as oz
+from conftest import ROOT, has_gpu
+
+
+def declared_functions():
+    names = []
+    for hdr in ("ozaie.h",):
+        text = open(os.path.join(ROOT, "include", hdr)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_ ]*?\**\s*\**\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(",
+                             text, flags=re.M):
+            names.append(m.group(1))
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = oz.lib()
+    names = declared_functions()
+    assert len(names) >= 16
+    out = subprocess.check_output(["nm", "-D", "--defined-only", oz.LIB_PATH]).decode()
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    for n in names:
+        assert hasattr(lib, n)
+    assert sorted(oz.EXPORTS) == names
+
+
+def test_version_and_options():
+    assert oz.version() >= 100
+    assert oz.get_option(oz.OPT_ROUNDING) in (0, 1)
+    assert oz.lib().ozaie_set_option(oz.OPT_ROUNDING, 5) == oz.ERR_ARG
+    assert oz.lib().ozaie_set_option(999, 0) == oz.ERR_ARG
+    oz.set_option(oz.OPT_KCHUNK, 4096)
+    assert oz.get_option(oz.OPT_KCHUNK) == 4096
+    oz.set_option(oz.OPT_KCHUNK, 0)
+
+
+def test_bad_dims_rejected_before_device():
+    assert oz.ozAIE_setup(0, 10, 10, oz.Mode.fast) == oz.ERR_DIMS
+    assert oz.ozAIE_setup(10, -1, 10, oz.Mode.fast) == oz.ERR_DIMS
+
+
+def test_strict_mode_reproduces_reference_validation():
+    oz.set_option(oz.OPT_STRICT, 1)
+    try:
+        # the repo example's own shape is rejected by the reference (non-square)
+        assert oz.ozAIE_setup(256, 512, 128, oz.Mode.accurate) == oz.ERR_DIMS
+        assert oz.ozAIE_setup(64, 64, 64, oz.Mode.accurate) == oz.ERR_DIMS
+        assert oz.ozAIE_setup(4096, 4096, 4096, oz.Mode.fast) == oz.ERR_DIMS
+        assert oz.ozAIE_setup(384, 384, 384, oz.Mode.fast) == oz.ERR_DIMS
+    finally:
+        oz.set_option(oz.OPT_STRICT, 0)
+
+
+def test_dgemm_before_setup():
+    import numpy as np
+    oz.release()
+    A = np.zeros((4, 4))
+    assert oz.ozAIE_DGEMM(4, 4, 4, A, A, A) == oz.ERR_NOSETUP
+
+
+@pytest.mark.skipif(has_gpu(), reason="checks the no-device path")
+def test_no_device_reported():
+    assert oz.ozAIE_setup(128, 128, 128, oz.Mode.fast) == oz.ERR_NODEVICE
+    import numpy as np
+    X = np.zeros((4, 4))
+    E = np.zeros(4, dtype=np.int32)
+    S = np.zeros((8, 4, 4), dtype=np.int8)
+    rc = oz.lib().ozaie_debug_split(oz._ptr(X), 4, 4, 4, 1, 7, 8, oz._ptr(E), oz._ptr(S))
+    assert rc == oz.ERR_NODEVICE
+
+
+def test_mode_table():
+    assert oz.mode_slices(oz.Mode.accurate, 128) == (3, 29)
+    assert oz.mode_slices(oz.Mode.accurate, 2048) == (3, 26)
+    assert oz.mode_slices(oz.Mode.accurate, 8192) == (3, 25)
+    assert oz.mode_slices(oz.Mode.accurate, 1 << 15) == (4, 18)
+    assert oz.mode_slices(oz.Mode.fast, 4096) == (7, 8)
+    assert oz.mode_slices(oz.Mode.fast3, 4096) == (7, 3)
+    assert oz.mode_slices(oz.Mode.accurate7, 4096) == (7, 7)
+
+
+def test_cxx_shim_compiles_against_the_abi(tmp_path):
+    """include/oz_aie.h lets a reference consumer (namespace oz_aie, Mode::accurate,
+    ozAIE_setup / ozAIE_DGEMM) compile and link unchanged."""
+    exe = tmp_path / "ozex"
+    src = os.path.join(ROOT, "examples", "oz_example.cpp")
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"), src,
+                           "-o", str(exe), "-L" + os.path.dirname(oz.LIB_PATH), "-lozaie",
+                           "-Wl,-rpath," + os.path.dirname(oz.LIB_PATH)])
+    assert exe.exists()
+
+
+@pytest.mark.skipif(not os.path.exists("/root/reference/example/oztest.cpp"),
+                    reason="reference example not mounted")
+def test_reference_example_compiles_unchanged(tmp_path):
+    """The reference's own consumer, read in place, builds against include/oz_aie.h."""
+    exe = tmp_path / "oztest"
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"),
+                           "/root/reference/example/oztest.cpp", "-o", str(exe),
+                           "-L" + os.path.dirname(oz.LIB_PATH), "-lozaie",
+                           "-Wl,-rpath," + os.path.dirname(oz.LIB_PATH)])
+    assert exe.exists()
