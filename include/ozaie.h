@@ -49,7 +49,9 @@ extern "C" {
 #define OZAIE_OPT_ROUNDING 1    /* OZAIE_ROUND_TRUNCATE (default) or OZAIE_ROUND_NEAREST   */
 #define OZAIE_OPT_STRICT 2      /* 1: ozAIE_setup rejects what the reference rejects        */
 #define OZAIE_OPT_KCHUNK 3      /* cap on contraction indices per pass (0 = automatic)      */
-#define OZAIE_OPT_KERNEL 4      /* 0 auto, 1 force the bin-group kernel                     */
+#define OZAIE_OPT_KERNEL 4      /* 0 auto, 1 bin-group kernel, 2 fused with LDS copies,
+                                   3 fused with direct register loads, 4 fused pipelined  */
+#define OZAIE_OPT_PHASE_TIMING 5 /* 1: plans record HIP events around the split and GEMM    */
 #define OZAIE_ROUND_TRUNCATE 0
 #define OZAIE_ROUND_NEAREST 1
 
@@ -96,6 +98,12 @@ int32_t ozaie_plan_query(ozaie_plan plan, int32_t *out5);
 int32_t ozaie_dgemm(ozaie_plan plan, int transA, int transB, const double *A, int64_t lda,
                     const double *B, int64_t ldb, double *C, int64_t ldc, void *stream);
 
+/* Milliseconds of the last ozaie_dgemm on `plan` (needs OZAIE_OPT_PHASE_TIMING = 1 when
+ * the plan was created): out[0] exponent max + split, out[1] slice GEMM passes (the
+ * fused GEMM kernel, or all bin-group passes), out[2] whole call.  Synchronizes on the
+ * events of that call.                                                                 */
+int32_t ozaie_plan_phase_times(ozaie_plan plan, float *out3);
+
 /* ---- inspection (reference full_dump equivalent, whole_array_iron_ozaki.py:485-558)  */
 
 /* Exponents and slices of one operand, as computed on the GPU, in plain layout.
@@ -105,8 +113,8 @@ int32_t ozaie_debug_split(const double *X, int64_t rows, int64_t K, int64_t sr, 
                           int bits, int slices, int32_t *E_out, int8_t *S_out);
 
 /* The d int32 bins of C = A * B (row-major device A, B) as computed by the slice GEMM
- * kernels (bins_out: host int32[d][M][N]).  K must fit one pass.  kernel: 0 fused,
- * 1 bin-group.                                                                        */
+ * kernels (bins_out: host int32[d][M][N]).  K must fit one pass.  kernel: as
+ * OZAIE_OPT_KERNEL (0 auto, 1 bin-group, 2/3/4 the fused families).                   */
 int32_t ozaie_debug_bins(int64_t M, int64_t N, int64_t K, const double *A, const double *B,
                          int bits, int slices, int kernel, int32_t *bins_out);
 
@@ -114,6 +122,14 @@ int32_t ozaie_debug_bins(int64_t M, int64_t N, int64_t K, const double *A, const
  * returns (ozaki_base.cc:143-340), computed from GPU bins (host int64[M][N][2]).      */
 int32_t ozaie_paired_output(int64_t M, int64_t N, int64_t K, const double *A,
                             const double *B, int bits, int slices, int64_t *out);
+
+/* ---- accuracy measurement (host, OpenMP)
+ * Correctly rounded exact values of selected entries of A * B (row-major host arrays):
+ * out[i] = round(sum_k A[rows[i]][k] * B[k][cols[i]]), every product split error-free
+ * with an FMA and the terms summed exactly.  Used to score results against the
+ * exact product (the reference scores against MPFR-106, common.h:164-232).           */
+int32_t ozaie_exact_entries(const double *A, int64_t lda, const double *B, int64_t ldb, int64_t K,
+                            const int64_t *rows, const int64_t *cols, int64_t count, double *out);
 
 /* Last HIP error string (for diagnostics). */
 const char *ozaie_last_error(void);

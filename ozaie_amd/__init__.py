@@ -35,6 +35,7 @@ OPT_ROUNDING = 1
 OPT_STRICT = 2
 OPT_KCHUNK = 3
 OPT_KERNEL = 4
+OPT_PHASE_TIMING = 5
 ROUND_TRUNCATE = 0
 ROUND_NEAREST = 1
 
@@ -52,7 +53,8 @@ EXPORTS = [
     "ozAIE_setup", "ozAIE_setup_slices", "ozAIE_DGEMM", "ozaie_set_option", "ozaie_get_option",
     "ozaie_plan_info", "ozaie_release", "ozaie_plan_create", "ozaie_plan_destroy",
     "ozaie_plan_query", "ozaie_dgemm", "ozaie_debug_split", "ozaie_debug_bins",
-    "ozaie_paired_output", "ozaie_last_error", "ozaie_version",
+    "ozaie_paired_output", "ozaie_last_error", "ozaie_version", "ozaie_plan_phase_times",
+    "ozaie_exact_entries",
 ]
 
 
@@ -118,6 +120,10 @@ def _declare(L):
     L.ozaie_debug_bins.restype = i32
     L.ozaie_paired_output.argtypes = [i64, i64, i64, P, P, c_int, c_int, P]
     L.ozaie_paired_output.restype = i32
+    L.ozaie_plan_phase_times.argtypes = [P, P]
+    L.ozaie_plan_phase_times.restype = i32
+    L.ozaie_exact_entries.argtypes = [P, i64, P, i64, i64, P, P, i64, P]
+    L.ozaie_exact_entries.restype = i32
     L.ozaie_last_error.argtypes = []
     L.ozaie_last_error.restype = ctypes.c_char_p
     L.ozaie_version.argtypes = []
@@ -236,6 +242,13 @@ class Plan:
                "ozaie_dgemm")
         return C
 
+    def phase_times(self):
+        """(split_ms, gemm_ms, total_ms) of the last call; the plan must have been created
+        with OPT_PHASE_TIMING set."""
+        out = np.zeros(3, dtype=np.float32)
+        _check(lib().ozaie_plan_phase_times(self._h, _ptr(out)), "ozaie_plan_phase_times")
+        return float(out[0]), float(out[1]), float(out[2])
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             lib().ozaie_plan_destroy(self._h)
@@ -291,6 +304,18 @@ def paired_output(A, B, bits, slices):
     out = np.zeros((M, N, 2), dtype=np.int64)
     _check(lib().ozaie_paired_output(M, N, K, _ptr(A), _ptr(B), bits, slices, _ptr(out)),
            "ozaie_paired_output")
+    return out
+
+
+def exact_entries(A, B, rows, cols):
+    """Correctly rounded exact (A @ B)[rows[i], cols[i]] for host float64 arrays."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    cols = np.ascontiguousarray(cols, dtype=np.int64)
+    out = np.zeros(len(rows), dtype=np.float64)
+    _check(lib().ozaie_exact_entries(_ptr(A), A.shape[1], _ptr(B), B.shape[1], A.shape[1], _ptr(rows),
+                                     _ptr(cols), len(rows), _ptr(out)), "ozaie_exact_entries")
     return out
 
 

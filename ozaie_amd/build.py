@@ -8,6 +8,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libozaie.so")
 ARCH = os.environ.get("OZAIE_OFFLOAD_ARCH", "gfx950")
 SOURCES = ["oz_kernels.hip", "oz_api.cpp"]
+HOST_SOURCES = ["oz_verify.cpp"]   # plain host C++ (g++, OpenMP)
 
 
 def hipcc():
@@ -43,10 +44,20 @@ def build(verbose=False, jobs=2):
             errs.append(f"{src}:\n{text}")
         elif verbose and text.strip():
             print(text)
+    for src in HOST_SOURCES:
+        obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
+        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off",
+               "-I" + os.path.join(HERE, "..", "include"), "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+        if r.returncode != 0:
+            errs.append(f"{src}:\n{r.stdout.decode(errors='replace')}")
+        objs.append(obj)
     if errs:
-        raise RuntimeError("hipcc failed:\n" + "\n".join(errs))
+        raise RuntimeError("build failed:\n" + "\n".join(errs))
     tmp = LIB + ".tmp"
-    link = [hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    link = [hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lgomp"]
     if verbose:
         print(" ".join(link), flush=True)
     subprocess.check_call(link)

@@ -30,6 +30,7 @@ int g_opt_rounding = OZAIE_ROUND_TRUNCATE;
 int g_opt_strict = 0;
 int g_opt_kchunk = 0;
 int g_opt_kernel = 0;
+int g_opt_timing = 0;
 
 int set_hip_error(hipError_t e, const char *where) {
     g_last_error = std::string(where) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
@@ -67,7 +68,7 @@ struct ozaie_plan_s {
     int b = 0, d = 0, rounding = 0;
     int64_t Mp = 0, Np = 0, Kp = 0, Kb = 0;
     int64_t kcb = 0;      // contraction blocks per pass
-    int nchunks = 0, ngroups = 1, kernel = 0;
+    int nchunks = 0, ngroups = 1, kernel = 0, family = 0;
     int device = 0;
     int8_t *SA = nullptr, *SB = nullptr;
     int32_t *Er = nullptr, *Ec = nullptr;
@@ -75,6 +76,9 @@ struct ozaie_plan_s {
     // staging for host pointers (reference-compatible entry point)
     double *dA = nullptr, *dB = nullptr, *dC = nullptr;
     hipStream_t stream = nullptr;
+    // optional phase timing: start, after split, after GEMM passes
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    bool timed = false;
 };
 
 namespace {
@@ -93,6 +97,8 @@ void plan_free(ozaie_plan p) {
     if (p->dB) (void)hipFree(p->dB);
     if (p->dC) (void)hipFree(p->dC);
     if (p->stream) (void)hipStreamDestroy(p->stream);
+    for (auto &e : p->ev)
+        if (e) (void)hipEventDestroy(e);
     (void)hipSetDevice(cur);
     delete p;
 }
@@ -102,7 +108,7 @@ void plan_free(ozaie_plan p) {
 int64_t max_kchunk(int b, int d) {
     const int64_t m = (1 << b) - 1;
     int64_t kc = (int64_t)INT32_MAX / ((int64_t)d * m * m);
-    return kc / kBlk * kBlk;
+    return kc / (2 * kBlk) * (2 * kBlk);
 }
 
 int plan_configure(ozaie_plan p, int64_t M, int64_t N, int64_t K, int b, int d, int rounding) {
@@ -122,13 +128,14 @@ int plan_configure(ozaie_plan p, int64_t M, int64_t N, int64_t K, int b, int d, 
     p->M = M; p->N = N; p->K = K; p->b = b; p->d = d; p->rounding = rounding;
     p->Mp = round_up(M, kRowPad);
     p->Np = round_up(N, kRowPad);
-    p->Kp = round_up(K, kBlk);
+    p->Kp = round_up(K, 2 * kBlk);   // even number of contraction blocks per pass
     p->Kb = p->Kp / kBlk;
     int64_t kc = max_kchunk(b, d);
-    if (g_opt_kchunk > 0) kc = std::min<int64_t>(kc, std::max<int64_t>(kBlk, g_opt_kchunk / kBlk * kBlk));
-    p->kcb = std::max<int64_t>(1, kc / kBlk);
+    if (g_opt_kchunk > 0) kc = std::min<int64_t>(kc, std::max<int64_t>(2 * kBlk, g_opt_kchunk / (2 * kBlk) * (2 * kBlk)));
+    p->kcb = std::max<int64_t>(2, kc / kBlk / 2 * 2);
     p->nchunks = (int)((p->Kb + p->kcb - 1) / p->kcb);
     p->kernel = (fused_supported(d) && g_opt_kernel != 1) ? 0 : 1;
+    p->family = g_opt_kernel >= 2 ? g_opt_kernel : KERNEL_AUTO;
     p->ngroups = p->kernel == 0 ? 1 : (d + kGroupBins - 1) / kGroupBins;
     if (p->Mp / kBlk > 65535 || p->Np / kBlk > 65535) {
         g_last_error = "too many rows for the split grid"; return OZAIE_ERR_DIMS;
@@ -150,6 +157,10 @@ int plan_alloc(ozaie_plan p, bool staging) {
         HIPCHK(hipMalloc(&p->dB, sizeof(double) * (size_t)p->K * p->N));
         HIPCHK(hipMalloc(&p->dC, sizeof(double) * (size_t)p->M * p->N));
         HIPCHK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    }
+    if (g_opt_timing) {
+        for (auto &e : p->ev) HIPCHK(hipEventCreate(&e));
+        p->timed = true;
     }
     return OZAIE_OK;
 }
@@ -190,6 +201,7 @@ GemmLaunch base_launch(const ozaie_plan p) {
     g.rounding = p->rounding;
     g.T = p->T; g.ldT = p->N;
     g.K_true = (int)std::min<int64_t>(p->K, INT32_MAX);
+    g.family = p->family;
     return g;
 }
 
@@ -229,8 +241,11 @@ int plan_execute(ozaie_plan p, int transA, int transB, const double *A, int64_t 
     int cur = 0;
     HIPCHK(hipGetDevice(&cur));
     if (cur != p->device) HIPCHK(hipSetDevice(p->device));
+    if (p->timed) HIPCHK(hipEventRecord(p->ev[0], st));
     int rc = run_split(p, view_A(p, transA, A, lda), view_B(p, transB, B, ldb), st);
+    if (!rc && p->timed) HIPCHK(hipEventRecord(p->ev[1], st));
     if (!rc) rc = run_passes(p, C, ldc, EPI_C, nullptr, st);
+    if (!rc && p->timed) HIPCHK(hipEventRecord(p->ev[2], st));
     if (cur != p->device) (void)hipSetDevice(cur);
     return rc;
 }
@@ -324,8 +339,9 @@ int32_t ozaie_set_option(int option, int value) {
             if (value < 0) return OZAIE_ERR_ARG;
             g_opt_kchunk = value; return OZAIE_OK;
         case OZAIE_OPT_KERNEL:
-            if (value < 0 || value > 1) return OZAIE_ERR_ARG;
+            if (value < 0 || value > 4) return OZAIE_ERR_ARG;
             g_opt_kernel = value; return OZAIE_OK;
+        case OZAIE_OPT_PHASE_TIMING: g_opt_timing = value ? 1 : 0; return OZAIE_OK;
         default: return OZAIE_ERR_ARG;
     }
 }
@@ -337,6 +353,7 @@ int32_t ozaie_get_option(int option) {
         case OZAIE_OPT_STRICT: return g_opt_strict;
         case OZAIE_OPT_KCHUNK: return g_opt_kchunk;
         case OZAIE_OPT_KERNEL: return g_opt_kernel;
+        case OZAIE_OPT_PHASE_TIMING: return g_opt_timing;
         default: return -OZAIE_ERR_ARG;
     }
 }
@@ -387,6 +404,16 @@ int32_t ozaie_dgemm(ozaie_plan plan, int transA, int transB, const double *A, in
     return plan_execute(plan, transA, transB, A, lda, B, ldb, C, ldc, (hipStream_t)stream);
 }
 
+int32_t ozaie_plan_phase_times(ozaie_plan plan, float *out3) {
+    if (!plan || !out3) return OZAIE_ERR_ARG;
+    if (!plan->timed) { g_last_error = "plan created without OZAIE_OPT_PHASE_TIMING"; return OZAIE_ERR_ARG; }
+    HIPCHK(hipEventSynchronize(plan->ev[2]));
+    HIPCHK(hipEventElapsedTime(&out3[0], plan->ev[0], plan->ev[1]));
+    HIPCHK(hipEventElapsedTime(&out3[1], plan->ev[1], plan->ev[2]));
+    HIPCHK(hipEventElapsedTime(&out3[2], plan->ev[0], plan->ev[2]));
+    return OZAIE_OK;
+}
+
 int32_t ozaie_debug_split(const double *X, int64_t rows, int64_t K, int64_t sr, int64_t sk,
                           int bits, int slices, int32_t *E_out, int8_t *S_out) {
     if (!X || !E_out || !S_out) return OZAIE_ERR_ARG;
@@ -396,7 +423,7 @@ int32_t ozaie_debug_split(const double *X, int64_t rows, int64_t K, int64_t sr, 
     std::lock_guard<std::mutex> lk(g_mu);
     int rc = ensure_device();
     if (rc) return rc;
-    const int64_t Rp = round_up(rows, kRowPad), Kp = round_up(K, kBlk);
+    const int64_t Rp = round_up(rows, kRowPad), Kp = round_up(K, 2 * kBlk);
     int32_t *E = nullptr;
     int8_t *S = nullptr;
     HIPCHK(hipMalloc(&E, sizeof(int32_t) * Rp));

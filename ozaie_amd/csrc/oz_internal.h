@@ -52,7 +52,11 @@ struct GemmLaunch {
     int32_t *bins;          // debug: [d][M][N] int32 instead of C
     int64_t *paired;        // reference wire format: [M][N][2] int64 (optional)
     int K_true;             // K of the product (paired-format group sizes)
+    int family;             // fused-kernel family (KERNEL_*), KERNEL_AUTO = per-d default
 };
+
+// Fused-kernel families (same arithmetic, different data movement).
+enum Family { KERNEL_AUTO = 0, KERNEL_LDS = 2, KERNEL_DIRECT = 3, KERNEL_PIPE = 4 };
 
 // Kernel launchers (oz_kernels.hip).  All enqueue on `st` and return hipError_t.
 hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
@@ -62,7 +66,6 @@ hipError_t launch_split(const OperandView &op, const int32_t *E, int b, int d, i
 // Fused kernel (all d bins in registers); returns hipErrorInvalidValue if d has no
 // instantiation.  `wn` picks the wave tile (1: 32x32, 2: 32x64 per wave).
 bool fused_supported(int d);
-int fused_wn(int d);
 hipError_t launch_gemm_fused(const GemmLaunch &g, int epi, hipStream_t st);
 // Bin-group kernel for any d <= kMaxSlices: bins [g.k0, g.k0 + 8).
 constexpr int kGroupBins = 8;

@@ -418,19 +418,221 @@ void k_gemm_fused(GemmLaunch g, int epi) {
                              tn * 64 * WN + wn * 32 * WN);
 }
 
+// Direct variant: no LDS in the main loop.  Each wave streams its own A and B
+// fragments HBM/L2 -> VGPR (one 1 KiB block per wave instruction, 16 bytes per lane),
+// double-buffered in registers, so there is no barrier and the compiler's per-register
+// wait counts overlap the loads of block t+1 with the MFMAs of block t.  Requires an
+// even number of contraction blocks per pass (the planner pads K to 64).
+template <int D, int WM, int WN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_gemm_direct(GemmLaunch g, int epi) {
+    __shared__ __attribute__((aligned(1024))) int8_t smem[4 * D * 1024];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = w & 1, wn = w >> 1;
+    const int ntm = (g.M + 64 * WM - 1) / (64 * WM), ntn = (g.N + 64 * WN - 1) / (64 * WN);
+    int tm, tn;
+    tile_map(blockIdx.x, ntm, ntn, tm, tn);
+    const int64_t planeA = g.RbA * g.Kb * kBlkBytes, planeB = g.RbB * g.Kb * kBlkBytes;
+    const int64_t rbA = (int64_t)tm * 2 * WM + wm * WM, rbB = (int64_t)tn * 2 * WN + wn * WN;
+    const int8_t *pa = g.SA + rbA * g.Kb * kBlkBytes + lane * 16;
+    const int8_t *pb = g.SB + rbB * g.Kb * kBlkBytes + lane * 16;
+    const int64_t rowstride = g.Kb * kBlkBytes;
+
+    v4i a0[D][WM], b0[D][WN], a1[D][WM], b1[D][WN];
+    auto load = [&](v4i (&af)[D][WM], v4i (&bf)[D][WN], int64_t kb) {
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+#pragma unroll
+            for (int mi = 0; mi < WM; mi++)
+                af[j][mi] = *reinterpret_cast<const v4i *>(pa + j * planeA + mi * rowstride + kb * kBlkBytes);
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++)
+                bf[j][ni] = *reinterpret_cast<const v4i *>(pb + j * planeB + ni * rowstride + kb * kBlkBytes);
+        }
+    };
+    v16i acc[D][WM][WN];
+#pragma unroll
+    for (int k = 0; k < D; k++)
+#pragma unroll
+        for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++) acc[k][mi][ni] = v16i{};
+    auto mma = [&](const v4i (&af)[D][WM], const v4i (&bf)[D][WN]) {
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; i + j < D; j++)
+#pragma unroll
+                for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+                    for (int ni = 0; ni < WN; ni++)
+                        acc[i + j][mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i][mi], bf[j][ni],
+                                                                                   acc[i + j][mi][ni], 0, 0, 0);
+    };
+    const int64_t kb0 = g.kb0, nk = g.kb1 - g.kb0;   // nk is even
+    if (nk > 0) load(a0, b0, kb0);
+    for (int64_t t = 0; t < nk; t += 2) {
+        load(a1, b1, kb0 + t + 1);
+        mma(a0, b0);
+        if (t + 2 < nk) load(a0, b0, kb0 + t + 2);
+        mma(a1, b1);
+    }
+    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * (D * 256);
+    wave_epilogue<D, WM, WN>(g, epi, 0, acc, scratch, lane, tm * 64 * WM + wm * 32 * WM,
+                             tn * 64 * WN + wn * 32 * WN);
+}
+
+// Pipelined LDS variant: the slice blocks of block t+2 are copied HBM -> LDS
+// (global_load_lds) while block t computes; fragments are double-buffered in registers
+// and the workgroup barrier sits in the middle of each block's MFMAs.  After the
+// barrier every wave reads block t+1's fragments and issues block t+2's copies, and the
+// second half of block t's MFMAs runs while those are in flight, so neither the LDS read
+// latency nor the barrier leaves the MFMA pipe idle.  Requires an even number of
+// contraction blocks per pass.
+template <int D>
+__host__ __device__ constexpr int pipe_split() {
+    int tot = D * (D + 1) / 2, acc = 0;
+    for (int i = 0; i < D; i++) {
+        acc += D - i;
+        if (2 * acc >= tot) return i + 1;
+    }
+    return D;
+}
+
+template <int D, int WM, int WN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_gemm_pipe(GemmLaunch g, int epi) {
+    using F = FusedCfg<D, WM, WN>;
+    constexpr int BA = F::BA, BB = F::BB, NBLK = F::NBLK, NLD = F::NLD, NMIN = F::NMIN;
+    constexpr int I1 = pipe_split<D>();
+    __shared__ __attribute__((aligned(1024))) int8_t smem[2 * F::STAGE];
+
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = w & 1, wn = w >> 1;
+    const int ntm = (g.M + 64 * WM - 1) / (64 * WM), ntn = (g.N + 64 * WN - 1) / (64 * WN);
+    int tm, tn;
+    tile_map(blockIdx.x, ntm, ntn, tm, tn);
+    const int64_t rbA0 = (int64_t)tm * BA, rbB0 = (int64_t)tn * BB;
+    const int64_t planeA = g.RbA * g.Kb * kBlkBytes, planeB = g.RbB * g.Kb * kBlkBytes;
+
+    const int8_t *src[NLD];
+    int ldsoff[NLD];
+#pragma unroll
+    for (int it = 0; it < NLD; it++) {
+        const int blk = w + 4 * it;
+        const int bc = blk < NBLK ? blk : NBLK - 1;
+        if (bc < D * BA) {
+            const int j = bc / BA, mb = bc % BA;
+            src[it] = g.SA + j * planeA + (rbA0 + mb) * g.Kb * kBlkBytes;
+        } else {
+            const int bb = bc - D * BA;
+            const int j = bb / BB, nb = bb % BB;
+            src[it] = g.SB + j * planeB + (rbB0 + nb) * g.Kb * kBlkBytes;
+        }
+        src[it] += lane * 16;
+        ldsoff[it] = bc * kBlkBytes;
+    }
+    auto issue = [&](int stage, int64_t kb) {
+#pragma unroll
+        for (int it = 0; it < NLD; it++) {
+            if (it < NMIN || w + 4 * it < NBLK)
+                __builtin_amdgcn_global_load_lds((const void *)(src[it] + kb * kBlkBytes),
+                                                 (lds_void *)(smem + stage * F::STAGE + ldsoff[it]), 16, 0, 0);
+        }
+    };
+    v4i a0[D][WM], b0[D][WN], a1[D][WM], b1[D][WN];
+    auto readf = [&](v4i (&af)[D][WM], v4i (&bf)[D][WN], int stage) {
+        const int8_t *st = smem + stage * F::STAGE;
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+#pragma unroll
+            for (int mi = 0; mi < WM; mi++)
+                af[j][mi] = *reinterpret_cast<const v4i *>(st + (j * BA + wm * WM + mi) * kBlkBytes + lane * 16);
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++)
+                bf[j][ni] = *reinterpret_cast<const v4i *>(st + (D * BA + j * BB + wn * WN + ni) * kBlkBytes + lane * 16);
+        }
+    };
+    v16i acc[D][WM][WN];
+#pragma unroll
+    for (int k = 0; k < D; k++)
+#pragma unroll
+        for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++) acc[k][mi][ni] = v16i{};
+    auto mma = [&](const v4i (&af)[D][WM], const v4i (&bf)[D][WN], int i0, int i1) {
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            if (i < i0 || i >= i1) continue;
+#pragma unroll
+            for (int j = 0; i + j < D; j++)
+#pragma unroll
+                for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+                    for (int ni = 0; ni < WN; ni++)
+                        acc[i + j][mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i][mi], bf[j][ni],
+                                                                                   acc[i + j][mi][ni], 0, 0, 0);
+        }
+    };
+
+    const int64_t kb0 = g.kb0, nk = g.kb1 - g.kb0;   // nk is even and >= 2
+    issue(0, kb0);
+    __syncthreads();
+    readf(a0, b0, 0);
+    issue(1, kb0 + 1);
+    for (int64_t t = 0; t < nk; t += 2) {
+        // block t in (a0, b0); block t+1 copying into stage 1
+        mma(a0, b0, 0, I1);
+        __builtin_amdgcn_sched_barrier(0);
+        __syncthreads();
+        readf(a1, b1, 1);
+        if (t + 2 < nk) issue(0, kb0 + t + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(a0, b0, I1, D);
+        // block t+1 in (a1, b1); block t+2 copying into stage 0
+        mma(a1, b1, 0, I1);
+        __builtin_amdgcn_sched_barrier(0);
+        if (t + 2 < nk) {
+            __syncthreads();
+            readf(a0, b0, 0);
+            if (t + 3 < nk) issue(1, kb0 + t + 3);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        mma(a1, b1, I1, D);
+    }
+    __syncthreads();
+
+    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * (D * 256);
+    wave_epilogue<D, WM, WN>(g, epi, 0, acc, scratch, lane, tm * 64 * WM + wm * 32 * WM,
+                             tn * 64 * WN + wn * 32 * WN);
+}
+
 bool fused_supported(int d) { return d >= 1 && d <= 16; }
-int fused_wn(int d) { return d <= 4 ? 2 : (d <= 9 ? 2 : 1); }
-static int fused_wm(int d) { return d <= 4 ? 2 : 1; }
+
+// Default fused-kernel family per slice count (measured on MI355X at 8192^3): the
+// direct-to-register kernel for d <= 8 (its two fragment sets fit in registers), the
+// LDS-copy kernel above that.
+static int default_family(int d) { return d <= 8 ? KERNEL_DIRECT : KERNEL_LDS; }
 
 template <int D, int WM, int WN>
 static hipError_t launch_fused_t(const GemmLaunch &g, int epi, hipStream_t st) {
     const int ntm = (g.M + 64 * WM - 1) / (64 * WM), ntn = (g.N + 64 * WN - 1) / (64 * WN);
     const int64_t tiles = (int64_t)ntm * ntn;
     if (tiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_gemm_fused<D, WM, WN>), dim3((unsigned)tiles), dim3(256), 0, st, g, epi);
+    int fam = g.family == KERNEL_AUTO ? default_family(D) : g.family;
+    if (fam == KERNEL_DIRECT)
+        hipLaunchKernelGGL((k_gemm_direct<D, WM, WN>), dim3((unsigned)tiles), dim3(256), 0, st, g, epi);
+    else if (fam == KERNEL_PIPE)
+        hipLaunchKernelGGL((k_gemm_pipe<D, WM, WN>), dim3((unsigned)tiles), dim3(256), 0, st, g, epi);
+    else
+        hipLaunchKernelGGL((k_gemm_fused<D, WM, WN>), dim3((unsigned)tiles), dim3(256), 0, st, g, epi);
     return hipGetLastError();
 }
 
+// Wave tile per slice count: the accumulators (d bins x 16 registers per 32 x 32 block)
+// must leave room for the fragments in the 512 registers of a wave.
 hipError_t launch_gemm_fused(const GemmLaunch &g, int epi, hipStream_t st) {
     switch (g.d) {
         case 1: return launch_fused_t<1, 2, 2>(g, epi, st);

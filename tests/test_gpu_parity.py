@@ -52,8 +52,11 @@ def test_split_bit_exact(gpu, b, d):
     assert np.array_equal(S, oracle.split(B, Eo, b, d, 0))
 
 
+# kernel: 0 auto, 1 bin-group, 2 fused LDS copies, 3 fused direct loads, 4 fused pipelined
 BIN_CASES = [(7, 8, 0), (7, 3, 0), (7, 7, 0), (7, 9, 0), (7, 1, 0), (7, 4, 0), (7, 10, 0), (6, 16, 0),
-             (3, 29, 1), (3, 26, 1), (7, 8, 1), (4, 18, 1)]
+             (3, 29, 1), (3, 26, 1), (7, 8, 1), (4, 18, 1),
+             (7, 8, 2), (7, 3, 2), (7, 9, 2), (7, 8, 3), (7, 3, 3), (7, 5, 3), (7, 8, 4), (7, 3, 4),
+             (7, 7, 4), (6, 12, 4)]
 
 
 @pytest.mark.parametrize("b,d,kernel", BIN_CASES)
