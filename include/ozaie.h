@@ -104,6 +104,41 @@ int32_t ozaie_dgemm(ozaie_plan plan, int transA, int transB, const double *A, in
  * events of that call.                                                                 */
 int32_t ozaie_plan_phase_times(ozaie_plan plan, float *out3);
 
+/* ---- grouped products (many independent C_i = A_i B_i of uneven shape)
+ * A_i (M_i x K_i), B_i (K_i x N_i), C_i (M_i x N_i): row-major, contiguous, device pointers.
+ * All problems share b and d.  Slice counts 3, 7, 8, 9 run as one exponent launch, one
+ * split launch and one fused-GEMM launch per tile configuration, each tile
+ * configuration chosen per size class by the autotuner table (or a heuristic when the
+ * class is untuned); other slice counts run one plan per problem.  Every K_i must fit
+ * one contraction pass.  The launches are captured once in a hipGraph (flags bit 1
+ * disables the graph) and replayed by ozaie_group_run; descriptors are re-uploaded only
+ * when the pointers change.  flags bit 0: nearest-even rounding.                      */
+typedef struct ozaie_group_s *ozaie_group;
+int32_t ozaie_group_create(ozaie_group *group, int count, const int64_t *M, const int64_t *N,
+                           const int64_t *K, int bits, int slices, int flags);
+/* A, B, C: host arrays of `count` device pointers.  Asynchronous on `stream`.         */
+int32_t ozaie_group_run(ozaie_group group, const double *const *A, const double *const *B,
+                        double *const *C, void *stream);
+/* out[0] kernel launches per run, out[1] graph captured, out[2] per-problem fallback,
+ * out[3] fused-GEMM workgroups per run.                                               */
+int32_t ozaie_group_query(ozaie_group group, int32_t *out4);
+/* Tile configuration chosen for each problem (-1 in the per-problem fallback).        */
+int32_t ozaie_group_configs(ozaie_group group, int32_t *cfg_out);
+int32_t ozaie_group_destroy(ozaie_group group);
+/* One-shot grouped product without a graph; blocking on `stream`.                     */
+int32_t ozaie_dgemm_grouped(int count, const int64_t *M, const int64_t *N, const int64_t *K,
+                            const double *const *A, const double *const *B, double *const *C,
+                            int bits, int slices, void *stream);
+
+/* Autotuner: times every tile configuration on representative shapes of the 9 size
+ * classes (max(M,N) and K each in {<=256, <=1024, >1024}) and keeps the fastest in the
+ * process-wide table (table9: class 3*mn_bucket + k_bucket -> configuration id;
+ * 0: 64x64 direct loads, 1: 64x128 direct, 2: 128x128 direct, 3: 64x64 LDS copies,
+ * 4: 64x128 LDS copies).  reps: timed runs per candidate.                             */
+int32_t ozaie_autotune(int bits, int slices, int reps, int32_t *table9);
+int32_t ozaie_tune_get(int slices, int32_t *table9);   /* -1 entries: untuned */
+int32_t ozaie_tune_set(int slices, const int32_t *table9);
+
 /* ---- inspection (reference full_dump equivalent, whole_array_iron_ozaki.py:485-558)  */
 
 /* Exponents and slices of one operand, as computed on the GPU, in plain layout.

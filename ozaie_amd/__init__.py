@@ -54,7 +54,9 @@ EXPORTS = [
     "ozaie_plan_info", "ozaie_release", "ozaie_plan_create", "ozaie_plan_destroy",
     "ozaie_plan_query", "ozaie_dgemm", "ozaie_debug_split", "ozaie_debug_bins",
     "ozaie_paired_output", "ozaie_last_error", "ozaie_version", "ozaie_plan_phase_times",
-    "ozaie_exact_entries",
+    "ozaie_exact_entries", "ozaie_group_create", "ozaie_group_run", "ozaie_group_query",
+    "ozaie_group_configs", "ozaie_group_destroy", "ozaie_dgemm_grouped", "ozaie_autotune",
+    "ozaie_tune_get", "ozaie_tune_set",
 ]
 
 
@@ -124,6 +126,24 @@ def _declare(L):
     L.ozaie_plan_phase_times.restype = i32
     L.ozaie_exact_entries.argtypes = [P, i64, P, i64, i64, P, P, i64, P]
     L.ozaie_exact_entries.restype = i32
+    L.ozaie_group_create.argtypes = [ctypes.POINTER(P), c_int, P, P, P, c_int, c_int, c_int]
+    L.ozaie_group_create.restype = i32
+    L.ozaie_group_run.argtypes = [P, P, P, P, P]
+    L.ozaie_group_run.restype = i32
+    L.ozaie_group_query.argtypes = [P, P]
+    L.ozaie_group_query.restype = i32
+    L.ozaie_group_configs.argtypes = [P, P]
+    L.ozaie_group_configs.restype = i32
+    L.ozaie_group_destroy.argtypes = [P]
+    L.ozaie_group_destroy.restype = i32
+    L.ozaie_dgemm_grouped.argtypes = [c_int, P, P, P, P, P, P, c_int, c_int, P]
+    L.ozaie_dgemm_grouped.restype = i32
+    L.ozaie_autotune.argtypes = [c_int, c_int, c_int, P]
+    L.ozaie_autotune.restype = i32
+    L.ozaie_tune_get.argtypes = [c_int, P]
+    L.ozaie_tune_get.restype = i32
+    L.ozaie_tune_set.argtypes = [c_int, P]
+    L.ozaie_tune_set.restype = i32
     L.ozaie_last_error.argtypes = []
     L.ozaie_last_error.restype = ctypes.c_char_p
     L.ozaie_version.argtypes = []
@@ -230,8 +250,8 @@ class Plan:
         for x in (A, B, C):
             if x.dtype != torch.float64 or not x.is_cuda:
                 raise ValueError("operands must be float64 device tensors")
-            if x.stride(1) != 1:
-                raise ValueError("operands must have unit stride along their last axis")
+            if x.stride(1) != 1 or x.stride(0) < x.shape[1]:
+                raise ValueError("operands must be row-major with unit stride along their last axis")
         ea = (self.K, self.M) if transA else (self.M, self.K)
         eb = (self.N, self.K) if transB else (self.K, self.N)
         if tuple(A.shape) != ea or tuple(B.shape) != eb or tuple(C.shape) != (self.M, self.N):
@@ -317,6 +337,83 @@ def exact_entries(A, B, rows, cols):
     _check(lib().ozaie_exact_entries(_ptr(A), A.shape[1], _ptr(B), B.shape[1], A.shape[1], _ptr(rows),
                                      _ptr(cols), len(rows), _ptr(out)), "ozaie_exact_entries")
     return out
+
+
+class Group:
+    """Many independent products C_i = A_i @ B_i (float64 device tensors, row-major) with
+    b bits and d slices each, replayed through one captured hipGraph."""
+
+    GRAPH = 0
+    NO_GRAPH = 2
+
+    def __init__(self, shapes, bits=7, slices=8, rounding=ROUND_TRUNCATE, graph=True):
+        self.shapes = [(int(m), int(n), int(k)) for m, n, k in shapes]
+        cnt = len(self.shapes)
+        M = np.array([s[0] for s in self.shapes], dtype=np.int64)
+        N = np.array([s[1] for s in self.shapes], dtype=np.int64)
+        K = np.array([s[2] for s in self.shapes], dtype=np.int64)
+        flags = (1 if rounding == ROUND_NEAREST else 0) | (0 if graph else 2)
+        h = ctypes.c_void_p()
+        _check(lib().ozaie_group_create(ctypes.byref(h), cnt, _ptr(M), _ptr(N), _ptr(K), int(bits),
+                                        int(slices), flags), "ozaie_group_create")
+        self._h = h
+        self.bits, self.slices = int(bits), int(slices)
+
+    def __call__(self, As, Bs, Cs, stream=None):
+        cnt = len(self.shapes)
+        if not (len(As) == len(Bs) == len(Cs) == cnt):
+            raise ValueError("need one A, B and C per problem")
+        for (m, n, k), a, b, c in zip(self.shapes, As, Bs, Cs):
+            if tuple(a.shape) != (m, k) or tuple(b.shape) != (k, n) or tuple(c.shape) != (m, n):
+                raise ValueError("operand shapes differ from the group's problem shapes")
+            for x in (a, b, c):
+                if not (x.is_cuda and str(x.dtype) == "torch.float64" and x.is_contiguous()):
+                    raise ValueError("operands must be contiguous float64 device tensors")
+        pa = np.array([a.data_ptr() for a in As], dtype=np.uint64)
+        pb = np.array([b.data_ptr() for b in Bs], dtype=np.uint64)
+        pc = np.array([c.data_ptr() for c in Cs], dtype=np.uint64)
+        _check(lib().ozaie_group_run(self._h, _ptr(pa), _ptr(pb), _ptr(pc), _stream_handle(stream)),
+               "ozaie_group_run")
+        return Cs
+
+    def query(self):
+        out = np.zeros(4, dtype=np.int32)
+        _check(lib().ozaie_group_query(self._h, _ptr(out)), "ozaie_group_query")
+        return dict(launches=int(out[0]), graph=bool(out[1]), fallback=bool(out[2]), workgroups=int(out[3]))
+
+    def configs(self):
+        out = np.zeros(len(self.shapes), dtype=np.int32)
+        _check(lib().ozaie_group_configs(self._h, _ptr(out)), "ozaie_group_configs")
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().ozaie_group_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(bits=7, slices=8, reps=3):
+    """Time the grouped tile configurations per size class; returns the 9-entry table."""
+    out = np.zeros(9, dtype=np.int32)
+    _check(lib().ozaie_autotune(int(bits), int(slices), int(reps), _ptr(out)), "ozaie_autotune")
+    return out
+
+
+def tune_table(slices):
+    out = np.zeros(9, dtype=np.int32)
+    _check(lib().ozaie_tune_get(int(slices), _ptr(out)), "ozaie_tune_get")
+    return out
+
+
+def set_tune_table(slices, table):
+    t = np.ascontiguousarray(table, dtype=np.int32)
+    _check(lib().ozaie_tune_set(int(slices), _ptr(t)), "ozaie_tune_set")
 
 
 def version():

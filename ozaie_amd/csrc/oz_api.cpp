@@ -63,6 +63,11 @@ int ensure_device() {
 
 }  // namespace
 
+namespace oz {
+int set_error(hipError_t e, const char *where) { return set_hip_error(e, where); }
+void set_error_msg(const char *msg) { g_last_error = msg; }
+}  // namespace oz
+
 struct ozaie_plan_s {
     int64_t M = 0, N = 0, K = 0;
     int b = 0, d = 0, rounding = 0;
@@ -110,6 +115,12 @@ int64_t max_kchunk(int b, int d) {
     int64_t kc = (int64_t)INT32_MAX / ((int64_t)d * m * m);
     return kc / (2 * kBlk) * (2 * kBlk);
 }
+
+}  // namespace
+
+int64_t oz::max_kchunk_blocks(int b, int d) { return std::max<int64_t>(2, max_kchunk(b, d) / kBlk / 2 * 2); }
+
+namespace {
 
 int plan_configure(ozaie_plan p, int64_t M, int64_t N, int64_t K, int b, int d, int rounding) {
     if (M < 1 || N < 1 || K < 1) { g_last_error = "dimensions must be >= 1"; return OZAIE_ERR_DIMS; }

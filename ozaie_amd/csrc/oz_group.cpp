@@ -1,0 +1,481 @@
+// oz_group.cpp -- grouped emulated DGEMM: many independent problems of uneven shape in
+// one split launch and one fused-GEMM launch per tile configuration, captured once in a
+// hipGraph and replayed; plus the autotuner that picks the tile configuration of each
+// size class.
+//
+// The reference has no batched entry point (include/oz_aie.h handles one square product
+// per setup); this is new functionality on top of the same arithmetic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/ozaie.h"
+#include "oz_internal.h"
+
+using namespace oz;
+
+namespace {
+
+std::mutex g_tune_mu;
+// g_tune[d][size class] = GroupCfg, -1 = not tuned (heuristic default)
+int g_tune[kMaxSlices + 1][9];
+bool g_tune_init = false;
+
+void tune_init() {
+    if (g_tune_init) return;
+    for (auto &row : g_tune)
+        for (int &v : row) v = -1;
+    g_tune_init = true;
+}
+
+int bucket(int64_t x) { return x <= 256 ? 0 : (x <= 1024 ? 1 : 2); }
+int size_class(int64_t M, int64_t N, int64_t K) { return 3 * bucket(std::max(M, N)) + bucket(K); }
+
+// Heuristic when a class has not been tuned: small problems take small tiles (more
+// workgroups), large ones the widest tile that fits in registers.
+int default_cfg(int d, int64_t M, int64_t N) {
+    const int64_t mn = std::max(M, N);
+    if (d == 3) return mn <= 256 ? GCFG_D11 : (mn <= 1024 ? GCFG_D12 : GCFG_D22);
+    if (d == 9) return mn <= 512 ? GCFG_D11 : GCFG_L12;
+    return mn <= 512 ? GCFG_D11 : GCFG_D12;
+}
+
+int pick_cfg(int d, int64_t M, int64_t N, int64_t K, int force) {
+    if (force >= 0 && group_cfg_supported(d, force)) return force;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_init();
+    const int t = g_tune[d][size_class(M, N, K)];
+    if (t >= 0 && group_cfg_supported(d, t)) return t;
+    return default_cfg(d, M, N);
+}
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+struct ozaie_group_s {
+    int count = 0, b = 0, d = 0, rounding = 0, flags = 0, device = 0;
+    std::vector<int64_t> M, N, K;
+    std::vector<int> cfg;
+    bool fallback = false;                 // slice count without grouped kernels
+    std::vector<ozaie_plan> plans;         // fallback: one plan per problem
+    // device
+    char *ws = nullptr;                    // exponents + slices of every problem
+    int32_t *E_all = nullptr;
+    int64_t n_E = 0;
+    GroupOperand *d_ops = nullptr;
+    GemmLaunch *d_probs = nullptr;
+    int4 *d_items_max = nullptr, *d_items_split = nullptr;
+    int n_max = 0, n_split = 0;
+    int2 *d_items_gemm[GCFG_COUNT] = {};
+    int n_gemm[GCFG_COUNT] = {};
+    int64_t tiles = 0;
+    // host mirrors of the descriptors (pinned) and the pointers they hold
+    GroupOperand *h_ops = nullptr;
+    GemmLaunch *h_probs = nullptr;
+    hipEvent_t upload_done = nullptr;
+    bool uploaded = false;
+    std::vector<const double *> lastA, lastB;
+    std::vector<double *> lastC;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void group_free(ozaie_group g) {
+    if (!g) return;
+    for (ozaie_plan p : g->plans) ozaie_plan_destroy(p);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->cap_stream) (void)hipStreamDestroy(g->cap_stream);
+    if (g->upload_done) (void)hipEventDestroy(g->upload_done);
+    if (g->ws) (void)hipFree(g->ws);
+    if (g->d_ops) (void)hipFree(g->d_ops);
+    if (g->d_probs) (void)hipFree(g->d_probs);
+    if (g->d_items_max) (void)hipFree(g->d_items_max);
+    if (g->d_items_split) (void)hipFree(g->d_items_split);
+    for (auto &p : g->d_items_gemm)
+        if (p) (void)hipFree(p);
+    if (g->h_ops) (void)hipHostFree(g->h_ops);
+    if (g->h_probs) (void)hipHostFree(g->h_probs);
+    delete g;
+}
+
+#define GCHK(expr)                                          \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return set_error(e_, #expr);  \
+    } while (0)
+
+template <typename T>
+int upload(T **dptr, const std::vector<T> &v) {
+    if (v.empty()) return OZAIE_OK;
+    GCHK(hipMalloc(dptr, sizeof(T) * v.size()));
+    GCHK(hipMemcpy(*dptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return OZAIE_OK;
+}
+
+// Workspace layout, descriptors and work-item tables; pointers to A, B, C are filled in
+// at run time.
+int group_build(ozaie_group g, int force_cfg) {
+    const int n = g->count;
+    const int64_t kmax = max_kchunk_blocks(g->b, g->d) * kBlk;
+    std::vector<int64_t> Mp(n), Np(n), Kp(n);
+    int64_t nE = 0, slice_bytes = 0;
+    for (int p = 0; p < n; p++) {
+        if (g->K[p] > kmax) {
+            set_error_msg("grouped problems must fit one contraction pass (K too long for these slices)");
+            return OZAIE_ERR_DIMS;
+        }
+        Mp[p] = round_up(g->M[p], kRowPad);
+        Np[p] = round_up(g->N[p], kRowPad);
+        Kp[p] = round_up(g->K[p], 2 * kBlk);
+        nE += Mp[p] + Np[p];
+        slice_bytes += (int64_t)g->d * (Mp[p] + Np[p]) * Kp[p];
+        g->cfg[p] = pick_cfg(g->d, g->M[p], g->N[p], g->K[p], force_cfg);
+    }
+    const int64_t e_bytes = round_up(nE * 4, 1024);
+    GCHK(hipMalloc(&g->ws, (size_t)(e_bytes + slice_bytes)));
+    g->E_all = reinterpret_cast<int32_t *>(g->ws);
+    g->n_E = nE;
+    GCHK(hipHostMalloc(&g->h_ops, sizeof(GroupOperand) * 2 * n));
+    GCHK(hipHostMalloc(&g->h_probs, sizeof(GemmLaunch) * n));
+    std::memset(g->h_ops, 0, sizeof(GroupOperand) * 2 * n);
+    std::memset(g->h_probs, 0, sizeof(GemmLaunch) * n);
+    std::vector<int4> imax, isplit;
+    std::vector<int2> igemm[GCFG_COUNT];
+    int64_t eoff = 0, soff = e_bytes;
+    for (int p = 0; p < n; p++) {
+        GroupOperand &a = g->h_ops[2 * p], &bo = g->h_ops[2 * p + 1];
+        a.R = g->M[p]; a.K = g->K[p]; a.Rp = Mp[p]; a.sr = g->K[p]; a.sk = 1;
+        bo.R = g->N[p]; bo.K = g->K[p]; bo.Rp = Np[p]; bo.sr = 1; bo.sk = g->N[p];
+        a.E = g->E_all + eoff; eoff += Mp[p];
+        bo.E = g->E_all + eoff; eoff += Np[p];
+        a.S = reinterpret_cast<int8_t *>(g->ws + soff); soff += (int64_t)g->d * Mp[p] * Kp[p];
+        bo.S = reinterpret_cast<int8_t *>(g->ws + soff); soff += (int64_t)g->d * Np[p] * Kp[p];
+        a.Rb = Mp[p] / kBlk; bo.Rb = Np[p] / kBlk;
+        a.Kb = bo.Kb = Kp[p] / kBlk;
+        // exponent max: rows of A (one wave per row), columns of B (slabs of 256 x 256)
+        for (int64_t q = 0; q < (Mp[p] + 3) / 4; q++) imax.push_back(int4{2 * p, (int)q, 0, 0});
+        const int64_t kper = 256;
+        for (int64_t cx = 0; cx < (g->N[p] + 255) / 256; cx++)
+            for (int64_t ky = 0; ky < (g->K[p] + kper - 1) / kper; ky++)
+                imax.push_back(int4{2 * p + 1, (int)cx, (int)ky, (int)kper});
+        // split tiles: 32 rows x 128 contraction indices
+        for (int op = 0; op < 2; op++) {
+            const GroupOperand &o = g->h_ops[2 * p + op];
+            for (int64_t rb = 0; rb < o.Rb; rb++)
+                for (int64_t kt = 0; kt < (o.Kb + 3) / 4; kt++)
+                    isplit.push_back(int4{2 * p + op, (int)rb, (int)kt, 0});
+        }
+        GemmLaunch &gl = g->h_probs[p];
+        gl.SA = a.S; gl.SB = bo.S; gl.Er = a.E; gl.Ec = bo.E;
+        gl.RbA = a.Rb; gl.RbB = bo.Rb; gl.Kb = a.Kb; gl.kb0 = 0; gl.kb1 = a.Kb;
+        gl.M = (int)g->M[p]; gl.N = (int)g->N[p]; gl.b = g->b; gl.d = g->d; gl.k0 = 0;
+        gl.rounding = g->rounding; gl.first = 1; gl.last = 1; gl.ldc = g->N[p];
+        gl.K_true = (int)g->K[p];
+        const int c = g->cfg[p];
+        const int64_t ntm = (g->M[p] + group_cfg_tile_m(c) - 1) / group_cfg_tile_m(c);
+        const int64_t ntn = (g->N[p] + group_cfg_tile_n(c) - 1) / group_cfg_tile_n(c);
+        for (int64_t t = 0; t < ntm * ntn; t++) igemm[c].push_back(int2{p, (int)t});
+        g->tiles += ntm * ntn;
+    }
+    int rc;
+    if ((rc = upload(&g->d_items_max, imax))) return rc;
+    if ((rc = upload(&g->d_items_split, isplit))) return rc;
+    g->n_max = (int)imax.size();
+    g->n_split = (int)isplit.size();
+    for (int c = 0; c < GCFG_COUNT; c++) {
+        if ((rc = upload(&g->d_items_gemm[c], igemm[c]))) return rc;
+        g->n_gemm[c] = (int)igemm[c].size();
+    }
+    GCHK(hipMalloc(&g->d_ops, sizeof(GroupOperand) * 2 * n));
+    GCHK(hipMalloc(&g->d_probs, sizeof(GemmLaunch) * n));
+    GCHK(hipEventCreateWithFlags(&g->upload_done, hipEventDisableTiming));
+    GCHK(hipStreamCreateWithFlags(&g->cap_stream, hipStreamNonBlocking));
+    g->lastA.assign(n, nullptr);
+    g->lastB.assign(n, nullptr);
+    g->lastC.assign(n, nullptr);
+    return OZAIE_OK;
+}
+
+int group_enqueue(ozaie_group g, hipStream_t st) {
+    GCHK(launch_fill_i32(g->E_all, g->n_E, 0, st));
+    GCHK(launch_group_maxexp(g->d_ops, g->d_items_max, g->n_max, st));
+    GCHK(launch_group_split(g->d_ops, g->d_items_split, g->n_split, g->b, g->d, st));
+    for (int c = 0; c < GCFG_COUNT; c++)
+        GCHK(launch_group_gemm(g->d_probs, g->d_items_gemm[c], g->n_gemm[c], g->d, c, st));
+    return OZAIE_OK;
+}
+
+int group_capture(ozaie_group g) {
+    GCHK(hipStreamBeginCapture(g->cap_stream, hipStreamCaptureModeThreadLocal));
+    int rc = group_enqueue(g, g->cap_stream);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(g->cap_stream, &graph);
+    if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    if (e != hipSuccess) return set_error(e, "hipStreamEndCapture");
+    g->graph = graph;
+    GCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    return OZAIE_OK;
+}
+
+int group_new(ozaie_group *out, int count, const int64_t *M, const int64_t *N, const int64_t *K, int bits,
+              int slices, int flags, int force_cfg) {
+    *out = nullptr;
+    if (count < 1 || !M || !N || !K) return OZAIE_ERR_ARG;
+    if (bits < 1 || bits > kMaxBits || slices < 1 || slices > kMaxSlices || bits * slices > 96) return OZAIE_ERR_ARG;
+    for (int p = 0; p < count; p++)
+        if (M[p] < 1 || N[p] < 1 || K[p] < 1 || M[p] > INT32_MAX / 2 || N[p] > INT32_MAX / 2 || K[p] > INT32_MAX / 2)
+            return OZAIE_ERR_DIMS;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        set_error_msg("no HIP device available");
+        return OZAIE_ERR_NODEVICE;
+    }
+    ozaie_group g = new ozaie_group_s();
+    g->count = count; g->b = bits; g->d = slices; g->flags = flags;
+    g->rounding = (flags & 1) ? OZAIE_ROUND_NEAREST : OZAIE_ROUND_TRUNCATE;
+    g->M.assign(M, M + count);
+    g->N.assign(N, N + count);
+    g->K.assign(K, K + count);
+    g->cfg.assign(count, 0);
+    int rc = OZAIE_OK;
+    hipError_t e = hipGetDevice(&g->device);
+    if (e != hipSuccess) rc = set_error(e, "hipGetDevice");
+    g->fallback = !group_cfg_supported(slices, GCFG_D11);
+    if (!rc && g->fallback) {
+        for (int p = 0; p < count && !rc; p++) {
+            ozaie_plan pl = nullptr;
+            rc = ozaie_plan_create(&pl, M[p], N[p], K[p], bits, slices, flags & 1);
+            if (!rc) g->plans.push_back(pl);
+        }
+        if (!rc) {
+            e = hipStreamCreateWithFlags(&g->cap_stream, hipStreamNonBlocking);
+            if (e != hipSuccess) rc = set_error(e, "hipStreamCreate");
+        }
+        g->lastA.assign(count, nullptr);
+        g->lastB.assign(count, nullptr);
+        g->lastC.assign(count, nullptr);
+    } else if (!rc) {
+        rc = group_build(g, force_cfg);
+    }
+    if (rc) { group_free(g); return rc; }
+    *out = g;
+    return OZAIE_OK;
+}
+
+int group_fallback_enqueue(ozaie_group g, hipStream_t st) {
+    for (int p = 0; p < g->count; p++) {
+        int rc = ozaie_dgemm(g->plans[p], 0, 0, g->lastA[p], g->K[p], g->lastB[p], g->N[p], g->lastC[p], g->N[p], st);
+        if (rc) return rc;
+    }
+    return OZAIE_OK;
+}
+
+int group_run(ozaie_group g, const double *const *A, const double *const *B, double *const *C, hipStream_t st) {
+    if (!A || !B || !C) return OZAIE_ERR_ARG;
+    bool changed = !g->uploaded;
+    for (int p = 0; p < g->count; p++) {
+        if (!A[p] || !B[p] || !C[p]) return OZAIE_ERR_ARG;
+        changed |= A[p] != g->lastA[p] || B[p] != g->lastB[p] || C[p] != g->lastC[p];
+    }
+    const bool use_graph = !(g->flags & 2);
+    if (g->fallback) {
+        if (changed && g->exec) {
+            (void)hipGraphExecDestroy(g->exec);
+            (void)hipGraphDestroy(g->graph);
+            g->exec = nullptr;
+            g->graph = nullptr;
+        }
+        for (int p = 0; p < g->count; p++) { g->lastA[p] = A[p]; g->lastB[p] = B[p]; g->lastC[p] = C[p]; }
+        g->uploaded = true;
+        if (!use_graph) return group_fallback_enqueue(g, st);
+        if (!g->exec) {
+            GCHK(hipStreamBeginCapture(g->cap_stream, hipStreamCaptureModeThreadLocal));
+            int rc = group_fallback_enqueue(g, g->cap_stream);
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(g->cap_stream, &graph);
+            if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+            if (e != hipSuccess) return set_error(e, "hipStreamEndCapture");
+            g->graph = graph;
+            GCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+        }
+        GCHK(hipGraphLaunch(g->exec, st));
+        return OZAIE_OK;
+    }
+    if (changed) {
+        // the previous upload must have left the pinned mirrors before they are rewritten
+        if (g->uploaded) GCHK(hipEventSynchronize(g->upload_done));
+        for (int p = 0; p < g->count; p++) {
+            g->h_ops[2 * p].X = A[p];
+            g->h_ops[2 * p + 1].X = B[p];
+            g->h_probs[p].C = C[p];
+            g->lastA[p] = A[p]; g->lastB[p] = B[p]; g->lastC[p] = C[p];
+        }
+        GCHK(hipMemcpyAsync(g->d_ops, g->h_ops, sizeof(GroupOperand) * 2 * g->count, hipMemcpyHostToDevice, st));
+        GCHK(hipMemcpyAsync(g->d_probs, g->h_probs, sizeof(GemmLaunch) * g->count, hipMemcpyHostToDevice, st));
+        GCHK(hipEventRecord(g->upload_done, st));
+        g->uploaded = true;
+    }
+    if (!use_graph) return group_enqueue(g, st);
+    if (!g->exec) {
+        int rc = group_capture(g);
+        if (rc) return rc;
+    }
+    GCHK(hipGraphLaunch(g->exec, st));
+    return OZAIE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ozaie_group_create(ozaie_group *g, int count, const int64_t *M, const int64_t *N, const int64_t *K,
+                           int bits, int slices, int flags) {
+    if (!g) return OZAIE_ERR_ARG;
+    return group_new(g, count, M, N, K, bits, slices, flags, -1);
+}
+
+int32_t ozaie_group_run(ozaie_group g, const double *const *A, const double *const *B, double *const *C,
+                        void *stream) {
+    if (!g) return OZAIE_ERR_ARG;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return OZAIE_ERR_HIP;
+    if (cur != g->device) (void)hipSetDevice(g->device);
+    int rc = group_run(g, A, B, C, (hipStream_t)stream);
+    if (cur != g->device) (void)hipSetDevice(cur);
+    return rc;
+}
+
+int32_t ozaie_group_query(ozaie_group g, int32_t *out4) {
+    if (!g || !out4) return OZAIE_ERR_ARG;
+    int launches = 0;
+    if (g->fallback) launches = g->count * 5;
+    else {
+        launches = 3;
+        for (int c = 0; c < GCFG_COUNT; c++) launches += g->n_gemm[c] > 0;
+    }
+    out4[0] = launches;
+    out4[1] = g->exec != nullptr;
+    out4[2] = g->fallback;
+    out4[3] = (int32_t)std::min<int64_t>(g->tiles, INT32_MAX);
+    return OZAIE_OK;
+}
+
+int32_t ozaie_group_configs(ozaie_group g, int32_t *cfg_out) {
+    if (!g || !cfg_out) return OZAIE_ERR_ARG;
+    for (int p = 0; p < g->count; p++) cfg_out[p] = g->fallback ? -1 : g->cfg[p];
+    return OZAIE_OK;
+}
+
+int32_t ozaie_group_destroy(ozaie_group g) {
+    if (g) {
+        if (g->cap_stream) (void)hipStreamSynchronize(g->cap_stream);
+        (void)hipDeviceSynchronize();
+    }
+    group_free(g);
+    return OZAIE_OK;
+}
+
+int32_t ozaie_tune_get(int slices, int32_t *table9) {
+    if (!table9 || slices < 1 || slices > kMaxSlices) return OZAIE_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_init();
+    for (int c = 0; c < 9; c++) table9[c] = g_tune[slices][c];
+    return OZAIE_OK;
+}
+
+int32_t ozaie_tune_set(int slices, const int32_t *table9) {
+    if (!table9 || slices < 1 || slices > kMaxSlices) return OZAIE_ERR_ARG;
+    for (int c = 0; c < 9; c++)
+        if (table9[c] >= 0 && !group_cfg_supported(slices, table9[c])) return OZAIE_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_init();
+    for (int c = 0; c < 9; c++) g_tune[slices][c] = table9[c];
+    return OZAIE_OK;
+}
+
+int32_t ozaie_autotune(int bits, int slices, int reps, int32_t *table9) {
+    if (bits < 1 || bits > kMaxBits || slices < 1 || slices > kMaxSlices) return OZAIE_ERR_ARG;
+    if (!group_cfg_supported(slices, GCFG_D11)) return OZAIE_ERR_ARG;
+    if (reps < 1) reps = 3;
+    static const int64_t rep_mn[3] = {128, 512, 2048}, rep_k[3] = {128, 512, 2048};
+    int32_t best[9];
+    for (int a = 0; a < 3; a++)
+        for (int kb = 0; kb < 3; kb++) {
+            const int64_t mn = rep_mn[a], kk = rep_k[kb];
+            // enough copies of the representative shape to fill the GPU a few times
+            const int64_t tiles1 = ((mn + 63) / 64) * ((mn + 63) / 64);
+            const int cnt = (int)std::max<int64_t>(1, std::min<int64_t>(64, 2048 / tiles1));
+            std::vector<int64_t> Ms(cnt, mn), Ns(cnt, mn), Ks(cnt, kk);
+            double *buf = nullptr;
+            const int64_t per = mn * kk;   // A and B are mn x kk / kk x mn
+            hipError_t e = hipMalloc(&buf, sizeof(double) * (size_t)(2 * per + mn * mn));
+            if (e != hipSuccess) return set_error(e, "hipMalloc");
+            e = launch_fill_random(buf, 2 * per, 1234u + a * 3 + kb, 0);
+            if (e != hipSuccess) { (void)hipFree(buf); return set_error(e, "fill"); }
+            std::vector<const double *> As(cnt, buf), Bs(cnt, buf + per);
+            std::vector<double *> Cs(cnt, buf + 2 * per);
+            float best_ms = 1e30f;
+            int best_cfg = default_cfg(slices, mn, mn);
+            for (int c = 0; c < GCFG_COUNT; c++) {
+                if (!group_cfg_supported(slices, c)) continue;
+                ozaie_group g = nullptr;
+                int rc = group_new(&g, cnt, Ms.data(), Ns.data(), Ks.data(), bits, slices, 0, c);
+                if (rc) { (void)hipFree(buf); return rc; }
+                hipEvent_t e0, e1;
+                (void)hipEventCreate(&e0);
+                (void)hipEventCreate(&e1);
+                rc = group_run(g, As.data(), Bs.data(), Cs.data(), 0);   // capture + warm-up
+                if (!rc) {
+                    (void)hipEventRecord(e0, 0);
+                    for (int r = 0; r < reps && !rc; r++) rc = group_run(g, As.data(), Bs.data(), Cs.data(), 0);
+                    (void)hipEventRecord(e1, 0);
+                    (void)hipEventSynchronize(e1);
+                    float ms = 0.f;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    if (!rc && ms < best_ms) { best_ms = ms; best_cfg = c; }
+                }
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                (void)hipDeviceSynchronize();
+                group_free(g);
+                if (rc) { (void)hipFree(buf); return rc; }
+            }
+            (void)hipFree(buf);
+            best[3 * a + kb] = best_cfg;
+        }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_init();
+        for (int c = 0; c < 9; c++) g_tune[slices][c] = best[c];
+    }
+    if (table9)
+        for (int c = 0; c < 9; c++) table9[c] = best[c];
+    return OZAIE_OK;
+}
+
+int32_t ozaie_dgemm_grouped(int count, const int64_t *M, const int64_t *N, const int64_t *K,
+                            const double *const *A, const double *const *B, double *const *C, int bits,
+                            int slices, void *stream) {
+    ozaie_group g = nullptr;
+    int rc = ozaie_group_create(&g, count, M, N, K, bits, slices, 2);
+    if (rc) return rc;
+    rc = ozaie_group_run(g, A, B, C, stream);
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+        if (e != hipSuccess) rc = set_error(e, "hipStreamSynchronize");
+    }
+    group_free(g);
+    return rc;
+}
+
+}  // extern "C"

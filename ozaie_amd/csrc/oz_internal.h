@@ -58,6 +58,35 @@ struct GemmLaunch {
 // Fused-kernel families (same arithmetic, different data movement).
 enum Family { KERNEL_AUTO = 0, KERNEL_LDS = 2, KERNEL_DIRECT = 3, KERNEL_PIPE = 4 };
 
+// One operand of one problem of a grouped launch (R rows x K contraction indices).
+struct GroupOperand {
+    const double *X;
+    int64_t sr, sk, R, K, Rp;
+    int32_t *E;
+    int8_t *S;
+    int64_t Rb, Kb;
+};
+
+// Tile configurations of the grouped fused GEMM (wave tile (32 WM) x (32 WN)).
+enum GroupCfg { GCFG_D11 = 0, GCFG_D12 = 1, GCFG_D22 = 2, GCFG_L11 = 3, GCFG_L12 = 4, GCFG_COUNT = 5 };
+bool group_cfg_supported(int d, int cfg);
+int group_cfg_tile_m(int cfg);   // rows of C per workgroup
+int group_cfg_tile_n(int cfg);   // columns of C per workgroup
+
+// Grouped launchers: items are int4 {operand, a, b, c} for the split phase and
+// int2 {problem, tile} for the GEMM phase.
+hipError_t launch_group_maxexp(const GroupOperand *ops, const int4 *items, int n, hipStream_t st);
+hipError_t launch_group_split(const GroupOperand *ops, const int4 *items, int n, int b, int d, hipStream_t st);
+hipError_t launch_group_gemm(const GemmLaunch *probs, const int2 *items, int n, int d, int cfg, hipStream_t st);
+
+// Synthetic (U - 0.5) 2^e data for timing runs.
+hipError_t launch_fill_random(double *p, int64_t n, uint32_t seed, hipStream_t st);
+
+// Shared by the C-ABI translation units (oz_api.cpp).
+int set_error(hipError_t e, const char *where);
+void set_error_msg(const char *msg);
+int64_t max_kchunk_blocks(int b, int d);
+
 // Kernel launchers (oz_kernels.hip).  All enqueue on `st` and return hipError_t.
 hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
 hipError_t launch_maxexp(const OperandView &op, int32_t *E, int64_t Rp, hipStream_t st);

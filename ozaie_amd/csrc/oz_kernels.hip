@@ -43,12 +43,8 @@ __device__ __forceinline__ int field_of(double x) {
 
 // ------------------------------------------------------------------ max exponent
 // Contraction axis contiguous (sk == 1): one wave per row, coalesced along k.
-__global__ __launch_bounds__(256) void k_maxexp_kc(const double *__restrict__ X, int64_t sr,
-                                                   int64_t R, int64_t K, int64_t Rp,
-                                                   int32_t *__restrict__ E) {
-    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (r >= Rp) return;  // whole wave leaves together
+__device__ __forceinline__ void maxexp_kc_row(const double *__restrict__ X, int64_t sr, int64_t R, int64_t K,
+                                              int32_t *__restrict__ E, int64_t r, int lane) {
     int m = 0;
     if (r < R) {
         const double *row = X + r * sr;
@@ -64,14 +60,19 @@ __global__ __launch_bounds__(256) void k_maxexp_kc(const double *__restrict__ X,
     if (lane == 0) E[r] = m;
 }
 
+__global__ __launch_bounds__(256) void k_maxexp_kc(const double *__restrict__ X, int64_t sr,
+                                                   int64_t R, int64_t K, int64_t Rp,
+                                                   int32_t *__restrict__ E) {
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= Rp) return;  // whole wave leaves together
+    maxexp_kc_row(X, sr, R, K, E, r, threadIdx.x & 63);
+}
+
 // Row index contiguous (sr == 1): thread per row, a slab of k per block row, combined
 // with atomicMax into E (zero-filled beforehand).
-__global__ __launch_bounds__(256) void k_maxexp_rc(const double *__restrict__ X, int64_t sk,
-                                                   int64_t R, int64_t K, int64_t kper,
-                                                   int32_t *__restrict__ E) {
-    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void maxexp_rc_slab(const double *__restrict__ X, int64_t sk, int64_t R, int64_t K,
+                                               int32_t *__restrict__ E, int64_t r, int64_t k0, int64_t kper) {
     if (r >= R) return;
-    const int64_t k0 = (int64_t)blockIdx.y * kper;
     const int64_t k1 = min(K, k0 + kper);
     int m = 0;
     int64_t k = k0;
@@ -82,6 +83,12 @@ __global__ __launch_bounds__(256) void k_maxexp_rc(const double *__restrict__ X,
     }
     for (; k < k1; k++) m = max(m, field_of(X[r + k * sk]));
     if (m > 0) atomicMax(E + r, m);
+}
+
+__global__ __launch_bounds__(256) void k_maxexp_rc(const double *__restrict__ X, int64_t sk,
+                                                   int64_t R, int64_t K, int64_t kper,
+                                                   int32_t *__restrict__ E) {
+    maxexp_rc_slab(X, sk, R, K, E, (int64_t)blockIdx.x * 256 + threadIdx.x, (int64_t)blockIdx.y * kper, kper);
 }
 
 hipError_t launch_maxexp(const OperandView &op, int32_t *E, int64_t Rp, hipStream_t st) {
@@ -112,15 +119,15 @@ hipError_t launch_maxexp(const OperandView &op, int32_t *E, int64_t Rp, hipStrea
 // tile is staged through LDS so that the global reads are coalesced along whichever
 // axis is contiguous; each thread then owns 16 consecutive k of one row and writes one
 // 16-byte lane chunk per slice plane (a wave writes one whole 1 KiB block per plane).
+constexpr int kSplitLds = 128 * 33 * 8;  // bytes of the FP64 staging tile (either orientation)
+
 template <bool KC>
-__global__ __launch_bounds__(256) void k_split(const double *__restrict__ X, int64_t sr,
-                                               int64_t sk, int64_t R, int64_t K,
-                                               const int32_t *__restrict__ E, int b, int d,
-                                               int8_t *__restrict__ S, int64_t Rb, int64_t Kb) {
+__device__ __forceinline__ void split_tile(const double *__restrict__ X, int64_t sr, int64_t sk, int64_t R,
+                                           int64_t K, const int32_t *__restrict__ E, int b, int d,
+                                           int8_t *__restrict__ S, int64_t Rb, int64_t Kb, int64_t rb,
+                                           int64_t kt, double *tile) {
     constexpr int TK = 128;
     constexpr int LDW = KC ? (TK + 1) : (32 + 1);
-    __shared__ double tile[KC ? 32 * LDW : TK * LDW];
-    const int64_t rb = blockIdx.y, kt = blockIdx.x;
     const int64_t r0 = rb * 32, k0 = kt * TK;
     const int t = threadIdx.x;
 #pragma unroll
@@ -176,6 +183,15 @@ __global__ __launch_bounds__(256) void k_split(const double *__restrict__ X, int
         }
         *reinterpret_cast<uint4 *>(dst + (int64_t)j * plane) = make_uint4(w[0], w[1], w[2], w[3]);
     }
+}
+
+template <bool KC>
+__global__ __launch_bounds__(256) void k_split(const double *__restrict__ X, int64_t sr,
+                                               int64_t sk, int64_t R, int64_t K,
+                                               const int32_t *__restrict__ E, int b, int d,
+                                               int8_t *__restrict__ S, int64_t Rb, int64_t Kb) {
+    __shared__ double tile[kSplitLds / 8];
+    split_tile<KC>(X, sr, sk, R, K, E, b, d, S, Rb, Kb, blockIdx.y, blockIdx.x, tile);
 }
 
 hipError_t launch_split(const OperandView &op, const int32_t *E, int b, int d, int8_t *S,
@@ -333,8 +349,7 @@ __device__ __forceinline__ void wave_epilogue(const GemmLaunch &g, int epi, int 
 }
 
 template <int D, int WM, int WN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
-void k_gemm_fused(GemmLaunch g, int epi) {
+__device__ __forceinline__ void gemm_lds_tile(const GemmLaunch &g, int tm, int tn, int epi) {
     using F = FusedCfg<D, WM, WN>;
     constexpr int BA = F::BA, BB = F::BB, NBLK = F::NBLK, NLD = F::NLD, NMIN = F::NMIN;
     __shared__ __attribute__((aligned(1024))) int8_t smem[2 * F::STAGE];
@@ -342,9 +357,6 @@ void k_gemm_fused(GemmLaunch g, int epi) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = w & 1, wn = w >> 1;
-    const int ntm = (g.M + 64 * WM - 1) / (64 * WM), ntn = (g.N + 64 * WN - 1) / (64 * WN);
-    int tm, tn;
-    tile_map(blockIdx.x, ntm, ntn, tm, tn);
     const int64_t rbA0 = (int64_t)tm * BA, rbB0 = (int64_t)tn * BB;
     const int64_t planeA = g.RbA * g.Kb * kBlkBytes, planeB = g.RbB * g.Kb * kBlkBytes;
 
@@ -418,21 +430,26 @@ void k_gemm_fused(GemmLaunch g, int epi) {
                              tn * 64 * WN + wn * 32 * WN);
 }
 
+template <int D, int WM, int WN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_gemm_fused(GemmLaunch g, int epi) {
+    const int ntm = (g.M + 64 * WM - 1) / (64 * WM), ntn = (g.N + 64 * WN - 1) / (64 * WN);
+    int tm, tn;
+    tile_map(blockIdx.x, ntm, ntn, tm, tn);
+    gemm_lds_tile<D, WM, WN>(g, tm, tn, epi);
+}
+
 // Direct variant: no LDS in the main loop.  Each wave streams its own A and B
 // fragments HBM/L2 -> VGPR (one 1 KiB block per wave instruction, 16 bytes per lane),
 // double-buffered in registers, so there is no barrier and the compiler's per-register
 // wait counts overlap the loads of block t+1 with the MFMAs of block t.  Requires an
 // even number of contraction blocks per pass (the planner pads K to 64).
 template <int D, int WM, int WN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
-void k_gemm_direct(GemmLaunch g, int epi) {
+__device__ __forceinline__ void gemm_direct_tile(const GemmLaunch &g, int tm, int tn, int epi) {
     __shared__ __attribute__((aligned(1024))) int8_t smem[4 * D * 1024];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = w & 1, wn = w >> 1;
-    const int ntm = (g.M + 64 * WM - 1) / (64 * WM), ntn = (g.N + 64 * WN - 1) / (64 * WN);
-    int tm, tn;
-    tile_map(blockIdx.x, ntm, ntn, tm, tn);
     const int64_t planeA = g.RbA * g.Kb * kBlkBytes, planeB = g.RbB * g.Kb * kBlkBytes;
     const int64_t rbA = (int64_t)tm * 2 * WM + wm * WM, rbB = (int64_t)tn * 2 * WN + wn * WN;
     const int8_t *pa = g.SA + rbA * g.Kb * kBlkBytes + lane * 16;
@@ -483,6 +500,15 @@ void k_gemm_direct(GemmLaunch g, int epi) {
                              tn * 64 * WN + wn * 32 * WN);
 }
 
+template <int D, int WM, int WN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_gemm_direct(GemmLaunch g, int epi) {
+    const int ntm = (g.M + 64 * WM - 1) / (64 * WM), ntn = (g.N + 64 * WN - 1) / (64 * WN);
+    int tm, tn;
+    tile_map(blockIdx.x, ntm, ntn, tm, tn);
+    gemm_direct_tile<D, WM, WN>(g, tm, tn, epi);
+}
+
 // Pipelined LDS variant: the slice blocks of block t+2 are copied HBM -> LDS
 // (global_load_lds) while block t computes; fragments are double-buffered in registers
 // and the workgroup barrier sits in the middle of each block's MFMAs.  After the
@@ -501,8 +527,7 @@ __host__ __device__ constexpr int pipe_split() {
 }
 
 template <int D, int WM, int WN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
-void k_gemm_pipe(GemmLaunch g, int epi) {
+__device__ __forceinline__ void gemm_pipe_tile(const GemmLaunch &g, int tm, int tn, int epi) {
     using F = FusedCfg<D, WM, WN>;
     constexpr int BA = F::BA, BB = F::BB, NBLK = F::NBLK, NLD = F::NLD, NMIN = F::NMIN;
     constexpr int I1 = pipe_split<D>();
@@ -511,9 +536,6 @@ void k_gemm_pipe(GemmLaunch g, int epi) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = w & 1, wn = w >> 1;
-    const int ntm = (g.M + 64 * WM - 1) / (64 * WM), ntn = (g.N + 64 * WN - 1) / (64 * WN);
-    int tm, tn;
-    tile_map(blockIdx.x, ntm, ntn, tm, tn);
     const int64_t rbA0 = (int64_t)tm * BA, rbB0 = (int64_t)tn * BB;
     const int64_t planeA = g.RbA * g.Kb * kBlkBytes, planeB = g.RbB * g.Kb * kBlkBytes;
 
@@ -607,6 +629,15 @@ void k_gemm_pipe(GemmLaunch g, int epi) {
     int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * (D * 256);
     wave_epilogue<D, WM, WN>(g, epi, 0, acc, scratch, lane, tm * 64 * WM + wm * 32 * WM,
                              tn * 64 * WN + wn * 32 * WN);
+}
+
+template <int D, int WM, int WN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_gemm_pipe(GemmLaunch g, int epi) {
+    const int ntm = (g.M + 64 * WM - 1) / (64 * WM), ntn = (g.N + 64 * WN - 1) / (64 * WN);
+    int tm, tn;
+    tile_map(blockIdx.x, ntm, ntn, tm, tn);
+    gemm_pipe_tile<D, WM, WN>(g, tm, tn, epi);
 }
 
 bool fused_supported(int d) { return d >= 1 && d <= 16; }
@@ -755,6 +786,116 @@ hipError_t launch_pack_paired(const int32_t *bins, int64_t M, int64_t N, int b, 
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_pack_paired, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, bins, M, N,
                        b, d, n0, Er, Ec, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ grouped launches
+// Many independent problems per launch: every block reads its work item (operand or
+// problem, and the block index inside it) from a table built on the host, then runs the
+// same tile body as the single-problem kernels.
+
+__global__ __launch_bounds__(256) void k_grp_maxexp(const GroupOperand *__restrict__ ops,
+                                                    const int4 *__restrict__ items) {
+    const int4 it = items[blockIdx.x];
+    const GroupOperand &o = ops[it.x];
+    if (o.sk == 1) {
+        const int64_t r = (int64_t)it.y * 4 + (threadIdx.x >> 6);
+        if (r < o.Rp) maxexp_kc_row(o.X, o.sr, o.R, o.K, o.E, r, threadIdx.x & 63);
+    } else {
+        maxexp_rc_slab(o.X, o.sk, o.R, o.K, o.E, (int64_t)it.y * 256 + threadIdx.x, (int64_t)it.z * it.w, it.w);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_grp_split(const GroupOperand *__restrict__ ops,
+                                                   const int4 *__restrict__ items, int b, int d) {
+    __shared__ double tile[kSplitLds / 8];
+    const int4 it = items[blockIdx.x];
+    const GroupOperand &o = ops[it.x];
+    if (o.sk == 1)
+        split_tile<true>(o.X, o.sr, o.sk, o.R, o.K, o.E, b, d, o.S, o.Rb, o.Kb, it.y, it.z, tile);
+    else
+        split_tile<false>(o.X, o.sr, o.sk, o.R, o.K, o.E, b, d, o.S, o.Rb, o.Kb, it.y, it.z, tile);
+}
+
+hipError_t launch_group_maxexp(const GroupOperand *ops, const int4 *items, int n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_grp_maxexp, dim3((unsigned)n), dim3(256), 0, st, ops, items);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_split(const GroupOperand *ops, const int4 *items, int n, int b, int d, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_grp_split, dim3((unsigned)n), dim3(256), 0, st, ops, items, b, d);
+    return hipGetLastError();
+}
+
+template <int D, int WM, int WN, int FAM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_grp_gemm(const GemmLaunch *__restrict__ probs, const int2 *__restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    const GemmLaunch g = probs[it.x];
+    const int ntn = (g.N + 64 * WN - 1) / (64 * WN);
+    const int tm = it.y / ntn, tn = it.y % ntn;
+    if (FAM == KERNEL_DIRECT) gemm_direct_tile<D, WM, WN>(g, tm, tn, EPI_C);
+    else gemm_lds_tile<D, WM, WN>(g, tm, tn, EPI_C);
+}
+
+int group_cfg_tile_m(int cfg) { return cfg == GCFG_D22 ? 128 : 64; }
+int group_cfg_tile_n(int cfg) { return (cfg == GCFG_D11 || cfg == GCFG_L11) ? 64 : 128; }
+
+bool group_cfg_supported(int d, int cfg) {
+    if (d != 3 && d != 7 && d != 8 && d != 9) return false;
+    if (cfg == GCFG_D22) return d == 3;
+    if (cfg == GCFG_D12) return d != 9;   // two fragment sets of 32 x 64 do not fit at d = 9
+    return cfg >= 0 && cfg < GCFG_COUNT;
+}
+
+template <int D>
+static hipError_t launch_grp_d(const GemmLaunch *probs, const int2 *items, int n, int cfg, hipStream_t st) {
+    const dim3 grid((unsigned)n), blk(256);
+    switch (cfg) {
+        case GCFG_D11: hipLaunchKernelGGL((k_grp_gemm<D, 1, 1, KERNEL_DIRECT>), grid, blk, 0, st, probs, items); break;
+        case GCFG_D12:
+            if constexpr (D <= 8) hipLaunchKernelGGL((k_grp_gemm<D, 1, 2, KERNEL_DIRECT>), grid, blk, 0, st, probs, items);
+            break;
+        case GCFG_D22:
+            if constexpr (D <= 4) hipLaunchKernelGGL((k_grp_gemm<D, 2, 2, KERNEL_DIRECT>), grid, blk, 0, st, probs, items);
+            break;
+        case GCFG_L11: hipLaunchKernelGGL((k_grp_gemm<D, 1, 1, KERNEL_LDS>), grid, blk, 0, st, probs, items); break;
+        case GCFG_L12: hipLaunchKernelGGL((k_grp_gemm<D, 1, 2, KERNEL_LDS>), grid, blk, 0, st, probs, items); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_group_gemm(const GemmLaunch *probs, const int2 *items, int n, int d, int cfg, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (!group_cfg_supported(d, cfg)) return hipErrorInvalidValue;
+    switch (d) {
+        case 3: return launch_grp_d<3>(probs, items, n, cfg, st);
+        case 7: return launch_grp_d<7>(probs, items, n, cfg, st);
+        case 8: return launch_grp_d<8>(probs, items, n, cfg, st);
+        case 9: return launch_grp_d<9>(probs, items, n, cfg, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// ------------------------------------------------------------------ synthetic data
+// (U - 0.5) * 2^e with a per-element xorshift; used by the autotuner so that timings see
+// the operand toggling of real data (int8 MFMA throughput depends on it).
+__global__ void k_fill_random(double *__restrict__ p, int64_t n, uint32_t seed) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t x = seed ^ (uint32_t)(i * 2654435761u) ^ (uint32_t)(i >> 32);
+    x ^= x << 13; x ^= x >> 17; x ^= x << 5;
+    x ^= x << 13; x ^= x >> 17; x ^= x << 5;
+    const double u = (double)(x >> 8) * (1.0 / 16777216.0) - 0.5;
+    p[i] = ldexp(u, (int)((x & 7u)) - 4);
+}
+
+hipError_t launch_fill_random(double *p, int64_t n, uint32_t seed, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_random, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, seed);
     return hipGetLastError();
 }
 

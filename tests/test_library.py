@@ -111,3 +111,35 @@ def test_reference_example_compiles_unchanged(tmp_path):
                            "-L" + os.path.dirname(oz.LIB_PATH), "-lozaie",
                            "-Wl,-rpath," + os.path.dirname(oz.LIB_PATH)])
     assert exe.exists()
+
+
+def test_exact_entries_vs_rationals_and_mpfr():
+    """The accuracy meter (TwoProduct + exact summation) returns the correctly rounded
+    exact dot product: checked against Python fractions and against MPFR at 2000 bits."""
+    from fractions import Fraction
+    import numpy as np
+    import oracle
+    rng = np.random.default_rng(0)
+    A = (rng.random((12, 257)) - 0.5) * np.exp(3 * rng.standard_normal((12, 257)))
+    B = (rng.random((257, 9)) - 0.5) * np.exp(3 * rng.standard_normal((257, 9)))
+    A[0, :] = 1e300
+    B[:, 0] = 1e-300
+    A[1, :4] = [1.0, 1e16, -1e16, 3.0]
+    rows = np.repeat(np.arange(12), 9)
+    cols = np.tile(np.arange(9), 12)
+    got = oz.exact_entries(A, B, rows, cols)
+    for i, (r, c) in enumerate(zip(rows, cols)):
+        ex = sum((Fraction(float(x)) * Fraction(float(y)) for x, y in zip(A[r], B[:, c])), Fraction(0))
+        assert got[i] == float(ex)
+    if oracle.have_mpfr():
+        assert np.array_equal(got, oracle.mpfr_sample(A, B, rows, cols, 2000))
+
+
+@pytest.mark.skipif(has_gpu(), reason="checks the no-device path")
+def test_group_without_device():
+    import ctypes
+    import numpy as np
+    M = np.array([64, 32], dtype=np.int64)
+    h = ctypes.c_void_p()
+    rc = oz.lib().ozaie_group_create(ctypes.byref(h), 2, oz._ptr(M), oz._ptr(M), oz._ptr(M), 7, 8, 0)
+    assert rc == oz.ERR_NODEVICE
