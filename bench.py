@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-target", action="store_true", help="skip the 8192^3 section")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="only run each preset's GEMM twice and one 1 GiB device copy (for rocprofv3 --pmc)")
     return ap.parse_args()
 
 
@@ -191,6 +193,25 @@ def cpu_model():
     return "unknown"
 
 
+def profile_run(args, torch, oz, n, slices):
+    """Minimal workload for counter collection: two calls per preset (the second is the
+    one to read) and a plain copy of 1 GiB, whose FETCH_SIZE / WRITE_SIZE calibrate the
+    counters against a known byte count."""
+    A, B = gen_inputs(torch, n, args.phi, args.seed, torch.device("cuda:0"))
+    C = torch.empty(n, n, dtype=torch.float64, device="cuda:0")
+    for d in slices:
+        p = oz.Plan(n, n, n, 7, d)
+        p(A, B, C)
+        p(A, B, C)
+        torch.cuda.synchronize()
+        p.close()
+    x = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print("profile run done", file=sys.stderr)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -205,6 +226,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     slices = [int(x) for x in args.slices.split(",")]
     n = args.size
+    if args.profile_run:
+        profile_run(args, torch, oz, n, slices)
+        return
     main_res, (Ah, Bh) = measure(args, torch, np, oz, n, args.steps, args.warmup, args.block, slices)
     m = main_res["matched"]
     d = m["slices"]

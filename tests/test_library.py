@@ -143,3 +143,17 @@ def test_group_without_device():
     h = ctypes.c_void_p()
     rc = oz.lib().ozaie_group_create(ctypes.byref(h), 2, oz._ptr(M), oz._ptr(M), oz._ptr(M), 7, 8, 0)
     assert rc == oz.ERR_NODEVICE
+
+
+def test_sweep_mode_table_and_resume_keys(tmp_path):
+    from ozaie_amd import sweep
+    assert sweep.mode_bd("accurate", 128) == (3, 29)
+    assert sweep.mode_bd("fast", 4096) == (7, 8)
+    assert sweep.mode_bd("b6d11", 512) == (6, 11)
+    out = tmp_path / "r.json"
+    rec = {"kind": "accuracy", "K": 128, "phi": 1.0, "seed": 0, "mode": "fast", "max_rel_err": 1e-10}
+    sweep.append(str(out), rec)
+    loaded = sweep.load_records(str(out))
+    assert loaded == [rec]
+    assert sweep.record_key(loaded[0]) == sweep.record_key({"kind": "accuracy", "K": 128, "phi": 1.0,
+                                                            "seed": 0, "mode": "fast"})
