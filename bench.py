@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--block", type=int, default=256, help="side of the sampled error block")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-target", action="store_true", help="skip the 8192^3 section")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the long-K, grouped and autograd sections")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--profile-run", action="store_true",
                     help="only run each preset's GEMM twice and one 1 GiB device copy (for rocprofv3 --pmc)")
@@ -212,6 +214,151 @@ def profile_run(args, torch, oz, n, slices):
     print("profile run done", file=sys.stderr)
 
 
+def long_k(args, torch, np, oz, slices=(7, 8, 9)):
+    """8192 x 8192 x 32768, phi = 2: every preset needs several int32-safe contraction
+    passes summed exactly in int128.  Error on 4 sampled rows x 64 columns."""
+    M = N = 8192
+    K = 32768
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(args.seed + 3)
+    A = (torch.rand(M, K, dtype=torch.float64, device="cuda:0", generator=g) - 0.5) * torch.exp(
+        2.0 * torch.randn(M, K, dtype=torch.float64, device="cuda:0", generator=g))
+    B = (torch.rand(K, N, dtype=torch.float64, device="cuda:0", generator=g) - 0.5) * torch.exp(
+        2.0 * torch.randn(K, N, dtype=torch.float64, device="cuda:0", generator=g))
+    C = torch.empty(M, N, dtype=torch.float64, device="cuda:0")
+    flop = 2.0 * M * N * K
+    t_nat = time_steps(torch, lambda: torch.matmul(A, B, out=C), 3, 1)
+    rng = np.random.default_rng(args.seed)
+    rows = np.repeat(rng.choice(M, 4, replace=False), 64)
+    cols = np.tile(rng.choice(N, 64, replace=False), 4)
+    Ah = A.cpu().numpy()
+    Bh = B.cpu().numpy()
+    exact = oz.exact_entries(Ah, Bh, rows, cols)
+    del Ah, Bh
+    ri = torch.from_numpy(rows).cuda()
+    ci = torch.from_numpy(cols).cuda()
+    nat_err = rel_errors(np, C[ri, ci].cpu().numpy(), exact)
+    rec = {"workload": "DGEMM M=N=8192 K=32768, phi=2", "native": {"ms": t_nat, "tflops": flop / t_nat / 1e9,
+           "max_rel_err": nat_err[0], "median_rel_err": nat_err[1]}, "presets": []}
+    for d in slices:
+        p = oz.Plan(M, N, K, 7, d)
+        t = time_steps(torch, lambda: p(A, B, C), 3, 1)
+        err = rel_errors(np, C[ri, ci].cpu().numpy(), exact)
+        rec["presets"].append({"bits": 7, "slices": d, "ms": t, "tflops": flop / t / 1e9, "speedup": t_nat / t,
+                               "max_rel_err": err[0], "median_rel_err": err[1], "passes": p.info()["passes"]})
+        log(args, f"  longK d={d}: {t:.1f} ms x{t_nat / t:.2f} err {err} passes {p.info()['passes']}")
+        p.close()
+    log(args, f"  longK native {t_nat:.1f} ms err {nat_err}")
+    del A, B, C
+    torch.cuda.empty_cache()
+    return rec
+
+
+def grouped(args, torch, np, oz, slices=(3, 7, 8, 9), count=512):
+    """512 independent products with M, N, K log-uniform in [32, 2048], phi = 1, in one
+    captured graph per preset, against native DGEMM called per problem."""
+    rng = np.random.default_rng(args.seed + 4)
+    shapes = [tuple(int(v) for v in row) for row in
+              np.exp(rng.uniform(np.log(32), np.log(2048), size=(count, 3))).astype(np.int64)]
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(args.seed + 4)
+    As, Bs, Cs = [], [], []
+    for m, n, k in shapes:
+        As.append((torch.rand(m, k, dtype=torch.float64, device="cuda:0", generator=g) - 0.5)
+                  * torch.exp(torch.randn(m, k, dtype=torch.float64, device="cuda:0", generator=g)))
+        Bs.append((torch.rand(k, n, dtype=torch.float64, device="cuda:0", generator=g) - 0.5)
+                  * torch.exp(torch.randn(k, n, dtype=torch.float64, device="cuda:0", generator=g)))
+        Cs.append(torch.empty(m, n, dtype=torch.float64, device="cuda:0"))
+    flop = sum(2.0 * m * n * k for m, n, k in shapes)
+
+    def native():
+        for a, b, c in zip(As, Bs, Cs):
+            torch.matmul(a, b, out=c)
+    t_nat = time_steps(torch, native, 3, 2)
+    # the same native loop replayed from a captured graph (no per-launch host cost)
+    t_nat_graph = None
+    try:
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            native()
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        cg = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(cg):
+            native()
+        t_nat_graph = time_steps(torch, cg.replay, 5, 2)
+        del cg
+    except Exception as e:  # graph capture of the library DGEMM is optional
+        log(args, f"  native graph capture unavailable: {e!r}")
+    # error: 8 problems, 32 sampled entries each
+    pick = rng.choice(count, 8, replace=False)
+    samples = []
+    for i in pick:
+        m, n, k = shapes[i]
+        r = rng.integers(0, m, 32)
+        c = rng.integers(0, n, 32)
+        samples.append((i, r, c, oz.exact_entries(As[i].cpu().numpy(), Bs[i].cpu().numpy(), r, c)))
+
+    def errs():
+        got = np.concatenate([Cs[i][torch.from_numpy(r).cuda(), torch.from_numpy(c).cuda()].cpu().numpy()
+                              for i, r, c, _ in samples])
+        ref = np.concatenate([e for _, _, _, e in samples])
+        return rel_errors(np, got, ref)
+    nat_err = errs()
+    rec = {"workload": f"{count} DGEMMs, M,N,K log-uniform in [32, 2048], phi=1",
+           "total_gflop": flop / 1e9,
+           "native": {"ms": t_nat, "tflops": flop / t_nat / 1e9, "max_rel_err": nat_err[0],
+                      "median_rel_err": nat_err[1], "launches": count,
+                      "graph_ms": t_nat_graph,
+                      "graph_tflops": flop / t_nat_graph / 1e9 if t_nat_graph else None}, "presets": []}
+    for d in slices:
+        grp = oz.Group(shapes, 7, d)
+        t = time_steps(torch, lambda: grp(As, Bs, Cs), 5, 3)
+        err = errs()
+        q = grp.query()
+        rec["presets"].append({"bits": 7, "slices": d, "ms": t, "tflops": flop / t / 1e9, "speedup": t_nat / t,
+                               "speedup_vs_native_graph": (t_nat_graph / t) if t_nat_graph else None,
+                               "max_rel_err": err[0], "median_rel_err": err[1], "launches": q["launches"],
+                               "graph": q["graph"], "workgroups": q["workgroups"]})
+        log(args, f"  grouped d={d}: {t:.2f} ms x{t_nat / t:.2f} err {err} launches {q['launches']}")
+        grp.close()
+    log(args, f"  grouped native {t_nat:.2f} ms (graph {t_nat_graph}) err {nat_err}")
+    del As, Bs, Cs
+    torch.cuda.empty_cache()
+    return rec
+
+
+def autograd_section(args, torch, np, oz, n=2048, slices=9):
+    """Forward + backward of loss = sum(C * W), C = A B, 2048^3, phi = 0.5: emulated
+    (dA = dC B^T, dB = A^T dC as emulated GEMMs) against FP64 torch.matmul."""
+    from ozaie_amd.autograd import ozaki_matmul
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(args.seed + 5)
+    A = ((torch.rand(n, n, dtype=torch.float64, device="cuda:0", generator=g) - 0.5)
+         * torch.exp(0.5 * torch.randn(n, n, dtype=torch.float64, device="cuda:0", generator=g))).requires_grad_(True)
+    B = ((torch.rand(n, n, dtype=torch.float64, device="cuda:0", generator=g) - 0.5)
+         * torch.exp(0.5 * torch.randn(n, n, dtype=torch.float64, device="cuda:0", generator=g))).requires_grad_(True)
+    W = torch.randn(n, n, dtype=torch.float64, device="cuda:0", generator=g)
+
+    def step(fn):
+        A.grad = None
+        B.grad = None
+        (fn(A, B) * W).sum().backward()
+    t_nat = time_steps(torch, lambda: step(torch.matmul), 3, 2)
+    dA_ref, dB_ref = A.grad.clone(), B.grad.clone()
+    t_emu = time_steps(torch, lambda: step(lambda a, b: ozaki_matmul(a, b, 7, slices)), 3, 2)
+    rel = lambda x, y: float(((x - y).abs().max() / y.abs().max()).item())
+    rec = {"workload": f"loss=sum((A@B)*W), {n}^3, phi=0.5, forward+backward", "slices": slices,
+           "native_ms": t_nat, "emulated_ms": t_emu, "speedup": t_nat / t_emu,
+           "effective_tflops": 3 * 2.0 * n ** 3 / t_emu / 1e9,
+           "grad_rel_err_vs_fp64": {"dA": rel(A.grad, dA_ref), "dB": rel(B.grad, dB_ref)}}
+    log(args, f"  autograd: native {t_nat:.2f} ms emulated {t_emu:.2f} ms {rec['grad_rel_err_vs_fp64']}")
+    del A, B, W
+    torch.cuda.empty_cache()
+    return rec
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -271,6 +418,10 @@ def main():
         line["target_8192"] = {"native_dgemm": tgt["native"], "presets": tgt["presets"],
                                "matched": tgt["matched"],
                                "speedup_vs_native": tgt["matched"]["speedup"]}
+    if not args.no_extra:
+        line["long_k"] = long_k(args, torch, np, oz)
+        line["grouped_512"] = grouped(args, torch, np, oz)
+        line["autograd_2048"] = autograd_section(args, torch, np, oz)
     print(json.dumps(line), flush=True)
 
 

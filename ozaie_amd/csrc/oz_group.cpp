@@ -71,7 +71,7 @@ struct ozaie_group_s {
     GemmLaunch *d_probs = nullptr;
     int4 *d_items_max = nullptr, *d_items_split = nullptr;
     int n_max = 0, n_split = 0;
-    int2 *d_items_gemm[GCFG_COUNT] = {};
+    int4 *d_items_gemm[GCFG_COUNT] = {};
     int n_gemm[GCFG_COUNT] = {};
     int64_t tiles = 0;
     // host mirrors of the descriptors (pinned) and the pointers they hold
@@ -149,7 +149,7 @@ int group_build(ozaie_group g, int force_cfg) {
     std::memset(g->h_ops, 0, sizeof(GroupOperand) * 2 * n);
     std::memset(g->h_probs, 0, sizeof(GemmLaunch) * n);
     std::vector<int4> imax, isplit;
-    std::vector<int2> igemm[GCFG_COUNT];
+    std::vector<int4> igemm[GCFG_COUNT];
     int64_t eoff = 0, soff = e_bytes;
     for (int p = 0; p < n; p++) {
         GroupOperand &a = g->h_ops[2 * p], &bo = g->h_ops[2 * p + 1];
@@ -183,7 +183,11 @@ int group_build(ozaie_group g, int force_cfg) {
         const int c = g->cfg[p];
         const int64_t ntm = (g->M[p] + group_cfg_tile_m(c) - 1) / group_cfg_tile_m(c);
         const int64_t ntn = (g->N[p] + group_cfg_tile_n(c) - 1) / group_cfg_tile_n(c);
-        for (int64_t t = 0; t < ntm * ntn; t++) igemm[c].push_back(int2{p, (int)t});
+        for (int64_t t = 0; t < ntm * ntn; t++) {
+            int tm, tn;
+            band_map((int)t, (int)ntm, (int)ntn, tm, tn);
+            igemm[c].push_back(int4{p, tm, tn, 0});
+        }
         g->tiles += ntm * ntn;
     }
     int rc;

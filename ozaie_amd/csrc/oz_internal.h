@@ -74,10 +74,20 @@ int group_cfg_tile_m(int cfg);   // rows of C per workgroup
 int group_cfg_tile_n(int cfg);   // columns of C per workgroup
 
 // Grouped launchers: items are int4 {operand, a, b, c} for the split phase and
-// int2 {problem, tile} for the GEMM phase.
+// int4 {problem, tile row, tile column, 0} for the GEMM phase.
 hipError_t launch_group_maxexp(const GroupOperand *ops, const int4 *items, int n, hipStream_t st);
 hipError_t launch_group_split(const GroupOperand *ops, const int4 *items, int n, int b, int d, hipStream_t st);
-hipError_t launch_group_gemm(const GemmLaunch *probs, const int2 *items, int n, int d, int cfg, hipStream_t st);
+hipError_t launch_group_gemm(const GemmLaunch *probs, const int4 *items, int n, int d, int cfg, hipStream_t st);
+// Tile order inside a problem: bands of 8 tile-rows, so tiles that run together share
+// rows of A (device tile mapping and the grouped planner).
+__host__ __device__ inline void band_map(int lin, int ntm, int ntn, int &tm, int &tn) {
+    constexpr int G = 8;
+    const int band = lin / (G * ntn);
+    const int inb = lin % (G * ntn);
+    const int gh = (ntm - band * G) < G ? (ntm - band * G) : G;
+    tm = band * G + inb % gh;
+    tn = inb / gh;
+}
 
 // Synthetic (U - 0.5) 2^e data for timing runs.
 hipError_t launch_fill_random(double *p, int64_t n, uint32_t seed, hipStream_t st);

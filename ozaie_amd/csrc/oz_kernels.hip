@@ -278,19 +278,15 @@ __device__ __forceinline__ void epilogue_elem(const GemmLaunch &g, int epi, int 
 // Block -> tile mapping.  Workgroups are dispatched round-robin over the 8 XCDs; give
 // each XCD a contiguous range of tiles, and walk that range in bands of 8 tile-rows so
 // the concurrently running tiles of one XCD share A rows and B columns in its L2.
-__device__ __forceinline__ void tile_map(int bid, int ntm, int ntn, int &tm, int &tn) {
-    const int total = ntm * ntn;
+__device__ __forceinline__ int xcd_remap(int bid, int total) {
     const int q = total / 8, rem = total % 8;
     const int xcd = bid % 8, idx = bid / 8;
-    const int lin = (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + idx;
-    constexpr int G = 8;
-    const int band = lin / (G * ntn);
-    const int inb = lin % (G * ntn);
-    const int gh = min(G, ntm - band * G);
-    tm = band * G + inb % gh;
-    tn = inb / gh;
+    return (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + idx;
 }
 
+__device__ __forceinline__ void tile_map(int bid, int ntm, int ntn, int &tm, int &tn) {
+    band_map(xcd_remap(bid, ntm * ntn), ntm, ntn, tm, tn);
+}
 
 template <int N>
 __device__ __forceinline__ void s_waitcnt_vm() {
@@ -831,13 +827,12 @@ hipError_t launch_group_split(const GroupOperand *ops, const int4 *items, int n,
 
 template <int D, int WM, int WN, int FAM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
-void k_grp_gemm(const GemmLaunch *__restrict__ probs, const int2 *__restrict__ items) {
-    const int2 it = items[blockIdx.x];
+void k_grp_gemm(const GemmLaunch *__restrict__ probs, const int4 *__restrict__ items) {
+    // consecutive items (tiles of one problem in bands of tile-rows) go to one XCD
+    const int4 it = items[xcd_remap(blockIdx.x, gridDim.x)];
     const GemmLaunch g = probs[it.x];
-    const int ntn = (g.N + 64 * WN - 1) / (64 * WN);
-    const int tm = it.y / ntn, tn = it.y % ntn;
-    if (FAM == KERNEL_DIRECT) gemm_direct_tile<D, WM, WN>(g, tm, tn, EPI_C);
-    else gemm_lds_tile<D, WM, WN>(g, tm, tn, EPI_C);
+    if (FAM == KERNEL_DIRECT) gemm_direct_tile<D, WM, WN>(g, it.y, it.z, EPI_C);
+    else gemm_lds_tile<D, WM, WN>(g, it.y, it.z, EPI_C);
 }
 
 int group_cfg_tile_m(int cfg) { return cfg == GCFG_D22 ? 128 : 64; }
@@ -851,7 +846,7 @@ bool group_cfg_supported(int d, int cfg) {
 }
 
 template <int D>
-static hipError_t launch_grp_d(const GemmLaunch *probs, const int2 *items, int n, int cfg, hipStream_t st) {
+static hipError_t launch_grp_d(const GemmLaunch *probs, const int4 *items, int n, int cfg, hipStream_t st) {
     const dim3 grid((unsigned)n), blk(256);
     switch (cfg) {
         case GCFG_D11: hipLaunchKernelGGL((k_grp_gemm<D, 1, 1, KERNEL_DIRECT>), grid, blk, 0, st, probs, items); break;
@@ -868,7 +863,7 @@ static hipError_t launch_grp_d(const GemmLaunch *probs, const int2 *items, int n
     return hipGetLastError();
 }
 
-hipError_t launch_group_gemm(const GemmLaunch *probs, const int2 *items, int n, int d, int cfg, hipStream_t st) {
+hipError_t launch_group_gemm(const GemmLaunch *probs, const int4 *items, int n, int d, int cfg, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     if (!group_cfg_supported(d, cfg)) return hipErrorInvalidValue;
     switch (d) {
