@@ -112,7 +112,16 @@ int32_t ozaie_plan_phase_times(ozaie_plan plan, float *out3);
  * class is untuned); other slice counts run one plan per problem.  Every K_i must fit
  * one contraction pass.  The launches are captured once in a hipGraph (flags bit 1
  * disables the graph) and replayed by ozaie_group_run; descriptors are re-uploaded only
- * when the pointers change.  flags bit 0: nearest-even rounding.                      */
+ * when the pointers change.  flags bit 0: nearest-even rounding.
+ * flags bit 2 (OZAIE_GROUP_PERSISTENT): slice counts 3, 7, 8, 9 run as ONE persistent
+ * launch (one workgroup per CU: exponent max + split of every operand, a grid barrier,
+ * then every GEMM tile, balanced per XCD on the host) with one tile shape for the group.
+ * flags bits 4..7: persistent tile configuration + 1 (0: per-d default; configuration
+ * ids as for ozaie_autotune, direct-load tiles only).
+ * Runs of one group must not overlap (they share the workspace and the control block). */
+#define OZAIE_GROUP_NEAREST 1
+#define OZAIE_GROUP_NO_GRAPH 2
+#define OZAIE_GROUP_PERSISTENT 4
 typedef struct ozaie_group_s *ozaie_group;
 int32_t ozaie_group_create(ozaie_group *group, int count, const int64_t *M, const int64_t *N,
                            const int64_t *K, int bits, int slices, int flags);
@@ -124,6 +133,11 @@ int32_t ozaie_group_run(ozaie_group group, const double *const *A, const double 
 int32_t ozaie_group_query(ozaie_group group, int32_t *out4);
 /* Tile configuration chosen for each problem (-1 in the per-problem fallback).        */
 int32_t ozaie_group_configs(ozaie_group group, int32_t *cfg_out);
+/* Waits for the group's last run and reports (then clears) the persistent kernel's error
+ * flag: 1 if a grid-barrier wait exceeded its bound (results of that run are invalid;
+ * this happens only when the grid could not be resident at once).  0 otherwise, and
+ * always 0 for the multi-launch modes.                                                 */
+int32_t ozaie_group_status(ozaie_group group, int32_t *error_out);
 int32_t ozaie_group_destroy(ozaie_group group);
 /* One-shot grouped product without a graph; blocking on `stream`.                     */
 int32_t ozaie_dgemm_grouped(int count, const int64_t *M, const int64_t *N, const int64_t *K,

@@ -55,7 +55,7 @@ EXPORTS = [
     "ozaie_plan_query", "ozaie_dgemm", "ozaie_debug_split", "ozaie_debug_bins",
     "ozaie_paired_output", "ozaie_last_error", "ozaie_version", "ozaie_plan_phase_times",
     "ozaie_exact_entries", "ozaie_group_create", "ozaie_group_run", "ozaie_group_query",
-    "ozaie_group_configs", "ozaie_group_destroy", "ozaie_dgemm_grouped", "ozaie_autotune",
+    "ozaie_group_configs", "ozaie_group_status", "ozaie_group_destroy", "ozaie_dgemm_grouped", "ozaie_autotune",
     "ozaie_tune_get", "ozaie_tune_set",
 ]
 
@@ -134,6 +134,8 @@ def _declare(L):
     L.ozaie_group_query.restype = i32
     L.ozaie_group_configs.argtypes = [P, P]
     L.ozaie_group_configs.restype = i32
+    L.ozaie_group_status.argtypes = [P, P]
+    L.ozaie_group_status.restype = i32
     L.ozaie_group_destroy.argtypes = [P]
     L.ozaie_group_destroy.restype = i32
     L.ozaie_dgemm_grouped.argtypes = [c_int, P, P, P, P, P, P, c_int, c_int, P]
@@ -341,18 +343,23 @@ def exact_entries(A, B, rows, cols):
 
 class Group:
     """Many independent products C_i = A_i @ B_i (float64 device tensors, row-major) with
-    b bits and d slices each, replayed through one captured hipGraph."""
+    b bits and d slices each, replayed through one captured hipGraph.  persistent=True
+    runs the whole group as one persistent kernel launch (slice counts 3, 7, 8, 9)."""
 
     GRAPH = 0
     NO_GRAPH = 2
+    PERSISTENT = 4
 
-    def __init__(self, shapes, bits=7, slices=8, rounding=ROUND_TRUNCATE, graph=True):
+    def __init__(self, shapes, bits=7, slices=8, rounding=ROUND_TRUNCATE, graph=True, persistent=False,
+                 persistent_cfg=None):
         self.shapes = [(int(m), int(n), int(k)) for m, n, k in shapes]
         cnt = len(self.shapes)
         M = np.array([s[0] for s in self.shapes], dtype=np.int64)
         N = np.array([s[1] for s in self.shapes], dtype=np.int64)
         K = np.array([s[2] for s in self.shapes], dtype=np.int64)
-        flags = (1 if rounding == ROUND_NEAREST else 0) | (0 if graph else 2)
+        flags = (1 if rounding == ROUND_NEAREST else 0) | (0 if graph else 2) | (4 if persistent else 0)
+        if persistent_cfg is not None:
+            flags |= (int(persistent_cfg) + 1) << 4
         h = ctypes.c_void_p()
         _check(lib().ozaie_group_create(ctypes.byref(h), cnt, _ptr(M), _ptr(N), _ptr(K), int(bits),
                                         int(slices), flags), "ozaie_group_create")
@@ -380,6 +387,13 @@ class Group:
         out = np.zeros(4, dtype=np.int32)
         _check(lib().ozaie_group_query(self._h, _ptr(out)), "ozaie_group_query")
         return dict(launches=int(out[0]), graph=bool(out[1]), fallback=bool(out[2]), workgroups=int(out[3]))
+
+    def status(self):
+        """Wait for the last run; 1 if the persistent kernel's barrier wait overran its bound
+        (that run's results are invalid), else 0.  Clears the flag."""
+        out = np.zeros(1, dtype=np.int32)
+        _check(lib().ozaie_group_status(self._h, _ptr(out)), "ozaie_group_status")
+        return int(out[0])
 
     def configs(self):
         out = np.zeros(len(self.shapes), dtype=np.int32)

@@ -84,6 +84,12 @@ struct ozaie_group_s {
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // persistent single-launch mode (flags bit 2)
+    bool persist = false;
+    int pcfg = 0;
+    char *pbuf = nullptr;                  // item lists + control block
+    PersistLaunch PL{};
+    hipStream_t last_stream = nullptr;
 };
 
 namespace {
@@ -96,6 +102,7 @@ void group_free(ozaie_group g) {
     if (g->cap_stream) (void)hipStreamDestroy(g->cap_stream);
     if (g->upload_done) (void)hipEventDestroy(g->upload_done);
     if (g->ws) (void)hipFree(g->ws);
+    if (g->pbuf) (void)hipFree(g->pbuf);
     if (g->d_ops) (void)hipFree(g->d_ops);
     if (g->d_probs) (void)hipFree(g->d_probs);
     if (g->d_items_max) (void)hipFree(g->d_items_max);
@@ -121,6 +128,130 @@ int upload(T **dptr, const std::vector<T> &v) {
     return OZAIE_OK;
 }
 
+// Default tile of the persistent kernel per slice count (registers allow 64 x 128 tiles
+// with prefetch for d = 3 and 7 only; 128 x 128 at d = 3).
+int default_persist_cfg(int d) { return d == 3 ? GCFG_D22 : (d == 7 ? GCFG_D12 : GCFG_D11); }
+
+// Longest-processing-time assignment of items (in the given order) to the workgroups in
+// `wgs`: each item goes to the least loaded one.  Appends to per-workgroup lists.
+void lpt_assign(const std::vector<int64_t> &cost, const std::vector<int> &order, const std::vector<int> &wgs,
+                std::vector<std::vector<int>> &lists) {
+    if (wgs.empty()) return;
+    typedef std::pair<int64_t, int> Slot;   // (load, workgroup)
+    std::vector<Slot> heap;
+    for (int w : wgs) heap.push_back(Slot(0, w));
+    auto cmp = [](const Slot &a, const Slot &b) { return a.first != b.first ? a.first > b.first : a.second > b.second; };
+    std::make_heap(heap.begin(), heap.end(), cmp);
+    for (int i : order) {
+        std::pop_heap(heap.begin(), heap.end(), cmp);
+        Slot &s = heap.back();
+        lists[s.second].push_back(i);
+        s.first += cost[i];
+        std::push_heap(heap.begin(), heap.end(), cmp);
+    }
+}
+
+// Item lists of the persistent kernel.  Phase A: one item per 32-row block of every
+// operand, costed by its contraction length, balanced over all waves (four per workgroup).  Phase B: the
+// tiles of every problem in band order, problems by decreasing tile cost; the tile
+// sequence is cut into 8 contiguous parts of equal cost, one per XCD (workgroup w runs on
+// XCD w mod 8), and each part is balanced over that XCD's workgroups.
+int build_persist(ozaie_group g, const std::vector<int64_t> &Kp) {
+    const int n = g->count;
+    int dev = 0, ncu = 0;
+    GCHK(hipGetDevice(&dev));
+    GCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int G = std::max(1, ncu);
+    // phase A
+    std::vector<int4> itA;
+    std::vector<int64_t> costA;
+    for (int p = 0; p < n; p++)
+        for (int op = 0; op < 2; op++) {
+            const GroupOperand &o = g->h_ops[2 * p + op];
+            for (int64_t rb = 0; rb < o.Rb; rb++) {
+                itA.push_back(int4{2 * p + op, (int)rb, 0, 0});
+                costA.push_back(o.K + 256);
+            }
+        }
+    std::vector<int> orderA(itA.size());
+    for (size_t i = 0; i < orderA.size(); i++) orderA[i] = (int)i;
+    std::stable_sort(orderA.begin(), orderA.end(), [&](int a, int b) { return costA[a] > costA[b]; });
+    // phase A runs one item per wave: slot 4 w + v is wave v of workgroup w
+    std::vector<int> slots(4 * G);
+    for (int w = 0; w < 4 * G; w++) slots[w] = w;
+    std::vector<std::vector<int>> listsA(4 * G), listsB(G);
+    lpt_assign(costA, orderA, slots, listsA);
+    // phase B
+    const int tm_rows = group_cfg_tile_m(g->pcfg), tn_cols = group_cfg_tile_n(g->pcfg);
+    const int64_t area = (int64_t)(tm_rows / 64) * (tn_cols / 64);
+    std::vector<int> porder(n);
+    for (int p = 0; p < n; p++) porder[p] = p;
+    std::stable_sort(porder.begin(), porder.end(), [&](int a, int b) { return Kp[a] > Kp[b]; });
+    std::vector<int4> itB;
+    std::vector<int64_t> costB;
+    for (int p : porder) {
+        const int64_t ntm = (g->M[p] + tm_rows - 1) / tm_rows, ntn = (g->N[p] + tn_cols - 1) / tn_cols;
+        for (int64_t t = 0; t < ntm * ntn; t++) {
+            int tm, tn;
+            band_map((int)t, (int)ntm, (int)ntn, tm, tn);
+            itB.push_back(int4{p, tm, tn, 0});
+            costB.push_back((Kp[p] / kBlk + 4) * area);   // contraction blocks + epilogue
+        }
+    }
+    g->tiles = (int64_t)itB.size();
+    int64_t total = 0;
+    for (int64_t c : costB) total += c;
+    const int nx = (G % 8 == 0) ? 8 : 1;
+    size_t pos = 0;
+    int64_t acc = 0;
+    for (int x = 0; x < nx; x++) {
+        std::vector<int> part;
+        const int64_t target = (x == nx - 1) ? INT64_MAX : total * (x + 1) / nx;
+        while (pos < itB.size() && acc < target) {
+            part.push_back((int)pos);
+            acc += costB[pos];
+            pos++;
+        }
+        std::vector<int> wgs;
+        for (int w = x; w < G; w += nx) wgs.push_back(w);
+        lpt_assign(costB, part, wgs, listsB);
+    }
+    // flatten: [itemsA][offA][itemsB][offB][ctl]
+    std::vector<int4> flatA, flatB;
+    std::vector<int> offA(4 * G + 1), offB(G + 1);
+    for (int w = 0; w < 4 * G; w++) {
+        offA[w] = (int)flatA.size();
+        for (int i : listsA[w]) flatA.push_back(itA[i]);
+    }
+    for (int w = 0; w < G; w++) {
+        offB[w] = (int)flatB.size();
+        for (int i : listsB[w]) flatB.push_back(itB[i]);
+    }
+    offA[4 * G] = (int)flatA.size();
+    offB[G] = (int)flatB.size();
+    const size_t bA = round_up(sizeof(int4) * flatA.size(), 256), bOA = round_up(sizeof(int) * (4 * G + 1), 256);
+    const size_t bB = round_up(sizeof(int4) * flatB.size(), 256), bOB = round_up(sizeof(int) * (G + 1), 256);
+    const size_t bC = round_up(persist_ctl_bytes(), 256);
+    GCHK(hipMalloc(&g->pbuf, bA + bOA + bB + bOB + bC));
+    char *q = g->pbuf;
+    GCHK(hipMemcpy(q, flatA.data(), sizeof(int4) * flatA.size(), hipMemcpyHostToDevice));
+    GCHK(hipMemcpy(q + bA, offA.data(), sizeof(int) * (4 * G + 1), hipMemcpyHostToDevice));
+    GCHK(hipMemcpy(q + bA + bOA, flatB.data(), sizeof(int4) * flatB.size(), hipMemcpyHostToDevice));
+    GCHK(hipMemcpy(q + bA + bOA + bB, offB.data(), sizeof(int) * (G + 1), hipMemcpyHostToDevice));
+    GCHK(hipMemset(q + bA + bOA + bB + bOB, 0, bC));   // once, outside any graph
+    GCHK(hipDeviceSynchronize());
+    PersistLaunch &L = g->PL;
+    L.itemsA = reinterpret_cast<const int4 *>(q);
+    L.offA = reinterpret_cast<const int *>(q + bA);
+    L.itemsB = reinterpret_cast<const int4 *>(q + bA + bOA);
+    L.offB = reinterpret_cast<const int *>(q + bA + bOA + bB);
+    L.ctl = q + bA + bOA + bB + bOB;
+    L.grid = G;
+    L.b = g->b;
+    L.rounding = g->rounding;
+    return OZAIE_OK;
+}
+
 // Workspace layout, descriptors and work-item tables; pointers to A, B, C are filled in
 // at run time.
 int group_build(ozaie_group g, int force_cfg) {
@@ -138,7 +269,7 @@ int group_build(ozaie_group g, int force_cfg) {
         Kp[p] = round_up(g->K[p], 2 * kBlk);
         nE += Mp[p] + Np[p];
         slice_bytes += (int64_t)g->d * (Mp[p] + Np[p]) * Kp[p];
-        g->cfg[p] = pick_cfg(g->d, g->M[p], g->N[p], g->K[p], force_cfg);
+        g->cfg[p] = g->persist ? g->pcfg : pick_cfg(g->d, g->M[p], g->N[p], g->K[p], force_cfg);
     }
     const int64_t e_bytes = round_up(nE * 4, 1024);
     GCHK(hipMalloc(&g->ws, (size_t)(e_bytes + slice_bytes)));
@@ -191,6 +322,20 @@ int group_build(ozaie_group g, int force_cfg) {
         g->tiles += ntm * ntn;
     }
     int rc;
+    GCHK(hipMalloc(&g->d_ops, sizeof(GroupOperand) * 2 * n));
+    GCHK(hipMalloc(&g->d_probs, sizeof(GemmLaunch) * n));
+    GCHK(hipEventCreateWithFlags(&g->upload_done, hipEventDisableTiming));
+    GCHK(hipStreamCreateWithFlags(&g->cap_stream, hipStreamNonBlocking));
+    g->lastA.assign(n, nullptr);
+    g->lastB.assign(n, nullptr);
+    g->lastC.assign(n, nullptr);
+    if (g->persist) {
+        g->tiles = 0;
+        if ((rc = build_persist(g, Kp))) return rc;
+        g->PL.ops = g->d_ops;
+        g->PL.probs = g->d_probs;
+        return OZAIE_OK;
+    }
     if ((rc = upload(&g->d_items_max, imax))) return rc;
     if ((rc = upload(&g->d_items_split, isplit))) return rc;
     g->n_max = (int)imax.size();
@@ -199,17 +344,14 @@ int group_build(ozaie_group g, int force_cfg) {
         if ((rc = upload(&g->d_items_gemm[c], igemm[c]))) return rc;
         g->n_gemm[c] = (int)igemm[c].size();
     }
-    GCHK(hipMalloc(&g->d_ops, sizeof(GroupOperand) * 2 * n));
-    GCHK(hipMalloc(&g->d_probs, sizeof(GemmLaunch) * n));
-    GCHK(hipEventCreateWithFlags(&g->upload_done, hipEventDisableTiming));
-    GCHK(hipStreamCreateWithFlags(&g->cap_stream, hipStreamNonBlocking));
-    g->lastA.assign(n, nullptr);
-    g->lastB.assign(n, nullptr);
-    g->lastC.assign(n, nullptr);
     return OZAIE_OK;
 }
 
 int group_enqueue(ozaie_group g, hipStream_t st) {
+    if (g->persist) {
+        GCHK(launch_group_persistent(g->PL, g->d, g->pcfg, st));
+        return OZAIE_OK;
+    }
     GCHK(launch_fill_i32(g->E_all, g->n_E, 0, st));
     GCHK(launch_group_maxexp(g->d_ops, g->d_items_max, g->n_max, st));
     GCHK(launch_group_split(g->d_ops, g->d_items_split, g->n_split, g->b, g->d, st));
@@ -255,6 +397,12 @@ int group_new(ozaie_group *out, int count, const int64_t *M, const int64_t *N, c
     hipError_t e = hipGetDevice(&g->device);
     if (e != hipSuccess) rc = set_error(e, "hipGetDevice");
     g->fallback = !group_cfg_supported(slices, GCFG_D11);
+    if (!g->fallback && (flags & 4)) {
+        // persistent single launch: forced configuration if it has a persistent kernel
+        const int want = ((flags >> 4) & 15) - 1 >= 0 ? ((flags >> 4) & 15) - 1 : force_cfg;
+        g->pcfg = (want >= 0 && persist_cfg_supported(slices, want)) ? want : default_persist_cfg(slices);
+        g->persist = persist_cfg_supported(slices, g->pcfg);
+    }
     if (!rc && g->fallback) {
         for (int p = 0; p < count && !rc; p++) {
             ozaie_plan pl = nullptr;
@@ -286,6 +434,7 @@ int group_fallback_enqueue(ozaie_group g, hipStream_t st) {
 
 int group_run(ozaie_group g, const double *const *A, const double *const *B, double *const *C, hipStream_t st) {
     if (!A || !B || !C) return OZAIE_ERR_ARG;
+    g->last_stream = st;
     bool changed = !g->uploaded;
     for (int p = 0; p < g->count; p++) {
         if (!A[p] || !B[p] || !C[p]) return OZAIE_ERR_ARG;
@@ -363,6 +512,7 @@ int32_t ozaie_group_query(ozaie_group g, int32_t *out4) {
     if (!g || !out4) return OZAIE_ERR_ARG;
     int launches = 0;
     if (g->fallback) launches = g->count * 5;
+    else if (g->persist) launches = 1;
     else {
         launches = 3;
         for (int c = 0; c < GCFG_COUNT; c++) launches += g->n_gemm[c] > 0;
@@ -378,6 +528,24 @@ int32_t ozaie_group_configs(ozaie_group g, int32_t *cfg_out) {
     if (!g || !cfg_out) return OZAIE_ERR_ARG;
     for (int p = 0; p < g->count; p++) cfg_out[p] = g->fallback ? -1 : g->cfg[p];
     return OZAIE_OK;
+}
+
+int32_t ozaie_group_status(ozaie_group g, int32_t *error_out) {
+    if (!g || !error_out) return OZAIE_ERR_ARG;
+    *error_out = 0;
+    if (!g->persist) return OZAIE_OK;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return OZAIE_ERR_HIP;
+    if (cur != g->device) (void)hipSetDevice(g->device);
+    int rc = OZAIE_OK;
+    int32_t ctl[4] = {0, 0, 0, 0};
+    hipError_t e = hipStreamSynchronize(g->last_stream);
+    if (e == hipSuccess) e = hipMemcpy(ctl, g->PL.ctl, sizeof(ctl), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && ctl[2]) e = hipMemset(static_cast<char *>(g->PL.ctl) + 2 * sizeof(int32_t), 0, sizeof(int32_t));
+    if (e != hipSuccess) rc = set_error(e, "ozaie_group_status");
+    *error_out = ctl[2];
+    if (cur != g->device) (void)hipSetDevice(cur);
+    return rc;
 }
 
 int32_t ozaie_group_destroy(ozaie_group g) {

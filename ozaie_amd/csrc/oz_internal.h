@@ -78,6 +78,25 @@ int group_cfg_tile_n(int cfg);   // columns of C per workgroup
 hipError_t launch_group_maxexp(const GroupOperand *ops, const int4 *items, int n, hipStream_t st);
 hipError_t launch_group_split(const GroupOperand *ops, const int4 *items, int n, int b, int d, hipStream_t st);
 hipError_t launch_group_gemm(const GemmLaunch *probs, const int4 *items, int n, int d, int cfg, hipStream_t st);
+// Persistent grouped launch (one kernel: split phase, grid barrier, GEMM phase).  Phase A
+// lists are per wave (items [offA[4w+v], offA[4w+v+1]) for wave v of workgroup w), phase B
+// lists per workgroup (items [offB[w], offB[w+1])); phase A items are
+// int4 {operand, row block, 0, 0}, phase B items int4 {problem, tile row, tile col, 0}.
+// ctl is persist_ctl_bytes() of zero-initialised device memory, reset by the kernel itself
+// at the end of every launch; its third int is a sticky error flag (barrier wait bound).
+struct PersistLaunch {
+    const GroupOperand *ops;
+    const int4 *itemsA;
+    const int *offA;
+    const GemmLaunch *probs;
+    const int4 *itemsB;
+    const int *offB;
+    void *ctl;
+    int grid, b, rounding;
+};
+bool persist_cfg_supported(int d, int cfg);
+int persist_ctl_bytes();
+hipError_t launch_group_persistent(const PersistLaunch &L, int d, int cfg, hipStream_t st);
 // Tile order inside a problem: bands of 8 tile-rows, so tiles that run together share
 // rows of A (device tile mapping and the grouped planner).
 __host__ __device__ inline void band_map(int lin, int ntm, int ntn, int &tm, int &tn) {

@@ -121,47 +121,56 @@ hipError_t launch_maxexp(const OperandView &op, int32_t *E, int64_t Rp, hipStrea
 // 16-byte lane chunk per slice plane (a wave writes one whole 1 KiB block per plane).
 constexpr int kSplitLds = 128 * 33 * 8;  // bytes of the FP64 staging tile (either orientation)
 
-template <bool KC>
-__device__ __forceinline__ void split_tile(const double *__restrict__ X, int64_t sr, int64_t sk, int64_t R,
-                                           int64_t K, const int32_t *__restrict__ E, int b, int d,
-                                           int8_t *__restrict__ S, int64_t Rb, int64_t Kb, int64_t rb,
-                                           int64_t kt, double *tile) {
-    constexpr int TK = 128;
-    constexpr int LDW = KC ? (TK + 1) : (32 + 1);
-    const int64_t r0 = rb * 32, k0 = kt * TK;
-    const int t = threadIdx.x;
+// Digits of 16 elements (mantissas mx with the implicit bit, bias = row exponent minus
+// element exponent, sign bits negm) into one 16-byte chunk per slice plane at dst.
+// With b*d <= 64 the whole digit string Q = floor(M_x 2^(bd - 53 - bias)) fits 64 bits and
+// digit j is the b-bit field of Q at bit b(d-1-j): one bit-field extract per digit.  Four
+// digits share a word; the negative elements' bytes are negated without carries between
+// bytes (every digit is below 128, so 0x80 - x fits a byte: -x = (0x80 - x) ^ 0x80).
+// Wider strings shift the mantissa per digit.
+__device__ __forceinline__ void emit_digits16(uint64_t (&mx)[16], const int (&bias)[16], uint32_t negm, int b,
+                                              int d, int8_t *__restrict__ dst, int64_t plane) {
+    uint32_t nbm[4];
 #pragma unroll
-    for (int i = 0; i < 16; i++) {
-        const int c = t + 256 * i;
-        int row, kk;
-        if (KC) { row = c >> 7; kk = c & 127; }
-        else { kk = c >> 5; row = c & 31; }
-        const int64_t r = r0 + row, k = k0 + kk;
-        double v = 0.0;
-        if (r < R && k < K) v = KC ? X[r * sr + k] : X[k * sk + r];
-        if (KC) tile[row * LDW + kk] = v;
-        else tile[kk * LDW + row] = v;
+    for (int gi = 0; gi < 4; gi++) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int h = 0; h < 4; h++) m |= ((negm >> (4 * gi + h)) & 1u) ? (0xFFu << (8 * h)) : 0u;
+        nbm[gi] = m;
     }
-    __syncthreads();
-    const int row = t & 31, q = t >> 5;  // k in [16q, 16q + 16) of the tile
-    const int64_t kb = kt * 4 + (q >> 1);
-    if (kb >= Kb) return;  // no barrier below
-    const int emax = (r0 + row < R) ? E[r0 + row] : 0;
-    uint64_t mx[16];
-    int bias[16];
-    uint32_t negm = 0;
+    if (b * d <= 64) {
+        uint32_t qlo[16], qhi[16];
 #pragma unroll
-    for (int i = 0; i < 16; i++) {
-        const double v = KC ? tile[row * LDW + 16 * q + i] : tile[(16 * q + i) * LDW + row];
-        const uint64_t u = (uint64_t)__double_as_longlong(v);
-        const int e = (int)((u >> 52) & 0x7FF);
-        mx[i] = (u & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
-        bias[i] = emax - e;
-        negm |= (uint32_t)(u >> 63) << i;
+        for (int i = 0; i < 16; i++) {
+            const int sh = 53 + bias[i] - b * d;   // Q = mx >> sh (mx << -sh below zero)
+            const uint64_t q = sh >= 64 ? 0ull : (sh >= 0 ? mx[i] >> sh : mx[i] << (-sh));
+            qlo[i] = (uint32_t)q;
+            qhi[i] = (uint32_t)(q >> 32);
+        }
+        const uint32_t mask = (1u << b) - 1u;
+        // one plane: x(i) is element i's digit; negative elements' bytes negated
+        auto plane_out = [&](int j, auto x) {
+            uint32_t w[4];
+#pragma unroll
+            for (int gi = 0; gi < 4; gi++) {
+                const uint32_t p = x(4 * gi) | (x(4 * gi + 1) << 8) | (x(4 * gi + 2) << 16) | (x(4 * gi + 3) << 24);
+                const uint32_t pn = (0x80808080u - p) ^ 0x80808080u;
+                w[gi] = (pn & nbm[gi]) | (p & ~nbm[gi]);
+            }
+            *reinterpret_cast<uint4 *>(dst + (int64_t)j * plane) = make_uint4(w[0], w[1], w[2], w[3]);
+        };
+        int j = 0;
+        for (; j < d && b * (d - 1 - j) >= 32; j++) {   // field inside the high word
+            const uint32_t off = (uint32_t)(b * (d - 1 - j) - 32);
+            plane_out(j, [&](int i) { return __builtin_amdgcn_ubfe(qhi[i], off, (uint32_t)b); });
+        }
+        for (; j < d; j++) {                              // field at bit < 32 (may straddle)
+            const uint32_t sj = (uint32_t)(b * (d - 1 - j));
+            plane_out(j, [&](int i) { return __builtin_amdgcn_alignbit(qhi[i], qlo[i], sj) & mask; });
+        }
+        return;
     }
     const uint64_t mask = (1ull << b) - 1ull;
-    const int64_t plane = Rb * Kb * (int64_t)kBlkBytes;
-    int8_t *dst = S + (rb * Kb + kb) * (int64_t)kBlkBytes + (q & 1) * 512 + row * 16;
     for (int j = 0; j < d; j++) {
         uint32_t w[4];
 #pragma unroll
@@ -185,13 +194,93 @@ __device__ __forceinline__ void split_tile(const double *__restrict__ X, int64_t
     }
 }
 
+// The three steps of a tile of 32 rows x TK contraction indices (TK = 128 or 256),
+// separable so that a caller can overlap the global reads of the next tile with the digit
+// extraction of the current one.
+template <bool KC, int TK = 128>
+__host__ __device__ constexpr int split_lds_bytes() { return KC ? 32 * (TK + 1) * 8 : TK * 33 * 8; }
+
+template <bool KC, int TK = 128>
+__device__ __forceinline__ void split_gather(const double *__restrict__ X, int64_t sr, int64_t sk, int64_t R,
+                                             int64_t K, int64_t rb, int64_t kt, double (&v)[TK / 8]) {
+    const int64_t r0 = rb * 32, k0 = kt * TK;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < TK / 8; i++) {
+        const int c = t + 256 * i;
+        int row, kk;
+        if (KC) { row = c / TK; kk = c % TK; }
+        else { kk = c >> 5; row = c & 31; }
+        const int64_t r = r0 + row, k = k0 + kk;
+        v[i] = (r < R && k < K) ? (KC ? X[r * sr + k] : X[k * sk + r]) : 0.0;
+    }
+}
+
+template <bool KC, int TK = 128>
+__device__ __forceinline__ void split_stage(const double (&v)[TK / 8], double *tile) {
+    constexpr int LDW = KC ? (TK + 1) : (32 + 1);
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < TK / 8; i++) {
+        const int c = t + 256 * i;
+        if (KC) tile[(c / TK) * LDW + (c % TK)] = v[i];
+        else tile[(c >> 5) * LDW + (c & 31)] = v[i];
+    }
+}
+
+// Digits of the staged tile: thread (row, q) owns 16 consecutive k of one row and writes
+// one 16-byte lane chunk per slice plane (q = t / 32 + 8 p for the TK / 128 passes p).
+template <bool KC, int TK = 128>
+__device__ __forceinline__ void split_emit(const double *tile, int64_t R, const int32_t *__restrict__ Erow, int b,
+                                           int d, int8_t *__restrict__ S, int64_t Rb, int64_t Kb, int64_t rb,
+                                           int64_t kt) {
+    constexpr int LDW = KC ? (TK + 1) : (32 + 1);
+    const int t = threadIdx.x;
+    const int64_t r0 = rb * 32;
+    const int row = t & 31;
+    const int emax = (r0 + row < R) ? Erow[row] : 0;
+    const int64_t plane = Rb * Kb * (int64_t)kBlkBytes;
+#pragma unroll
+    for (int pass = 0; pass < TK / 128; pass++) {
+        const int q = (t >> 5) + 8 * pass;   // k in [16q, 16q + 16) of the tile
+        const int64_t kb = kt * (TK / 32) + (q >> 1);
+        if (kb >= Kb) break;  // no barrier below
+        uint64_t mx[16];
+        int bias[16];
+        uint32_t negm = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const double v = KC ? tile[row * LDW + 16 * q + i] : tile[(16 * q + i) * LDW + row];
+            const uint64_t u = (uint64_t)__double_as_longlong(v);
+            const int e = (int)((u >> 52) & 0x7FF);
+            mx[i] = (u & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
+            bias[i] = emax - e;
+            negm |= (uint32_t)(u >> 63) << i;
+        }
+        int8_t *dst = S + (rb * Kb + kb) * (int64_t)kBlkBytes + (q & 1) * 512 + row * 16;
+        emit_digits16(mx, bias, negm, b, d, dst, plane);
+    }
+}
+
+template <bool KC>
+__device__ __forceinline__ void split_tile(const double *__restrict__ X, int64_t sr, int64_t sk, int64_t R,
+                                           int64_t K, const int32_t *__restrict__ Erow, int b, int d,
+                                           int8_t *__restrict__ S, int64_t Rb, int64_t Kb, int64_t rb,
+                                           int64_t kt, double *tile) {
+    double v[16];
+    split_gather<KC>(X, sr, sk, R, K, rb, kt, v);
+    split_stage<KC>(v, tile);
+    __syncthreads();
+    split_emit<KC>(tile, R, Erow, b, d, S, Rb, Kb, rb, kt);
+}
+
 template <bool KC>
 __global__ __launch_bounds__(256) void k_split(const double *__restrict__ X, int64_t sr,
                                                int64_t sk, int64_t R, int64_t K,
                                                const int32_t *__restrict__ E, int b, int d,
                                                int8_t *__restrict__ S, int64_t Rb, int64_t Kb) {
     __shared__ double tile[kSplitLds / 8];
-    split_tile<KC>(X, sr, sk, R, K, E, b, d, S, Rb, Kb, blockIdx.y, blockIdx.x, tile);
+    split_tile<KC>(X, sr, sk, R, K, E + (int64_t)blockIdx.y * 32, b, d, S, Rb, Kb, blockIdx.y, blockIdx.x, tile);
 }
 
 hipError_t launch_split(const OperandView &op, const int32_t *E, int b, int d, int8_t *S,
@@ -250,10 +339,11 @@ __device__ __forceinline__ double t_to_double(i128 t, int es, int b, int d, int 
     return sgn ? -v : v;
 }
 
-// Store one element: bins k0 .. k0+NB-1 of this pass (values vals[kk]).
+// Store one element: bins k0 .. k0+NB-1 of this pass (values vals[kk]); es is the sum of
+// the element's row and column exponent fields.
 template <int NB>
 __device__ __forceinline__ void epilogue_elem(const GemmLaunch &g, int epi, int row, int col,
-                                              const int32_t (&vals)[NB], int k0) {
+                                              const int32_t (&vals)[NB], int k0, int es) {
     if (row >= g.M || col >= g.N) return;
     if (epi == EPI_BINS) {
 #pragma unroll
@@ -270,7 +360,40 @@ __device__ __forceinline__ void epilogue_elem(const GemmLaunch &g, int epi, int 
     i128 *tp = g.T + (int64_t)row * g.ldT + col;
     if (!g.first) T += *tp;
     if (g.last)
-        g.C[(int64_t)row * g.ldc + col] = t_to_double(T, g.Er[row] + g.Ec[col], g.b, g.d, g.rounding);
+        g.C[(int64_t)row * g.ldc + col] = t_to_double(T, es, g.b, g.d, g.rounding);
+    else
+        *tp = T;
+}
+
+// Exact T = sum_k v_k 2^(B(D-1-k)) for a compile-time digit width B with all D bins of one
+// pass: the bins are summed in 64-bit chunks of 31/B + 1 consecutive bins (relative
+// shifts up to 31 bits keep a chunk of int32 terms inside int64), and the chunks are
+// combined in 128-bit arithmetic, so every shift is a constant.
+template <int D, int B>
+__device__ __forceinline__ i128 bins_to_T(const int32_t (&v)[D]) {
+    constexpr int CH = 31 / B + 1;
+    i128 T = 0;
+#pragma unroll
+    for (int top = D - 1; top >= 0; top -= CH) {
+        int64_t c = 0;
+#pragma unroll
+        for (int i = 0; i < CH; i++)
+            if (top - i >= 0) c += (int64_t)v[top - i] << (B * i);
+        T += (i128)c << (B * (D - 1 - top));
+    }
+    return T;
+}
+
+// The same store for EPI_C when the pass holds every bin (k0 = 0, NB = d) and b = B.
+template <int D, int B>
+__device__ __forceinline__ void epilogue_elem_c(const GemmLaunch &g, int row, int col, const int32_t (&vals)[D],
+                                                int es) {
+    if (row >= g.M || col >= g.N) return;
+    i128 T = bins_to_T<D, B>(vals);
+    i128 *tp = g.T + (int64_t)row * g.ldT + col;
+    if (!g.first) T += *tp;
+    if (g.last)
+        g.C[(int64_t)row * g.ldc + col] = t_to_double(T, es, B, D, g.rounding);
     else
         *tp = T;
 }
@@ -315,11 +438,17 @@ struct FusedCfg {
 };
 
 // Epilogue of one wave: bins leave the accumulators four elements at a time through a
-// per-wave LDS scratch (each lane reads back only its own values), and a rolled loop
-// turns each element's bins into T and FP64 C, keeping register pressure flat.
+// per-wave LDS scratch (each lane reads back only its own values) and each element's bins
+// become T and FP64 C.  The scratch of a wave holds NB * 256 bin words followed by the
+// exponent fields of the wave's 64 rows and 64 columns (er_v / ec_v: this lane's row
+// rbase + lane and column cbase + lane, loaded by the caller before its main loop).
 template <int NB, int WM, int WN>
-__device__ __forceinline__ void wave_epilogue(const GemmLaunch &g, int epi, int k0, v16i (&acc)[NB][WM][WN],
-                                              int32_t *scratch, int lane, int rbase, int cbase) {
+__host__ __device__ constexpr int wave_scratch_words() { return NB * 256 + 128; }
+
+template <int NB, int WM, int WN, int B>
+__device__ __forceinline__ void wave_epilogue_t(const GemmLaunch &g, int epi, int k0, v16i (&acc)[NB][WM][WN],
+                                                int32_t *scratch, int lane, int rbase, int cbase) {
+    const int32_t *ex = scratch + NB * 256;
 #pragma unroll
     for (int mi = 0; mi < WM; mi++)
 #pragma unroll
@@ -331,30 +460,71 @@ __device__ __forceinline__ void wave_epilogue(const GemmLaunch &g, int epi, int 
 #pragma unroll
                     for (int e = 0; e < 4; e++) scratch[(k * 4 + e) * 64 + lane] = acc[k][mi][ni][4 * q + e];
                 __builtin_amdgcn_sched_barrier(0);
-#pragma unroll 1
-                for (int e = 0; e < 4; e++) {
-                    const int row = rbase + mi * 32 + q * 8 + (lane >> 5) * 4 + e;
-                    const int col = cbase + ni * 32 + (lane & 31);
-                    int32_t vals[NB];
+                const int lcol = ni * 32 + (lane & 31);
+                const int ecol = ex[64 + lcol];
+                if constexpr (B > 0) {
+                    // four independent elements per step
 #pragma unroll
-                    for (int k = 0; k < NB; k++) vals[k] = scratch[(k * 4 + e) * 64 + lane];
-                    epilogue_elem<NB>(g, epi, row, col, vals, k0);
+                    for (int e = 0; e < 4; e++) {
+                        const int lrow = mi * 32 + q * 8 + (lane >> 5) * 4 + e;
+                        int32_t vals[NB];
+#pragma unroll
+                        for (int k = 0; k < NB; k++) vals[k] = scratch[(k * 4 + e) * 64 + lane];
+                        epilogue_elem_c<NB, B>(g, rbase + lrow, cbase + lcol, vals, ex[lrow] + ecol);
+                    }
+                } else {
+#pragma unroll 1
+                    for (int e = 0; e < 4; e++) {
+                        const int lrow = mi * 32 + q * 8 + (lane >> 5) * 4 + e;
+                        int32_t vals[NB];
+#pragma unroll
+                        for (int k = 0; k < NB; k++) vals[k] = scratch[(k * 4 + e) * 64 + lane];
+                        epilogue_elem<NB>(g, epi, rbase + lrow, cbase + lcol, vals, k0, ex[lrow] + ecol);
+                    }
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
 }
 
+template <int NB, int WM, int WN>
+__device__ __forceinline__ void wave_epilogue(const GemmLaunch &g, int epi, int k0, v16i (&acc)[NB][WM][WN],
+                                              int32_t *scratch, int lane, int rbase, int cbase, int er_v,
+                                              int ec_v) {
+    scratch[NB * 256 + lane] = er_v;
+    scratch[NB * 256 + 64 + lane] = ec_v;
+    if (epi == EPI_C && k0 == 0 && NB == g.d && g.b == 7)
+        wave_epilogue_t<NB, WM, WN, 7>(g, epi, k0, acc, scratch, lane, rbase, cbase);
+    else
+        wave_epilogue_t<NB, WM, WN, 0>(g, epi, k0, acc, scratch, lane, rbase, cbase);
+}
+
+// Exponent fields this lane hands to the epilogue: row rbase + lane of A (lane < rows)
+// and column cbase + lane of B (lane < cols).
+__device__ __forceinline__ void load_tile_exps(const GemmLaunch &g, int epi, int lane, int rbase, int rows,
+                                               int cbase, int cols, int &er_v, int &ec_v) {
+    er_v = 0;
+    ec_v = 0;
+    if (epi != EPI_C) return;
+    if (lane < rows && rbase + lane < g.M) er_v = g.Er[rbase + lane];
+    if (lane < cols && cbase + lane < g.N) ec_v = g.Ec[cbase + lane];
+}
+
 template <int D, int WM, int WN>
-__device__ __forceinline__ void gemm_lds_tile(const GemmLaunch &g, int tm, int tn, int epi) {
+__host__ __device__ constexpr int lds_tile_bytes() { return 2 * FusedCfg<D, WM, WN>::STAGE; }
+
+template <int D, int WM, int WN>
+__device__ __forceinline__ void gemm_lds_tile(const GemmLaunch &g, int tm, int tn, int epi, int8_t *smem) {
     using F = FusedCfg<D, WM, WN>;
     constexpr int BA = F::BA, BB = F::BB, NBLK = F::NBLK, NLD = F::NLD, NMIN = F::NMIN;
-    __shared__ __attribute__((aligned(1024))) int8_t smem[2 * F::STAGE];
 
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = w & 1, wn = w >> 1;
     const int64_t rbA0 = (int64_t)tm * BA, rbB0 = (int64_t)tn * BB;
     const int64_t planeA = g.RbA * g.Kb * kBlkBytes, planeB = g.RbB * g.Kb * kBlkBytes;
+    const int rbase = tm * 64 * WM + wm * 32 * WM, cbase = tn * 64 * WN + wn * 32 * WN;
+    int er_v, ec_v;
+    load_tile_exps(g, epi, lane, rbase, 32 * WM, cbase, 32 * WN, er_v, ec_v);
 
     const int8_t *src[NLD];
     int ldsoff[NLD];
@@ -421,9 +591,8 @@ __device__ __forceinline__ void gemm_lds_tile(const GemmLaunch &g, int tm, int t
     }
     __syncthreads();
 
-    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * (D * 256);
-    wave_epilogue<D, WM, WN>(g, epi, 0, acc, scratch, lane, tm * 64 * WM + wm * 32 * WM,
-                             tn * 64 * WN + wn * 32 * WN);
+    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * wave_scratch_words<D, WM, WN>();
+    wave_epilogue<D, WM, WN>(g, epi, 0, acc, scratch, lane, rbase, cbase, er_v, ec_v);
 }
 
 template <int D, int WM, int WN>
@@ -432,7 +601,8 @@ void k_gemm_fused(GemmLaunch g, int epi) {
     const int ntm = (g.M + 64 * WM - 1) / (64 * WM), ntn = (g.N + 64 * WN - 1) / (64 * WN);
     int tm, tn;
     tile_map(blockIdx.x, ntm, ntn, tm, tn);
-    gemm_lds_tile<D, WM, WN>(g, tm, tn, epi);
+    __shared__ __attribute__((aligned(1024))) int8_t smem[lds_tile_bytes<D, WM, WN>()];
+    gemm_lds_tile<D, WM, WN>(g, tm, tn, epi, smem);
 }
 
 // Direct variant: no LDS in the main loop.  Each wave streams its own A and B
@@ -440,14 +610,19 @@ void k_gemm_fused(GemmLaunch g, int epi) {
 // double-buffered in registers, so there is no barrier and the compiler's per-register
 // wait counts overlap the loads of block t+1 with the MFMAs of block t.  Requires an
 // even number of contraction blocks per pass (the planner pads K to 64).
+template <int D>
+__host__ __device__ constexpr int direct_tile_bytes() { return 4 * 4 * (D * 256 + 128); }
+
 template <int D, int WM, int WN>
-__device__ __forceinline__ void gemm_direct_tile(const GemmLaunch &g, int tm, int tn, int epi) {
-    __shared__ __attribute__((aligned(1024))) int8_t smem[4 * D * 1024];
+__device__ __forceinline__ void gemm_direct_tile(const GemmLaunch &g, int tm, int tn, int epi, int8_t *smem) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = w & 1, wn = w >> 1;
     const int64_t planeA = g.RbA * g.Kb * kBlkBytes, planeB = g.RbB * g.Kb * kBlkBytes;
     const int64_t rbA = (int64_t)tm * 2 * WM + wm * WM, rbB = (int64_t)tn * 2 * WN + wn * WN;
+    const int rbase = tm * 64 * WM + wm * 32 * WM, cbase = tn * 64 * WN + wn * 32 * WN;
+    int er_v, ec_v;
+    load_tile_exps(g, epi, lane, rbase, 32 * WM, cbase, 32 * WN, er_v, ec_v);
     const int8_t *pa = g.SA + rbA * g.Kb * kBlkBytes + lane * 16;
     const int8_t *pb = g.SB + rbB * g.Kb * kBlkBytes + lane * 16;
     const int64_t rowstride = g.Kb * kBlkBytes;
@@ -491,9 +666,8 @@ __device__ __forceinline__ void gemm_direct_tile(const GemmLaunch &g, int tm, in
         if (t + 2 < nk) load(a0, b0, kb0 + t + 2);
         mma(a1, b1);
     }
-    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * (D * 256);
-    wave_epilogue<D, WM, WN>(g, epi, 0, acc, scratch, lane, tm * 64 * WM + wm * 32 * WM,
-                             tn * 64 * WN + wn * 32 * WN);
+    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * wave_scratch_words<D, WM, WN>();
+    wave_epilogue<D, WM, WN>(g, epi, 0, acc, scratch, lane, rbase, cbase, er_v, ec_v);
 }
 
 template <int D, int WM, int WN>
@@ -502,7 +676,8 @@ void k_gemm_direct(GemmLaunch g, int epi) {
     const int ntm = (g.M + 64 * WM - 1) / (64 * WM), ntn = (g.N + 64 * WN - 1) / (64 * WN);
     int tm, tn;
     tile_map(blockIdx.x, ntm, ntn, tm, tn);
-    gemm_direct_tile<D, WM, WN>(g, tm, tn, epi);
+    __shared__ __attribute__((aligned(1024))) int8_t smem[direct_tile_bytes<D>()];
+    gemm_direct_tile<D, WM, WN>(g, tm, tn, epi, smem);
 }
 
 // Pipelined LDS variant: the slice blocks of block t+2 are copied HBM -> LDS
@@ -534,6 +709,9 @@ __device__ __forceinline__ void gemm_pipe_tile(const GemmLaunch &g, int tm, int 
     const int wm = w & 1, wn = w >> 1;
     const int64_t rbA0 = (int64_t)tm * BA, rbB0 = (int64_t)tn * BB;
     const int64_t planeA = g.RbA * g.Kb * kBlkBytes, planeB = g.RbB * g.Kb * kBlkBytes;
+    const int rbase = tm * 64 * WM + wm * 32 * WM, cbase = tn * 64 * WN + wn * 32 * WN;
+    int er_v, ec_v;
+    load_tile_exps(g, epi, lane, rbase, 32 * WM, cbase, 32 * WN, er_v, ec_v);
 
     const int8_t *src[NLD];
     int ldsoff[NLD];
@@ -622,9 +800,8 @@ __device__ __forceinline__ void gemm_pipe_tile(const GemmLaunch &g, int tm, int 
     }
     __syncthreads();
 
-    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * (D * 256);
-    wave_epilogue<D, WM, WN>(g, epi, 0, acc, scratch, lane, tm * 64 * WM + wm * 32 * WM,
-                             tn * 64 * WN + wn * 32 * WN);
+    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * wave_scratch_words<D, WM, WN>();
+    wave_epilogue<D, WM, WN>(g, epi, 0, acc, scratch, lane, rbase, cbase, er_v, ec_v);
 }
 
 template <int D, int WM, int WN>
@@ -703,6 +880,9 @@ void k_gemm_group(GemmLaunch g, int epi) {
     const int nblk = nsl * (BA + BB);
     const int64_t planeA = g.RbA * g.Kb * kBlkBytes, planeB = g.RbB * g.Kb * kBlkBytes;
     const int64_t rbA0 = (int64_t)tm * BA, rbB0 = (int64_t)tn * BB;
+    const int rbase = tm * 64 + wm * 32, cbase = tn * 64 + wn * 32;
+    int er_v, ec_v;
+    load_tile_exps(g, epi, lane, rbase, 32, cbase, 32, er_v, ec_v);
 
     v16i acc[NB];
 #pragma unroll
@@ -742,8 +922,8 @@ void k_gemm_group(GemmLaunch g, int epi) {
     v16i acc3[NB][1][1];
 #pragma unroll
     for (int k = 0; k < NB; k++) acc3[k][0][0] = acc[k];
-    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * (NB * 256);
-    wave_epilogue<NB, 1, 1>(g, epi, k0, acc3, scratch, lane, tm * 64 + wm * 32, tn * 64 + wn * 32);
+    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * wave_scratch_words<NB, 1, 1>();
+    wave_epilogue<NB, 1, 1>(g, epi, k0, acc3, scratch, lane, rbase, cbase, er_v, ec_v);
 }
 
 hipError_t launch_gemm_group(const GemmLaunch &g, int epi, hipStream_t st) {
@@ -807,10 +987,11 @@ __global__ __launch_bounds__(256) void k_grp_split(const GroupOperand *__restric
     __shared__ double tile[kSplitLds / 8];
     const int4 it = items[blockIdx.x];
     const GroupOperand &o = ops[it.x];
+    const int32_t *Erow = o.E + (int64_t)it.y * 32;
     if (o.sk == 1)
-        split_tile<true>(o.X, o.sr, o.sk, o.R, o.K, o.E, b, d, o.S, o.Rb, o.Kb, it.y, it.z, tile);
+        split_tile<true>(o.X, o.sr, o.sk, o.R, o.K, Erow, b, d, o.S, o.Rb, o.Kb, it.y, it.z, tile);
     else
-        split_tile<false>(o.X, o.sr, o.sk, o.R, o.K, o.E, b, d, o.S, o.Rb, o.Kb, it.y, it.z, tile);
+        split_tile<false>(o.X, o.sr, o.sk, o.R, o.K, Erow, b, d, o.S, o.Rb, o.Kb, it.y, it.z, tile);
 }
 
 hipError_t launch_group_maxexp(const GroupOperand *ops, const int4 *items, int n, hipStream_t st) {
@@ -831,8 +1012,10 @@ void k_grp_gemm(const GemmLaunch *__restrict__ probs, const int4 *__restrict__ i
     // consecutive items (tiles of one problem in bands of tile-rows) go to one XCD
     const int4 it = items[xcd_remap(blockIdx.x, gridDim.x)];
     const GemmLaunch g = probs[it.x];
-    if (FAM == KERNEL_DIRECT) gemm_direct_tile<D, WM, WN>(g, it.y, it.z, EPI_C);
-    else gemm_lds_tile<D, WM, WN>(g, it.y, it.z, EPI_C);
+    constexpr int LDS = FAM == KERNEL_DIRECT ? direct_tile_bytes<D>() : lds_tile_bytes<D, WM, WN>();
+    __shared__ __attribute__((aligned(1024))) int8_t smem[LDS];
+    if (FAM == KERNEL_DIRECT) gemm_direct_tile<D, WM, WN>(g, it.y, it.z, EPI_C, smem);
+    else gemm_lds_tile<D, WM, WN>(g, it.y, it.z, EPI_C, smem);
 }
 
 int group_cfg_tile_m(int cfg) { return cfg == GCFG_D22 ? 128 : 64; }
@@ -871,6 +1054,390 @@ hipError_t launch_group_gemm(const GemmLaunch *probs, const int4 *items, int n, 
         case 7: return launch_grp_d<7>(probs, items, n, cfg, st);
         case 8: return launch_grp_d<8>(probs, items, n, cfg, st);
         case 9: return launch_grp_d<9>(probs, items, n, cfg, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// ------------------------------------------------------------------ persistent grouped launch
+// One launch for a whole group: a grid of one workgroup per CU runs
+//   phase A  the exponent maximum and the split of every 32-row block of every operand
+//            (one item = 32 rows over the whole contraction axis, so no item depends on
+//            another),
+//   a grid-wide barrier (every operand is split before any product starts),
+//   phase B  the fused slice-GEMM tiles of every problem, with the first contraction block
+//            of a workgroup's next tile loaded into registers while the current tile's
+//            epilogue runs.
+// Items are assigned to workgroups on the host (balanced lists, tiles of one problem on
+// one XCD), so the only cross-workgroup synchronisation is the barrier.
+
+// Phase A item, run by ONE wave: the 32 rows r0 = rb*32 .. +32 of operand o.  First the
+// exponent maximum of each row over the whole contraction axis (into o.E), then the
+// split in tiles of 32 rows x 64 contraction indices (two slice blocks): the tile is read
+// coalesced along the contiguous axis, transposed through the wave's own LDS region
+// (32 x 65 doubles), and lane l emits the 16-byte chunk it owns in each block (row l % 32,
+// k 16 (l / 32) .. +16).  The reads of tile t+1 are in flight while tile t is emitted.
+// Rows past R read row R - 1 (always valid) and are masked, so loads carry no branches.
+constexpr int kWaveTileLd = 65;   // LDS row stride (doubles) of the per-wave tile
+
+template <bool KC>
+__device__ __forceinline__ void wave_gather(const GroupOperand &o, int64_t r0, int64_t k0, int lane, double (&v)[32]) {
+    if (KC) {
+        // load i: row r0 + i, contraction index k0 + lane
+        const int64_t k = k0 + lane;
+#pragma unroll
+        for (int i = 0; i < 32; i++) {
+            const int64_t r = r0 + i;
+            v[i] = (r < o.R && k < o.K) ? o.X[r * o.sr + k] : 0.0;
+        }
+    } else {
+        // load i: contraction index k0 + 2i + lane / 32, row r0 + lane % 32
+        const int64_t r = r0 + (lane & 31);
+#pragma unroll
+        for (int i = 0; i < 32; i++) {
+            const int64_t k = k0 + 2 * i + (lane >> 5);
+            v[i] = (r < o.R && k < o.K) ? o.X[k * o.sk + r] : 0.0;
+        }
+    }
+}
+
+template <bool KC>
+__device__ __forceinline__ void wave_stage(const double (&v)[32], double *wt, int lane) {
+#pragma unroll
+    for (int i = 0; i < 32; i++) {
+        if (KC) wt[i * kWaveTileLd + lane] = v[i];
+        else wt[(lane & 31) * kWaveTileLd + 2 * i + (lane >> 5)] = v[i];
+    }
+}
+
+// Digits of one 16-element chunk (row `row`, elements wt[row][c0 .. c0+16)) into the
+// lane's 16-byte chunk of d slice planes at dst.
+__device__ __forceinline__ void wave_emit_chunk(const double *wt, int row, int c0, int emax, int b, int d,
+                                                int8_t *__restrict__ dst, int64_t plane) {
+    uint64_t mx[16];
+    int bias[16];
+    uint32_t negm = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint64_t u = (uint64_t)__double_as_longlong(wt[row * kWaveTileLd + c0 + i]);
+        const int e = (int)((u >> 52) & 0x7FF);
+        mx[i] = (u & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
+        bias[i] = emax - e;
+        negm |= (uint32_t)(u >> 63) << i;
+    }
+    emit_digits16(mx, bias, negm, b, d, dst, plane);
+}
+
+template <bool KC>
+__device__ __forceinline__ void wave_rowblock(const GroupOperand &o, int64_t rb, int b, int d, double *wt, int lane) {
+    const int64_t r0 = rb * 32;
+    // 1. exponent maxima (64 loads in flight per lane); lane l < 32 ends with row l's
+    int emax;
+    if (KC) {
+        const double *rowp[8];
+        int m[32];
+#pragma unroll
+        for (int i = 0; i < 32; i++) m[i] = 0;
+#pragma unroll
+        for (int gi = 0; gi < 4; gi++) {
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int64_t r = r0 + gi * 8 + i;
+                rowp[i] = o.X + (r < o.R ? r : o.R - 1) * o.sr;
+            }
+            int64_t k = lane;
+            for (; k + 448 < o.K; k += 512) {
+                double a[8][8];
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+#pragma unroll
+                    for (int u = 0; u < 8; u++) a[i][u] = rowp[i][k + 64 * u];
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+#pragma unroll
+                    for (int u = 0; u < 8; u++) m[gi * 8 + i] = max(m[gi * 8 + i], field_of(a[i][u]));
+            }
+            for (; k < o.K; k += 64) {
+#pragma unroll
+                for (int i = 0; i < 8; i++) m[gi * 8 + i] = max(m[gi * 8 + i], field_of(rowp[i][k]));
+            }
+        }
+        // transpose-reduce through LDS: [32 rows][64 lanes]
+        int32_t *red = reinterpret_cast<int32_t *>(wt);
+#pragma unroll
+        for (int i = 0; i < 32; i++) red[i * 64 + lane] = m[i];
+        emax = 0;
+        if (lane < 32) {
+#pragma unroll
+            for (int q = 0; q < 64; q += 4) {
+                const int4 x = *reinterpret_cast<const int4 *>(red + lane * 64 + q);
+                emax = max(emax, max(max(x.x, x.y), max(x.z, x.w)));
+            }
+        }
+    } else {
+        // lane l: row r0 + l % 32, contraction indices of parity l / 32
+        const int64_t r = r0 + (lane & 31);
+        const double *col = o.X + (r < o.R ? r : o.R - 1);
+        int m = 0;
+        int64_t k = lane >> 5;
+        for (; k + 2 * 63 < o.K; k += 128) {
+            double a[64];
+#pragma unroll
+            for (int u = 0; u < 64; u++) a[u] = col[(k + 2 * u) * o.sk];
+#pragma unroll
+            for (int u = 0; u < 64; u++) m = max(m, field_of(a[u]));
+        }
+        for (; k < o.K; k += 2) m = max(m, field_of(col[k * o.sk]));
+        emax = max(m, __shfl_xor(m, 32, 64));
+    }
+    if (r0 + (lane & 31) >= o.R) emax = 0;
+    if (lane < 32) o.E[r0 + lane] = emax;
+    emax = __shfl(emax, lane & 31, 64);
+    // 2. split, tiles of 64 contraction indices (Kb is even), two tiles in flight
+    const int64_t nt = o.Kb / 2;
+    const int64_t plane = o.Rb * o.Kb * (int64_t)kBlkBytes;
+    const int row = lane & 31, half = lane >> 5;
+    auto emit = [&](int64_t t) {
+#pragma unroll
+        for (int blk = 0; blk < 2; blk++) {
+            int8_t *dst = o.S + (rb * o.Kb + 2 * t + blk) * (int64_t)kBlkBytes + lane * 16;
+            wave_emit_chunk(wt, row, 32 * blk + 16 * half, emax, b, d, dst, plane);
+        }
+    };
+    double v0[32], v1[32];
+    wave_gather<KC>(o, r0, 0, lane, v0);
+    if (nt > 1) wave_gather<KC>(o, r0, 64, lane, v1);
+    for (int64_t t = 0; t < nt; t += 2) {
+        wave_stage<KC>(v0, wt, lane);
+        if (t + 2 < nt) wave_gather<KC>(o, r0, (t + 2) * 64, lane, v0);
+        emit(t);
+        if (t + 1 < nt) {
+            wave_stage<KC>(v1, wt, lane);
+            if (t + 3 < nt) wave_gather<KC>(o, r0, (t + 3) * 64, lane, v1);
+            emit(t + 1);
+        }
+    }
+}
+
+struct PersistCtl {
+    int arrive;   // workgroups through the phase barrier
+    int done;     // workgroups finished (the last one resets both counters)
+    int error;    // sticky: a barrier wait exceeded its bound
+    int pad[61];
+};
+
+constexpr int kSpinLimit = 1 << 20;   // barrier polls (each a device-coherent load + sleep)
+
+// Grid barrier: every wave makes its stores visible at agent scope, one thread per
+// workgroup arrives and polls; a wait longer than kSpinLimit polls (or an error raised by
+// another workgroup) sets the error flag and releases everyone.  Returns false on error.
+__device__ bool grid_barrier(PersistCtl *ctl, int G, int *flag_lds) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        __hip_atomic_fetch_add(&ctl->arrive, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int spin = 0;; spin++) {
+            if (__hip_atomic_load(&ctl->arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= G) break;
+            if (spin >= kSpinLimit || __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                __hip_atomic_store(&ctl->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        *flag_lds = ok;
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return *flag_lds != 0;
+}
+
+// Operands of one tile of phase B (all wave-uniform: the lane offset is added at the loads).
+struct PTile {
+    const int8_t *pa, *pb;
+    int64_t planeA, planeB, rowstride;
+    int nk, rbase, cbase;
+    int M, N;
+    int64_t ldc;
+    double *C;
+    const int32_t *Er, *Ec;
+};
+
+template <int WM, int WN>
+__device__ __forceinline__ PTile ptile_setup(const GemmLaunch *__restrict__ probs, int4 it, int wm, int wn) {
+    const GemmLaunch &g = probs[it.x];
+    PTile t;
+    const int64_t rowstride = g.Kb * kBlkBytes;
+    const int64_t rbA = (int64_t)it.y * 2 * WM + wm * WM, rbB = (int64_t)it.z * 2 * WN + wn * WN;
+    t.pa = g.SA + rbA * rowstride;
+    t.pb = g.SB + rbB * rowstride;
+    t.planeA = g.RbA * rowstride;
+    t.planeB = g.RbB * rowstride;
+    t.rowstride = rowstride;
+    t.nk = (int)g.Kb;
+    t.rbase = it.y * 64 * WM + wm * 32 * WM;
+    t.cbase = it.z * 64 * WN + wn * 32 * WN;
+    t.M = g.M;
+    t.N = g.N;
+    t.ldc = g.ldc;
+    t.C = g.C;
+    t.Er = g.Er;
+    t.Ec = g.Ec;
+    return t;
+}
+
+template <int D>
+__host__ __device__ constexpr int persist_lds_bytes() {
+    constexpr int a = 4 * 32 * kWaveTileLd * 8;   // phase A: one tile per wave
+    return a > direct_tile_bytes<D>() ? a : direct_tile_bytes<D>();
+}
+
+template <int D, int WM, int WN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_grp_persistent(const GroupOperand *__restrict__ ops, const int4 *__restrict__ itemsA,
+                      const int *__restrict__ offA, const GemmLaunch *__restrict__ probs,
+                      const int4 *__restrict__ itemsB, const int *__restrict__ offB, PersistCtl *ctl,
+                      int b, int rounding) {
+    __shared__ __attribute__((aligned(1024))) int8_t smem[persist_lds_bytes<D>()];
+    __shared__ int flag_lds;
+    const int G = gridDim.x;
+
+    // ---- phase A: exponent maximum + split, one 32-row block per item, each wave working
+    // through its own item list (no workgroup barriers)
+    {
+        const int lane = threadIdx.x & 63;
+        const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        double *wt = reinterpret_cast<double *>(smem) + w * (32 * kWaveTileLd);
+        const int slot = blockIdx.x * 4 + w;
+        const int a0 = offA[slot], a1 = offA[slot + 1];
+        for (int i = a0; i < a1; i++) {
+            const int4 it = itemsA[i];
+            const GroupOperand &o = ops[it.x];
+            if (o.sk == 1) wave_rowblock<true>(o, it.y, b, D, wt, lane);
+            else wave_rowblock<false>(o, it.y, b, D, wt, lane);
+        }
+    }
+
+    const bool ok = grid_barrier(ctl, G, &flag_lds);
+
+    // ---- phase B: fused slice-GEMM tiles
+    if (ok) {
+        const int lane = threadIdx.x & 63;
+        const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const int wm = w & 1, wn = w >> 1;
+        int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * wave_scratch_words<D, WM, WN>();
+        const int i0 = offB[blockIdx.x], i1 = offB[blockIdx.x + 1];
+        const int loff = lane * 16;
+
+        v4i a0[D][WM], b0[D][WN], a1[D][WM], b1[D][WN];
+        auto load = [&](v4i (&af)[D][WM], v4i (&bf)[D][WN], const PTile &t, int kb) {
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+#pragma unroll
+                for (int mi = 0; mi < WM; mi++)
+                    af[j][mi] = *reinterpret_cast<const v4i *>(t.pa + j * t.planeA + mi * t.rowstride +
+                                                              (int64_t)kb * kBlkBytes + loff);
+#pragma unroll
+                for (int ni = 0; ni < WN; ni++)
+                    bf[j][ni] = *reinterpret_cast<const v4i *>(t.pb + j * t.planeB + ni * t.rowstride +
+                                                              (int64_t)kb * kBlkBytes + loff);
+            }
+        };
+        v16i acc[D][WM][WN];
+        auto mma = [&](const v4i (&af)[D][WM], const v4i (&bf)[D][WN]) {
+#pragma unroll
+            for (int i = 0; i < D; i++)
+#pragma unroll
+                for (int j = 0; i + j < D; j++)
+#pragma unroll
+                    for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+                        for (int ni = 0; ni < WN; ni++)
+                            acc[i + j][mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i][mi], bf[j][ni],
+                                                                                       acc[i + j][mi][ni], 0, 0, 0);
+        };
+
+        PTile cur{}, nxt{};
+        if (i0 < i1) {
+            cur = ptile_setup<WM, WN>(probs, itemsB[i0], wm, wn);
+            load(a0, b0, cur, 0);
+        }
+        for (int i = i0; i < i1; i++) {
+            const bool has_next = i + 1 < i1;
+            if (has_next) nxt = ptile_setup<WM, WN>(probs, itemsB[i + 1], wm, wn);
+            GemmLaunch g{};
+            g.M = cur.M;
+            g.N = cur.N;
+            g.b = b;
+            g.d = D;
+            g.rounding = rounding;
+            g.first = 1;
+            g.last = 1;
+            g.C = cur.C;
+            g.ldc = cur.ldc;
+            g.Er = cur.Er;
+            g.Ec = cur.Ec;
+            int er_v, ec_v;
+            load_tile_exps(g, EPI_C, lane, cur.rbase, 32 * WM, cur.cbase, 32 * WN, er_v, ec_v);
+#pragma unroll
+            for (int k = 0; k < D; k++)
+#pragma unroll
+                for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+                    for (int ni = 0; ni < WN; ni++) acc[k][mi][ni] = v16i{};
+            const int nk = cur.nk;   // even
+            for (int t = 0; t < nk; t += 2) {
+                load(a1, b1, cur, t + 1);
+                mma(a0, b0);
+                if (t + 2 < nk) load(a0, b0, cur, t + 2);
+                else if (has_next) load(a0, b0, nxt, 0);   // next tile's first block
+                mma(a1, b1);
+            }
+            wave_epilogue<D, WM, WN>(g, EPI_C, 0, acc, scratch, lane, cur.rbase, cur.cbase, er_v, ec_v);
+            cur = nxt;
+        }
+    }
+
+    // ---- exit: the last workgroup resets the counters for the next launch
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int old = __hip_atomic_fetch_add(&ctl->done, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == G - 1) {
+            __hip_atomic_store(&ctl->arrive, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctl->done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+bool persist_cfg_supported(int d, int cfg) {
+    if (d != 3 && d != 7 && d != 8 && d != 9) return false;
+    if (cfg == GCFG_D11) return true;
+    if (cfg == GCFG_D12) return d == 3 || d == 7;   // d = 8 and 9 would spill registers
+    if (cfg == GCFG_D22) return d == 3;
+    return false;
+}
+
+int persist_ctl_bytes() { return (int)sizeof(PersistCtl); }
+
+template <int D, int WM, int WN>
+static hipError_t launch_persist_t(const PersistLaunch &L, hipStream_t st) {
+    hipLaunchKernelGGL((k_grp_persistent<D, WM, WN>), dim3((unsigned)L.grid), dim3(256), 0, st, L.ops, L.itemsA,
+                       L.offA, L.probs, L.itemsB, L.offB, reinterpret_cast<PersistCtl *>(L.ctl), L.b,
+                       L.rounding);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_persistent(const PersistLaunch &L, int d, int cfg, hipStream_t st) {
+    if (L.grid <= 0) return hipSuccess;
+    if (!persist_cfg_supported(d, cfg)) return hipErrorInvalidValue;
+    switch (d * 8 + cfg) {
+        case 3 * 8 + GCFG_D11: return launch_persist_t<3, 1, 1>(L, st);
+        case 3 * 8 + GCFG_D12: return launch_persist_t<3, 1, 2>(L, st);
+        case 3 * 8 + GCFG_D22: return launch_persist_t<3, 2, 2>(L, st);
+        case 7 * 8 + GCFG_D11: return launch_persist_t<7, 1, 1>(L, st);
+        case 7 * 8 + GCFG_D12: return launch_persist_t<7, 1, 2>(L, st);
+        case 8 * 8 + GCFG_D11: return launch_persist_t<8, 1, 1>(L, st);
+        case 9 * 8 + GCFG_D11: return launch_persist_t<9, 1, 1>(L, st);
         default: return hipErrorInvalidValue;
     }
 }
