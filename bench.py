@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the long-K, grouped and autograd sections")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--grouped-only", action="store_true",
+                    help="only the 512-problem grouped section (prints its record as JSON)")
     ap.add_argument("--profile-run", action="store_true",
                     help="only run each preset's GEMM twice and one 1 GiB device copy (for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -155,19 +157,30 @@ def load_traffic(d, n):
         return None
     try:
         t = json.load(open(path))
-        return t.get(f"gemm_d{d}_n{n}")
+        if t.get("workload_n") != n:
+            return None
+        hits = [x for x in t["dispatches"] if "k_gemm_" in x["kernel"] and f"<{d}," in x["kernel"]]
+        return hits[-1]["hbm_bytes"] if hits else None   # the second call of the preset
     except Exception:
         return None
 
 
 def cpu_baseline(args, np, Ah, Bh, d, n):
     import oracle
-    # bounded sample: the first rows of C (the whole of B is split, as the GPU does)
+    # bounded sample: the first rows of C (the whole of B is split, as the GPU does); a
+    # 64-row probe sizes the sample to about OZAIE_CPU_SECONDS (default 10) of CPU work
+    target = float(os.environ.get("OZAIE_CPU_SECONDS", "10"))
     ms = max(8, min(n, int(os.environ.get("OZAIE_CPU_ROWS", "64"))))
-    A_s = np.ascontiguousarray(Ah[:ms])
     t0 = time.time()
-    _, ov = oracle.ozaki_dgemm(A_s, Bh, 7, d)
+    oracle.ozaki_dgemm(np.ascontiguousarray(Ah[:ms]), Bh, 7, d)
     t = time.time() - t0
+    if "OZAIE_CPU_ROWS" not in os.environ and t < target:
+        ms = max(ms, min(n, int(ms * target / max(t, 1e-3))))
+        A_s = np.ascontiguousarray(Ah[:ms])
+        t0 = time.time()
+        oracle.ozaki_dgemm(A_s, Bh, 7, d)
+        t = time.time() - t0
+    A_s = np.ascontiguousarray(Ah[:ms])
     rec = {"value": 2.0 * ms * n * n / t / 1e12, "unit": "TFLOP/s (effective FP64)",
            "cores": oracle.num_threads(), "kind": "port",
            "sample": f"{ms} of {n} rows of C, b=7 d={d}, full split of B; {t:.2f} s"}
@@ -255,8 +268,10 @@ def long_k(args, torch, np, oz, slices=(7, 8, 9)):
 
 
 def grouped(args, torch, np, oz, slices=(3, 7, 8, 9), count=512):
-    """512 independent products with M, N, K log-uniform in [32, 2048], phi = 1, in one
-    captured graph per preset, against native DGEMM called per problem."""
+    """512 independent products with M, N, K log-uniform in [32, 2048], phi = 1, per preset
+    (a) in one captured graph of per-phase launches with the autotuner's tile table and
+    (b) as one persistent launch, against native DGEMM called per problem (eager and
+    replayed from a captured graph)."""
     rng = np.random.default_rng(args.seed + 4)
     shapes = [tuple(int(v) for v in row) for row in
               np.exp(rng.uniform(np.log(32), np.log(2048), size=(count, 3))).astype(np.int64)]
@@ -313,16 +328,39 @@ def grouped(args, torch, np, oz, slices=(3, 7, 8, 9), count=512):
                       "graph_ms": t_nat_graph,
                       "graph_tflops": flop / t_nat_graph / 1e9 if t_nat_graph else None}, "presets": []}
     for d in slices:
-        grp = oz.Group(shapes, 7, d)
-        t = time_steps(torch, lambda: grp(As, Bs, Cs), 5, 3)
-        err = errs()
-        q = grp.query()
-        rec["presets"].append({"bits": 7, "slices": d, "ms": t, "tflops": flop / t / 1e9, "speedup": t_nat / t,
-                               "speedup_vs_native_graph": (t_nat_graph / t) if t_nat_graph else None,
-                               "max_rel_err": err[0], "median_rel_err": err[1], "launches": q["launches"],
-                               "graph": q["graph"], "workgroups": q["workgroups"]})
-        log(args, f"  grouped d={d}: {t:.2f} ms x{t_nat / t:.2f} err {err} launches {q['launches']}")
-        grp.close()
+        # tile configuration per size class from the autotuner (multi-launch mode), then
+        # the same products as ONE persistent launch; both must give identical C
+        t0 = time.time()
+        table = [int(x) for x in oz.autotune(7, d, reps=2)]
+        t_tune = time.time() - t0
+        entry = {"bits": 7, "slices": d, "autotune_table": table, "autotune_s": t_tune}
+        ref = None
+        for mode in ("multi", "persistent"):
+            grp = oz.Group(shapes, 7, d, persistent=(mode == "persistent"))
+            t = time_steps(torch, lambda: grp(As, Bs, Cs), 5, 3)
+            q = grp.query()
+            status = grp.status()
+            err = errs()
+            if ref is None:
+                ref = [c.clone() for c in Cs]
+                same = True
+            else:
+                same = all(torch.equal(a, b) for a, b in zip(Cs, ref))
+            entry[mode] = {"ms": t, "tflops": flop / t / 1e9, "speedup": t_nat / t,
+                           "speedup_vs_native_graph": (t_nat_graph / t) if t_nat_graph else None,
+                           "launches": q["launches"], "graph": q["graph"], "workgroups": q["workgroups"],
+                           "barrier_error": status, "identical_to_multi": same}
+            entry["max_rel_err"], entry["median_rel_err"] = err
+            log(args, f"  grouped d={d} {mode}: {t:.2f} ms x{t_nat / t:.2f} err {err} launches {q['launches']}"
+                      f" same {same} status {status}")
+            grp.close()
+        del ref
+        best = min(("multi", "persistent"), key=lambda m: entry[m]["ms"])
+        entry.update({"best_mode": best, "ms": entry[best]["ms"], "tflops": entry[best]["tflops"],
+                      "speedup": entry[best]["speedup"],
+                      "speedup_vs_native_graph": entry[best]["speedup_vs_native_graph"]})
+        rec["presets"].append(entry)
+        oz.set_tune_table(d, [-1] * 9)
     log(args, f"  grouped native {t_nat:.2f} ms (graph {t_nat_graph}) err {nat_err}")
     del As, Bs, Cs
     torch.cuda.empty_cache()
@@ -363,8 +401,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1"}))
-        print("bench.py measures the single-GPU emulated DGEMM; multi-GPU runs are not part of "
-              "this build (see the sharding section of DESIGN.md)", file=sys.stderr)
+        print("bench.py measures the single-GPU emulated DGEMM; runs on several GPUs are not "
+              "measured by this benchmark", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
@@ -376,6 +414,9 @@ def main():
     if args.profile_run:
         profile_run(args, torch, oz, n, slices)
         return
+    if args.grouped_only:
+        print(json.dumps(grouped(args, torch, np, oz, slices=tuple(slices))), flush=True)
+        return 0
     main_res, (Ah, Bh) = measure(args, torch, np, oz, n, args.steps, args.warmup, args.block, slices)
     m = main_res["matched"]
     d = m["slices"]
