@@ -44,6 +44,10 @@ extern "C" {
 #define OZAIE_MODE_FAST 1
 #define OZAIE_MODE_FAST3 2
 #define OZAIE_MODE_ACCURATE7 3
+#define OZAIE_MODE_CRT 4        /* no reference counterpart: exact integer product of the
+                                   operands scaled to the largest bit count that 16 int8
+                                   residue GEMMs rebuild exactly (ozaie_plan_create, flag
+                                   OZAIE_PLAN_CRT); K <= 65536                           */
 
 /* ---- options (ozaie_set_option) */
 #define OZAIE_OPT_ROUNDING 1    /* OZAIE_ROUND_TRUNCATE (default) or OZAIE_ROUND_NEAREST   */
@@ -76,7 +80,8 @@ int32_t ozaie_set_option(int option, int value);
 int32_t ozaie_get_option(int option);
 
 /* Number of (b, d) and passes of the current global plan:
- * out[0]=b out[1]=d out[2]=k-chunk out[3]=passes out[4]=kernel (0 fused, 1 group)       */
+ * out[0]=b out[1]=d out[2]=k-chunk out[3]=passes out[4]=kernel (0 fused, 1 group,
+ * 2 CRT mode: then b = bits per operand and d = number of moduli)                       */
 int32_t ozaie_plan_info(int32_t *out5);
 
 /* Release every device buffer and plan (reference: cleanup_xrt, include/oz_aie.h:153). */
@@ -86,7 +91,12 @@ int32_t ozaie_release(void);
 typedef struct ozaie_plan_s *ozaie_plan;
 
 /* Create a plan for C (M x N) = op(A) (M x K) * op(B) (K x N) with b bits and d slices.
- * flags: bit 0 -> nearest-even rounding (default truncation).                          */
+ * flags: bit 0 -> nearest-even rounding (default truncation).
+ * flags bit 2 (OZAIE_PLAN_CRT) -> CRT mode: `slices` = number of moduli (2..16), `bits`
+ * = bits per operand (0 = the largest the moduli rebuild exactly for this K).  No
+ * reference counterpart; K <= 65536.                                                   */
+#define OZAIE_PLAN_NEAREST 1
+#define OZAIE_PLAN_CRT 4
 int32_t ozaie_plan_create(ozaie_plan *plan, int64_t M, int64_t N, int64_t K, int bits,
                           int slices, int flags);
 int32_t ozaie_plan_destroy(ozaie_plan plan);
@@ -154,6 +164,12 @@ int32_t ozaie_tune_get(int slices, int32_t *table9);   /* -1 entries: untuned */
 int32_t ozaie_tune_set(int slices, const int32_t *table9);
 
 /* ---- inspection (reference full_dump equivalent, whole_array_iron_ozaki.py:485-558)  */
+
+/* CRT-mode intermediates of C = A * B (row-major device A, B), as computed on the GPU:
+ * SA_out int8[moduli][M][K] and SB_out int8[moduli][N][K] residues (host), G_out
+ * uint8[moduli][M][N] residue products mod p (host).  No reference counterpart.        */
+int32_t ozaie_debug_crt(const double *A, const double *B, int64_t M, int64_t N, int64_t K, int moduli,
+                        int bits, int8_t *SA_out, int8_t *SB_out, uint8_t *G_out);
 
 /* Exponents and slices of one operand, as computed on the GPU, in plain layout.
  * X is a device pointer to a rows x K view: element (r, k) at X[r*sr + k*sk].

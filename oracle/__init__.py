@@ -65,6 +65,21 @@ def _setup(L):
     L.oro_ozaki_dgemm.restype = i32
     L.oro_naive_dgemm.argtypes = [i64, i64, i64, P, P, P]
     L.oro_num_threads.restype = i32
+    # CRT mode (no reference counterpart)
+    L.oro_crt_modulus.argtypes = [i32]
+    L.oro_crt_modulus.restype = i32
+    L.oro_crt_beta.argtypes = [i32, i64]
+    L.oro_crt_beta.restype = i32
+    L.oro_crt_consts.argtypes = [i32, P, P, P]
+    L.oro_crt_q.argtypes = [ctypes.c_double, i32, i32]
+    L.oro_crt_q.restype = i64
+    L.oro_crt_split.argtypes = [P, i64, i64, i64, i64, P, i32, i32, P]
+    L.oro_crt_products.argtypes = [P, P, i64, i64, i64, i32, P]
+    L.oro_crt_rebuild.argtypes = [P, i64, i32, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+    L.oro_crt_to_double.argtypes = [i64, i64, i32, i32]
+    L.oro_crt_to_double.restype = ctypes.c_double
+    L.oro_crt_dgemm.argtypes = [i64, i64, i64, P, i64, P, i64, P, i64, i32, i32]
+    L.oro_crt_dgemm.restype = i32
     if hasattr(L, "oro_mpfr_dot"):
         L.oro_mpfr_dot.argtypes = [P, i64, P, i64, i64, i32]
         L.oro_mpfr_dot.restype = ctypes.c_double
@@ -213,3 +228,83 @@ def mpfr_sample(A, B, rows, cols, prec=106):
 
 def num_threads():
     return lib().oro_num_threads()
+
+
+# ---------------------------------------------------------------- CRT mode
+# Not in the reference: exact integer product of per-row / per-column scaled operands,
+# computed from int8 residue GEMMs modulo 16 pairwise-coprime moduli and rebuilt with
+# the Chinese remainder theorem (crt_oracle.c states the arithmetic).
+def crt_moduli(n=16):
+    return [lib().oro_crt_modulus(i) for i in range(n)]
+
+
+def crt_beta(n, K):
+    return lib().oro_crt_beta(int(n), int(K))
+
+
+def crt_consts(n):
+    W = np.zeros(2 * n, dtype=np.uint64)
+    f = np.zeros(n, dtype=np.float64)
+    Mw = np.zeros(2, dtype=np.uint64)
+    lib().oro_crt_consts(int(n), _p(W), _p(f), _p(Mw))
+    return [int(W[2 * i]) | (int(W[2 * i + 1]) << 64) for i in range(n)], f, int(Mw[0]) | (int(Mw[1]) << 64)
+
+
+def crt_q(x, emax, beta):
+    return lib().oro_crt_q(float(x), int(emax), int(beta))
+
+
+def crt_split(X, E, beta, n, axis_contract=1):
+    """Residues int8[n][rows][K] of X (rows along the other axis), E per row."""
+    X = np.asarray(X, dtype=np.float64)
+    if axis_contract == 1:
+        rows, K = X.shape
+        sr, sc = X.strides[0] // 8, X.strides[1] // 8
+    else:
+        K, rows = X.shape
+        sr, sc = X.strides[1] // 8, X.strides[0] // 8
+    E = np.ascontiguousarray(E, dtype=np.int16)
+    S = np.zeros((n, rows, K), dtype=np.int8)
+    lib().oro_crt_split(_p(X), rows, K, sr, sc, _p(E), int(beta), int(n), _p(S))
+    return S
+
+
+def crt_products(SA, SB):
+    """g[i][r][c] = (SA[i][r] . SB[i][c]) mod p_i in [0, p_i)."""
+    SA = np.ascontiguousarray(SA, dtype=np.int8)
+    SB = np.ascontiguousarray(SB, dtype=np.int8)
+    n, M, K = SA.shape
+    N = SB.shape[1]
+    G = np.zeros((n, M, N), dtype=np.uint8)
+    lib().oro_crt_products(_p(SA), _p(SB), M, N, K, n, _p(G))
+    return G
+
+
+def crt_rebuild(g):
+    """Signed integer from the residues g[0..n) (Python int)."""
+    g = np.ascontiguousarray(g, dtype=np.uint8)
+    hi = ctypes.c_int64()
+    lo = ctypes.c_int64()
+    lib().oro_crt_rebuild(_p(g), 1, len(g), ctypes.byref(hi), ctypes.byref(lo))
+    v = ((hi.value & (2**64 - 1)) << 64) | (lo.value & (2**64 - 1))
+    return v - 2**128 if v >= 2**127 else v
+
+
+def crt_to_double(v, e2, rounding=1):
+    v &= 2**128 - 1
+    hi, lo = v >> 64, v & (2**64 - 1)
+    to_s = lambda w: w - 2**64 if w >= 2**63 else w
+    return lib().oro_crt_to_double(to_s(hi), to_s(lo), int(e2), int(rounding))
+
+
+def crt_dgemm(A, B, n=16, rounding=1):
+    """End-to-end CPU CRT-mode DGEMM."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    M, K = A.shape
+    N = B.shape[1]
+    C = np.zeros((M, N), dtype=np.float64)
+    rc = lib().oro_crt_dgemm(M, N, K, _p(A), K, _p(B), N, _p(C), N, int(n), int(rounding))
+    if rc:
+        raise ValueError("no bit count fits these moduli and K")
+    return C

@@ -1,6 +1,7 @@
 """Build recipe for the oracle's shared libraries (test infrastructure only).
 
-Compiles oracle/ozaki_oracle.c (CPU restatement of the reference Ozaki path) and
+Compiles oracle/ozaki_oracle.c (CPU restatement of the reference Ozaki path),
+oracle/crt_oracle.c (the library's CRT mode, which has no reference counterpart) and
 oracle/mpfr_oracle.c (the reference's MPFR-106 accuracy oracle) with gcc into
 oracle/liboz_oracle.so.  The reference itself (AIE C++ kernels needing aie_api and the
 Peano compiler, host code needing XRT) cannot be built in this image, so there is no
@@ -18,7 +19,7 @@ MPFR_LIBDIRS = ["/opt/conda/lib", "/lib/x86_64-linux-gnu"]
 
 
 def build(verbose=False):
-    srcs = [os.path.join(HERE, "ozaki_oracle.c")]
+    srcs = [os.path.join(HERE, "ozaki_oracle.c"), os.path.join(HERE, "crt_oracle.c")]
     cmd = ["gcc", "-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", "-std=gnu11"]
     link = []
     if os.path.exists(os.path.join(MPFR_INC, "mpfr.h")):

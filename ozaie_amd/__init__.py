@@ -11,6 +11,7 @@ missing, every call raises.
 """
 import ctypes
 import enum
+import math
 import os
 
 import numpy as np
@@ -46,13 +47,14 @@ class Mode(enum.IntEnum):
     fast = 1       # reference: b = 7, d = 8
     fast3 = 2      # b = 7, d = 3
     accurate7 = 3  # b = 7, d = 7
+    crt = 4        # no reference counterpart: 16 int8 residue GEMMs, exact integer product
 
 
 # C symbols declared in include/ozaie.h (the tests check that every one is exported).
 EXPORTS = [
     "ozAIE_setup", "ozAIE_setup_slices", "ozAIE_DGEMM", "ozaie_set_option", "ozaie_get_option",
     "ozaie_plan_info", "ozaie_release", "ozaie_plan_create", "ozaie_plan_destroy",
-    "ozaie_plan_query", "ozaie_dgemm", "ozaie_debug_split", "ozaie_debug_bins",
+    "ozaie_plan_query", "ozaie_dgemm", "ozaie_debug_split", "ozaie_debug_bins", "ozaie_debug_crt",
     "ozaie_paired_output", "ozaie_last_error", "ozaie_version", "ozaie_plan_phase_times",
     "ozaie_exact_entries", "ozaie_group_create", "ozaie_group_run", "ozaie_group_query",
     "ozaie_group_configs", "ozaie_group_status", "ozaie_group_destroy", "ozaie_dgemm_grouped", "ozaie_autotune",
@@ -118,6 +120,8 @@ def _declare(L):
     L.ozaie_dgemm.restype = i32
     L.ozaie_debug_split.argtypes = [P, i64, i64, i64, i64, c_int, c_int, P, P]
     L.ozaie_debug_split.restype = i32
+    L.ozaie_debug_crt.argtypes = [P, P, i64, i64, i64, c_int, c_int, P, P, P]
+    L.ozaie_debug_crt.restype = i32
     L.ozaie_debug_bins.argtypes = [i64, i64, i64, P, P, c_int, c_int, c_int, P]
     L.ozaie_debug_bins.restype = i32
     L.ozaie_paired_output.argtypes = [i64, i64, i64, P, P, c_int, c_int, P]
@@ -174,7 +178,21 @@ def mode_slices(mode, K):
                  13: (3, 25), 14: (4, 18)}
         lk = min(max(int(math.log2(K)), 7), 14)
         return table[lk]
+    if mode == Mode.crt:
+        return crt_beta(16, K), 16
     return {Mode.fast: (7, 8), Mode.fast3: (7, 3), Mode.accurate7: (7, 7)}[mode]
+
+
+CRT_MODULI = (256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211, 199, 197, 193, 191)
+
+
+def crt_beta(moduli, K):
+    """Bits per operand of the CRT mode: the largest beta with K 2^(2 beta) <= 2^(L-2),
+    L = floor(log2 prod p), at most 62 (mirrors the library's planner)."""
+    L = math.prod(CRT_MODULI[:moduli]).bit_length() - 1
+    lk = (int(K) - 1).bit_length()
+    beta = min(62, (L - 2 - lk) // 2)
+    return beta if beta >= 1 else -1
 
 
 # ---------------------------------------------------------------- reference-style API
@@ -228,13 +246,15 @@ def _stream_handle(stream):
 
 
 class Plan:
-    """A device plan for C (M x N) = op(A) (M x K) @ op(B) (K x N) with b bits, d slices."""
+    """A device plan for C (M x N) = op(A) (M x K) @ op(B) (K x N) with b bits, d slices.
+    crt=True: CRT mode, `slices` moduli (2..16) and `bits` per operand (0 = largest exact)."""
 
-    def __init__(self, M, N, K, bits=7, slices=8, rounding=ROUND_TRUNCATE):
+    def __init__(self, M, N, K, bits=7, slices=8, rounding=ROUND_TRUNCATE, crt=False):
         self.M, self.N, self.K, self.bits, self.slices = int(M), int(N), int(K), int(bits), int(slices)
         h = ctypes.c_void_p()
+        flags = (1 if rounding == ROUND_NEAREST else 0) | (4 if crt else 0)
         _check(lib().ozaie_plan_create(ctypes.byref(h), self.M, self.N, self.K, self.bits,
-                                       self.slices, 1 if rounding == ROUND_NEAREST else 0),
+                                       self.slices, flags),
                "ozaie_plan_create")
         self._h = h
 
@@ -242,7 +262,7 @@ class Plan:
         out = np.zeros(5, dtype=np.int32)
         _check(lib().ozaie_plan_query(self._h, _ptr(out)), "ozaie_plan_query")
         return dict(bits=int(out[0]), slices=int(out[1]), kchunk=int(out[2]), passes=int(out[3]),
-                    kernel=("fused", "group")[int(out[4])])
+                    kernel=("fused", "group", "crt")[int(out[4])])
 
     def __call__(self, A, B, C=None, transA=False, transB=False, stream=None):
         """C = op(A) @ op(B) on device float64 tensors (asynchronous on `stream`)."""
@@ -307,6 +327,19 @@ def debug_split(X, bits, slices, contract_axis=1):
     _check(lib().ozaie_debug_split(_ptr(X), rows, K, sr, sk, bits, slices, _ptr(E), _ptr(S)),
            "ozaie_debug_split")
     return E, S
+
+
+def debug_crt(A, B, moduli=16, bits=0):
+    """CRT-mode intermediates of A @ B (device float64, row-major): residues int8
+    [n][M][K] and [n][N][K], residue products uint8 [n][M][N]."""
+    M, K = A.shape
+    N = B.shape[1]
+    SA = np.zeros((moduli, M, K), dtype=np.int8)
+    SB = np.zeros((moduli, N, K), dtype=np.int8)
+    G = np.zeros((moduli, M, N), dtype=np.uint8)
+    _check(lib().ozaie_debug_crt(_ptr(A), _ptr(B), M, N, K, moduli, bits, _ptr(SA), _ptr(SB), _ptr(G)),
+           "ozaie_debug_crt")
+    return SA, SB, G
 
 
 def debug_bins(A, B, bits, slices, kernel=0):

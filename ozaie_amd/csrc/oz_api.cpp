@@ -66,6 +66,60 @@ int ensure_device() {
 namespace oz {
 int set_error(hipError_t e, const char *where) { return set_hip_error(e, where); }
 void set_error_msg(const char *msg) { g_last_error = msg; }
+
+// CRT mode constants: the first n of 16 pairwise-coprime moduli <= 256 (256 = 2^8 and odd
+// numbers whose prime factors are all distinct), M = prod p < 2^126.
+static const int kCrtModuli[kCrtMaxMod] = {256, 255, 253, 251, 247, 241, 239, 233,
+                                           229, 227, 223, 211, 199, 197, 193, 191};
+
+int crt_modulus(int i) { return (i >= 0 && i < kCrtMaxMod) ? kCrtModuli[i] : 0; }
+
+static unsigned __int128 crt_M(int n) {
+    unsigned __int128 m = 1;
+    for (int i = 0; i < n; i++) m *= (unsigned __int128)kCrtModuli[i];
+    return m;
+}
+
+// Largest beta with K 2^(2 beta) <= 2^(floor(log2 M) - 2) <= M / 4 (at most 62).
+int crt_beta(int n, int64_t K) {
+    if (n < 2 || n > kCrtMaxMod || K < 1) return -1;
+    unsigned __int128 m = crt_M(n);
+    int L = -1;
+    while (m) { m >>= 1; L++; }
+    int lk = 0;
+    while (((int64_t)1 << lk) < K) lk++;
+    const int beta = std::min(62, (L - 2 - lk) / 2);
+    return beta >= 1 ? beta : -1;
+}
+
+void crt_consts(int n, int beta, CrtConsts &c) {
+    std::memset(&c, 0, sizeof(c));
+    c.n = n;
+    c.beta = beta;
+    const unsigned __int128 M = crt_M(n);
+    for (int i = 0; i < n; i++) {
+        const int p = kCrtModuli[i];
+        const unsigned __int128 Mi = M / (unsigned __int128)p;
+        // inverse of Mi mod p by the extended Euclidean algorithm
+        int t = 0, nt = 1, r = p, nr = (int)(Mi % (unsigned __int128)p);
+        while (nr) {
+            const int qq = r / nr;
+            int tmp = t - qq * nt; t = nt; nt = tmp;
+            tmp = r - qq * nr; r = nr; nr = tmp;
+        }
+        if (t < 0) t += p;
+        const unsigned __int128 w = Mi * (unsigned __int128)t;   // < M
+        c.p[i] = p;
+        c.inv[i] = 1.0 / p;
+        c.c31[i] = (double)((1ull << 31) % (uint64_t)p);
+        c.half[i] = p == 256 ? 127.0 : (double)((p - 1) / 2);
+        c.f[i] = (double)w / (double)M;
+        c.W[i][0] = (uint64_t)w;
+        c.W[i][1] = (uint64_t)(w >> 64);
+    }
+    c.Mw[0] = (uint64_t)M;
+    c.Mw[1] = (uint64_t)(M >> 64);
+}
 }  // namespace oz
 
 struct ozaie_plan_s {
@@ -75,6 +129,9 @@ struct ozaie_plan_s {
     int64_t kcb = 0;      // contraction blocks per pass
     int nchunks = 0, ngroups = 1, kernel = 0, family = 0;
     int device = 0;
+    bool crt = false;     // CRT mode: b = bits per operand, d = number of moduli
+    CrtConsts cc{};
+    uint32_t *R = nullptr;  // CRT residue products [d][Mp/4][Np]
     int8_t *SA = nullptr, *SB = nullptr;
     int32_t *Er = nullptr, *Ec = nullptr;
     __int128 *T = nullptr;
@@ -98,6 +155,7 @@ void plan_free(ozaie_plan p) {
     if (p->Er) (void)hipFree(p->Er);
     if (p->Ec) (void)hipFree(p->Ec);
     if (p->T) (void)hipFree(p->T);
+    if (p->R) (void)hipFree(p->R);
     if (p->dA) (void)hipFree(p->dA);
     if (p->dB) (void)hipFree(p->dB);
     if (p->dC) (void)hipFree(p->dC);
@@ -154,6 +212,32 @@ int plan_configure(ozaie_plan p, int64_t M, int64_t N, int64_t K, int b, int d, 
     return OZAIE_OK;
 }
 
+// CRT mode: n moduli, beta bits per operand (0: the largest exact one for K).
+int plan_configure_crt(ozaie_plan p, int64_t M, int64_t N, int64_t K, int beta, int n, int rounding) {
+    if (M < 1 || N < 1 || K < 1) { g_last_error = "dimensions must be >= 1"; return OZAIE_ERR_DIMS; }
+    if (M > (1ll << 31) - 512 || N > (1ll << 31) - 512) { g_last_error = "dimensions too large"; return OZAIE_ERR_DIMS; }
+    if (K > 65536) { g_last_error = "CRT mode needs K <= 65536 (exact int32 residue sums)"; return OZAIE_ERR_DIMS; }
+    if (n < 2 || n > kCrtMaxMod) { g_last_error = "CRT mode needs 2 <= moduli <= 16"; return OZAIE_ERR_ARG; }
+    const int bmax = crt_beta(n, K);
+    if (bmax < 1) { g_last_error = "no exact bit count for these moduli and K"; return OZAIE_ERR_ARG; }
+    if (beta == 0) beta = bmax;
+    if (beta < 1 || beta > bmax) { g_last_error = "CRT bits exceed the exact range of the moduli"; return OZAIE_ERR_ARG; }
+    p->M = M; p->N = N; p->K = K; p->b = beta; p->d = n; p->rounding = rounding;
+    p->crt = true;
+    crt_consts(n, beta, p->cc);
+    p->Mp = round_up(M, kCrtRowPad);
+    p->Np = round_up(N, kCrtRowPad);
+    p->Kp = round_up(K, 4 * kBlk);   // the modulus GEMM prefetches four blocks
+    p->Kb = p->Kp / kBlk;
+    p->kcb = p->Kb;
+    p->nchunks = 1;
+    p->ngroups = 1;
+    p->kernel = 2;
+    p->family = g_opt_kernel >= 2 ? g_opt_kernel : KERNEL_AUTO;
+    if (p->Mp / kBlk > 65535 || p->Np / kBlk > 65535) { g_last_error = "too many rows for the split grid"; return OZAIE_ERR_DIMS; }
+    return OZAIE_OK;
+}
+
 int plan_alloc(ozaie_plan p, bool staging) {
     HIPCHK(hipGetDevice(&p->device));
     const size_t sa = (size_t)p->d * p->Mp * p->Kp, sb = (size_t)p->d * p->Np * p->Kp;
@@ -161,8 +245,9 @@ int plan_alloc(ozaie_plan p, bool staging) {
     HIPCHK(hipMalloc(&p->SB, sb));
     HIPCHK(hipMalloc(&p->Er, sizeof(int32_t) * p->Mp));
     HIPCHK(hipMalloc(&p->Ec, sizeof(int32_t) * p->Np));
-    if ((int64_t)p->nchunks * p->ngroups > 1)
+    if (!p->crt && (int64_t)p->nchunks * p->ngroups > 1)
         HIPCHK(hipMalloc(&p->T, sizeof(__int128) * (size_t)p->M * p->N));
+    if (p->crt) HIPCHK(hipMalloc(&p->R, (size_t)p->d * p->Mp * p->Np));
     if (staging) {
         HIPCHK(hipMalloc(&p->dA, sizeof(double) * (size_t)p->M * p->K));
         HIPCHK(hipMalloc(&p->dB, sizeof(double) * (size_t)p->K * p->N));
@@ -177,12 +262,12 @@ int plan_alloc(ozaie_plan p, bool staging) {
 }
 
 int plan_new(ozaie_plan *out, int64_t M, int64_t N, int64_t K, int b, int d, int rounding,
-             bool staging) {
+             bool staging, bool crt = false) {
     *out = nullptr;
     int rc = ensure_device();
     if (rc) return rc;
     ozaie_plan p = new ozaie_plan_s();
-    rc = plan_configure(p, M, N, K, b, d, rounding);
+    rc = crt ? plan_configure_crt(p, M, N, K, b, d, rounding) : plan_configure(p, M, N, K, b, d, rounding);
     if (!rc) rc = plan_alloc(p, staging);
     if (rc) { plan_free(p); return rc; }
     *out = p;
@@ -220,12 +305,26 @@ GemmLaunch base_launch(const ozaie_plan p) {
 int run_split(ozaie_plan p, const OperandView &va, const OperandView &vb, hipStream_t st) {
     HIPCHK(launch_maxexp(va, p->Er, p->Mp, st));
     HIPCHK(launch_maxexp(vb, p->Ec, p->Np, st));
+    if (p->crt) {
+        HIPCHK(launch_crt_split(va, p->Er, p->cc, p->SA, p->Mp, p->Kp, st));
+        HIPCHK(launch_crt_split(vb, p->Ec, p->cc, p->SB, p->Np, p->Kp, st));
+        return OZAIE_OK;
+    }
     HIPCHK(launch_split(va, p->Er, p->b, p->d, p->SA, p->Mp, p->Kp, st));
     HIPCHK(launch_split(vb, p->Ec, p->b, p->d, p->SB, p->Np, p->Kp, st));
     return OZAIE_OK;
 }
 
 int run_passes(ozaie_plan p, double *C, int64_t ldc, int epi, int32_t *bins, hipStream_t st) {
+    if (p->crt) {
+        CrtGemm cg;
+        cg.SA = p->SA; cg.SB = p->SB; cg.Mp = p->Mp; cg.Np = p->Np; cg.Kb = p->Kb; cg.R = p->R;
+        cg.variant = p->family == KERNEL_DIRECT ? 1 : 0;
+        HIPCHK(launch_crt_gemm(cg, p->cc, st));
+        if (epi == EPI_C)
+            HIPCHK(launch_crt_rebuild(p->R, p->Mp, p->Np, p->M, p->N, p->Er, p->Ec, C, ldc, p->rounding, p->cc, st));
+        return OZAIE_OK;
+    }
     GemmLaunch g = base_launch(p);
     g.C = C; g.ldc = ldc; g.bins = bins;
     const int npass = p->nchunks * p->ngroups;
@@ -302,6 +401,9 @@ int32_t ozAIE_setup(int M, int N, int K, int mode) {
         case OZAIE_MODE_FAST: b = 7; d = 8; break;
         case OZAIE_MODE_FAST3: b = 7; d = 3; break;
         case OZAIE_MODE_ACCURATE7: b = 7; d = 7; break;
+        case OZAIE_MODE_CRT:
+            if (g_plan) { plan_free(g_plan); g_plan = nullptr; }
+            return plan_new(&g_plan, M, N, K, 0, kCrtMaxMod, g_opt_rounding, true, true);
         default:
             g_last_error = "unknown mode";
             return OZAIE_ERR_ARG;
@@ -394,7 +496,8 @@ int32_t ozaie_plan_create(ozaie_plan *plan, int64_t M, int64_t N, int64_t K, int
                           int flags) {
     if (!plan) return OZAIE_ERR_ARG;
     std::lock_guard<std::mutex> lk(g_mu);
-    return plan_new(plan, M, N, K, bits, slices, (flags & 1) ? OZAIE_ROUND_NEAREST : OZAIE_ROUND_TRUNCATE, false);
+    return plan_new(plan, M, N, K, bits, slices, (flags & 1) ? OZAIE_ROUND_NEAREST : OZAIE_ROUND_TRUNCATE, false,
+                    (flags & OZAIE_PLAN_CRT) != 0);
 }
 
 int32_t ozaie_plan_destroy(ozaie_plan plan) {
@@ -464,6 +567,53 @@ int32_t ozaie_debug_split(const double *X, int64_t rows, int64_t K, int64_t sr, 
                 S_out[((int64_t)j * rows + r) * K + k] = hs[off];
             }
     return OZAIE_OK;
+}
+
+int32_t ozaie_debug_crt(const double *A, const double *B, int64_t M, int64_t N, int64_t K, int moduli, int bits,
+                        int8_t *SA_out, int8_t *SB_out, uint8_t *G_out) {
+    if (!A || !B || !SA_out || !SB_out || !G_out) return OZAIE_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc = ensure_device();
+    if (rc) return rc;
+    ozaie_plan p = nullptr;
+    rc = plan_new(&p, M, N, K, bits, moduli, OZAIE_ROUND_TRUNCATE, false, true);
+    if (rc) return rc;
+    rc = run_split(p, view_A(p, 0, A, K), view_B(p, 0, B, N), 0);
+    if (!rc) rc = run_passes(p, nullptr, N, EPI_BINS, nullptr, 0);
+    hipError_t e = rc ? hipSuccess : hipDeviceSynchronize();
+    std::vector<int8_t> ha, hb;
+    std::vector<uint32_t> hr;
+    if (!rc && e == hipSuccess) {
+        ha.resize((size_t)moduli * p->Mp * p->Kp);
+        hb.resize((size_t)moduli * p->Np * p->Kp);
+        hr.resize((size_t)moduli * (p->Mp / 4) * p->Np);
+        e = hipMemcpy(ha.data(), p->SA, ha.size(), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(hb.data(), p->SB, hb.size(), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(hr.data(), p->R, hr.size() * 4, hipMemcpyDeviceToHost);
+    }
+    if (!rc && e != hipSuccess) rc = set_hip_error(e, "ozaie_debug_crt");
+    if (!rc) {
+        // undo the slice-block layout (as ozaie_debug_split) and the four-row words of R
+        const int64_t Kb = p->Kb;
+        auto unblock = [&](const std::vector<int8_t> &h, int64_t rows, int64_t Rp, int8_t *out) {
+            for (int j = 0; j < moduli; j++)
+                for (int64_t r = 0; r < rows; r++)
+                    for (int64_t k = 0; k < K; k++) {
+                        const int64_t off = (((int64_t)j * (Rp / kBlk) + r / kBlk) * Kb + k / kBlk) * kBlkBytes +
+                                            ((k % kBlk) / 16) * 512 + (r % kBlk) * 16 + (k % 16);
+                        out[((int64_t)j * rows + r) * K + k] = h[off];
+                    }
+        };
+        unblock(ha, M, p->Mp, SA_out);
+        unblock(hb, N, p->Np, SB_out);
+        for (int j = 0; j < moduli; j++)
+            for (int64_t r = 0; r < M; r++)
+                for (int64_t c = 0; c < N; c++)
+                    G_out[((int64_t)j * M + r) * N + c] =
+                        (uint8_t)(hr[((int64_t)j * (p->Mp / 4) + r / 4) * p->Np + c] >> (8 * (r % 4)));
+    }
+    plan_free(p);
+    return rc;
 }
 
 static int debug_bins_impl(int64_t M, int64_t N, int64_t K, const double *A, const double *B, int bits,

@@ -108,6 +108,40 @@ __host__ __device__ inline void band_map(int lin, int ntm, int ntn, int &tm, int
     tn = inb / gh;
 }
 
+// CRT mode (exact integer product from residue GEMMs modulo n <= 16 coprime moduli,
+// rebuilt with the Chinese remainder theorem).  Constants are computed on the host.
+constexpr int kCrtMaxMod = 16;
+constexpr int kCrtRowPad = 256;   // the modulus GEMM's workgroup tile is 256 x 256
+struct CrtConsts {
+    int n, beta;
+    double p[kCrtMaxMod];      // modulus
+    double inv[kCrtMaxMod];    // 1 / p
+    double c31[kCrtMaxMod];    // 2^31 mod p
+    double half[kCrtMaxMod];   // largest non-negative symmetric residue (127 for 256)
+    double f[kCrtMaxMod];      // fl(W_p / M)
+    uint64_t W[kCrtMaxMod][2]; // W_p = (M/p) ((M/p)^-1 mod p) mod M, (lo, hi)
+    uint64_t Mw[2];            // M = prod p, (lo, hi)
+};
+// Modulus list, bits per operand for (n, K) (or -1), and the constants.
+int crt_modulus(int i);
+int crt_beta(int n, int64_t K);
+void crt_consts(int n, int beta, CrtConsts &c);
+// Residues S[j][rb][kb][1024] (int8, slice-block layout) of one operand.
+hipError_t launch_crt_split(const OperandView &op, const int32_t *E, const CrtConsts &c, int8_t *S, int64_t Rp,
+                            int64_t Kp, hipStream_t st);
+// g_j = (A_j B_j^T) mod p_j for every modulus: R[j][Mp/4][Np] words of four rows (uint8 each).
+struct CrtGemm {
+    const int8_t *SA, *SB;
+    int64_t Mp, Np, Kb;
+    uint32_t *R;
+    int variant;   // 0: fragments shared through LDS, 1: per-wave loads (OZAIE_OPT_KERNEL 3)
+};
+hipError_t launch_crt_gemm(const CrtGemm &g, const CrtConsts &c, hipStream_t st);
+// C = rebuild(R) * 2^(E_r + E_c - 2044 - 2 beta), truncated (rounding 0) or nearest (1).
+hipError_t launch_crt_rebuild(const uint32_t *R, int64_t Mp, int64_t Np, int64_t M, int64_t N, const int32_t *Er,
+                              const int32_t *Ec, double *C, int64_t ldc, int rounding, const CrtConsts &c,
+                              hipStream_t st);
+
 // Synthetic (U - 0.5) 2^e data for timing runs.
 hipError_t launch_fill_random(double *p, int64_t n, uint32_t seed, hipStream_t st);
 
