@@ -4,20 +4,23 @@ native FP64 DGEMM on the same GPU, at matched maximum relative error.
 
 Workload (one "step" = one C = A B over device-resident FP64 inputs):
   square DGEMM M = N = K = 4096 (--size), entries (U(0,1) - 0.5) exp(phi N(0,1)), phi = 1,
-  seeded synthetic data; b = 7 presets with 3 (fast), 7 (accurate), 8 and 9 slices.
+  seeded synthetic data; b = 7 split-scheme presets with 3 (fast), 7 (accurate), 8 and 9
+  slices, and the CRT mode with 16 moduli (exact integer product of the scaled operands).
 Errors are max / median |C - C*| / |C*| on a sampled 256 x 256 block of C against the
 correctly rounded exact product (libozaie's exact-entry meter).  `value` is the
 effective TFLOP/s (2 M N K / time) of the fastest preset whose maximum error is no
 worse than native DGEMM's on the same entries ("matched").
 
 The JSON line also carries:
-  roofline      the fused slice-GEMM kernel of the matched preset: int8 ops per launch
-                (d(d+1)/2 * 2 M N K) / its average duration (HIP events recorded by the
-                library on the stream it launches on) against the 5.0 POP/s dense int8 peak;
+  roofline      the GEMM kernel of the matched preset: int8 ops per launch (d(d+1)/2 * 2 M N K
+                for the split scheme, moduli * 2 M N K for the CRT mode) / its average
+                duration (HIP events recorded by the library on the stream it launches on)
+                against the 5.0 POP/s dense int8 peak;
                 traffic = HBM bytes per launch from the rocprofv3 FETCH/WRITE summaries in
                 profiles/ when present
-  cpu_baseline  the oracle's CPU Ozaki path (a port of the reference rule) on a row
-                sample of the same workload, on this host's cores
+  cpu_baseline  the oracle's CPU path of the matched preset (the port of the reference
+                rule, or of the CRT mode) on a row sample of the same workload, on this
+                host's cores
   target_8192   the same measurements at 8192^3
 """
 import argparse
@@ -43,6 +46,7 @@ def parse():
     ap.add_argument("--phi", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=12445)
     ap.add_argument("--slices", default="3,7,8,9", help="b=7 slice counts to measure")
+    ap.add_argument("--crt", default="16", help="CRT-mode modulus counts to measure ('' for none)")
     ap.add_argument("--block", type=int, default=256, help="side of the sampled error block")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-target", action="store_true", help="skip the 8192^3 section")
@@ -83,6 +87,16 @@ def time_steps(torch, fn, steps, warmup):
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / steps
+
+
+def crt_list(args):
+    return [int(x) for x in args.crt.split(",") if x.strip()]
+
+
+def preset_ops(p, n):
+    """int8 ops of the preset's GEMM kernel(s) for an n^3 product."""
+    k = p["moduli"] if p.get("scheme") == "crt" else p["slices"] * (p["slices"] + 1) // 2
+    return k * 2.0 * n ** 3
 
 
 def rel_errors(np, C, ref):
@@ -127,13 +141,37 @@ def measure(args, torch, np, oz, n, steps, warmup, block, slices):
         err = rel_errors(np, C[blk].cpu().numpy().ravel(), exact)
         info = plan.info()
         prods = d * (d + 1) // 2
-        rec = {"bits": 7, "slices": d, "ms": t, "tflops": flop / t / 1e9, "speedup": t_nat / t,
+        rec = {"scheme": "split", "bits": 7, "slices": d, "ms": t, "tflops": flop / t / 1e9, "speedup": t_nat / t,
                "max_rel_err": err[0], "median_rel_err": err[1], "split_ms": float(np.mean(split)),
                "gemm_ms": float(np.mean(gemm)), "gemm_int8_tops": prods * flop / float(np.mean(gemm)) / 1e9,
                "passes": info["passes"], "kernel": info["kernel"]}
         out["presets"].append(rec)
         log(args, f"  n={n}: d={d} {t:.3f} ms {rec['tflops']:.1f} TF/s x{rec['speedup']:.2f} err {err} "
                   f"gemm {rec['gemm_ms']:.3f} ms {rec['gemm_int8_tops']:.0f} TOPS")
+        plan.close()
+    for mods in crt_list(args):
+        # CRT mode: exact integer product from `mods` int8 residue GEMMs (nearest rounding)
+        plan = oz.Plan(n, n, n, 0, mods, rounding=oz.ROUND_NEAREST, crt=True)
+        t = time_steps(torch, lambda: plan(A, B, C), steps, warmup)
+        gemm, split, tot = [], [], []
+        for _ in range(max(3, steps)):
+            plan(A, B, C)
+            s_ms, g_ms, a_ms = plan.phase_times()
+            split.append(s_ms)
+            gemm.append(g_ms)
+            tot.append(a_ms)
+        err = rel_errors(np, C[blk].cpu().numpy().ravel(), exact)
+        info = plan.info()
+        g_ms = float(np.mean(gemm))
+        rec = {"scheme": "crt", "moduli": mods, "bits": info["bits"], "ms": t, "tflops": flop / t / 1e9,
+               "speedup": t_nat / t, "max_rel_err": err[0], "median_rel_err": err[1],
+               "split_ms": float(np.mean(split)), "gemm_ms": g_ms,
+               "rebuild_ms": float(np.mean(tot)) - float(np.mean(split)) - g_ms,
+               "gemm_int8_tops": mods * flop / g_ms / 1e9, "passes": 1, "kernel": info["kernel"],
+               "rounding": "nearest"}
+        out["presets"].append(rec)
+        log(args, f"  n={n}: crt {mods} moduli ({info['bits']} bits) {t:.3f} ms {rec['tflops']:.1f} TF/s "
+                  f"x{rec['speedup']:.2f} err {err} gemm {g_ms:.3f} ms {rec['gemm_int8_tops']:.0f} TOPS")
         plan.close()
     oz.set_option(oz.OPT_PHASE_TIMING, 0)
     # matched: the fastest preset whose max error is no worse than native's
@@ -159,31 +197,36 @@ def load_traffic(d, n):
         t = json.load(open(path))
         if t.get("workload_n") != n:
             return None
-        hits = [x for x in t["dispatches"] if "k_gemm_" in x["kernel"] and f"<{d}," in x["kernel"]]
+        if d == "crt":
+            hits = [x for x in t["dispatches"] if "k_crt_gemm" in x["kernel"]]
+        else:
+            hits = [x for x in t["dispatches"] if "k_gemm_" in x["kernel"] and f"<{d}," in x["kernel"]]
         return hits[-1]["hbm_bytes"] if hits else None   # the second call of the preset
     except Exception:
         return None
 
 
-def cpu_baseline(args, np, Ah, Bh, d, n):
+def cpu_baseline(args, np, Ah, Bh, d, n, crt=False):
     import oracle
+    run = (lambda A_: oracle.crt_dgemm(A_, Bh, d, 1)) if crt else (lambda A_: oracle.ozaki_dgemm(A_, Bh, 7, d))
     # bounded sample: the first rows of C (the whole of B is split, as the GPU does); a
     # 64-row probe sizes the sample to about OZAIE_CPU_SECONDS (default 10) of CPU work
     target = float(os.environ.get("OZAIE_CPU_SECONDS", "10"))
     ms = max(8, min(n, int(os.environ.get("OZAIE_CPU_ROWS", "64"))))
     t0 = time.time()
-    oracle.ozaki_dgemm(np.ascontiguousarray(Ah[:ms]), Bh, 7, d)
+    run(np.ascontiguousarray(Ah[:ms]))
     t = time.time() - t0
     if "OZAIE_CPU_ROWS" not in os.environ and t < target:
         ms = max(ms, min(n, int(ms * target / max(t, 1e-3))))
         A_s = np.ascontiguousarray(Ah[:ms])
         t0 = time.time()
-        oracle.ozaki_dgemm(A_s, Bh, 7, d)
+        run(A_s)
         t = time.time() - t0
     A_s = np.ascontiguousarray(Ah[:ms])
     rec = {"value": 2.0 * ms * n * n / t / 1e12, "unit": "TFLOP/s (effective FP64)",
            "cores": oracle.num_threads(), "kind": "port",
-           "sample": f"{ms} of {n} rows of C, b=7 d={d}, full split of B; {t:.2f} s"}
+           "sample": (f"{ms} of {n} rows of C, CRT mode with {d} moduli, full residue split of B; {t:.2f} s"
+                      if crt else f"{ms} of {n} rows of C, b=7 d={d}, full split of B; {t:.2f} s")}
     # host BLAS DGEMM on the same row sample, for reference
     t0 = time.time()
     A_s @ Bh
@@ -216,6 +259,12 @@ def profile_run(args, torch, oz, n, slices):
     C = torch.empty(n, n, dtype=torch.float64, device="cuda:0")
     for d in slices:
         p = oz.Plan(n, n, n, 7, d)
+        p(A, B, C)
+        p(A, B, C)
+        torch.cuda.synchronize()
+        p.close()
+    for mods in crt_list(args):
+        p = oz.Plan(n, n, n, 0, mods, rounding=oz.ROUND_NEAREST, crt=True)
         p(A, B, C)
         p(A, B, C)
         torch.cuda.synchronize()
@@ -257,9 +306,19 @@ def long_k(args, torch, np, oz, slices=(7, 8, 9)):
         p = oz.Plan(M, N, K, 7, d)
         t = time_steps(torch, lambda: p(A, B, C), 3, 1)
         err = rel_errors(np, C[ri, ci].cpu().numpy(), exact)
-        rec["presets"].append({"bits": 7, "slices": d, "ms": t, "tflops": flop / t / 1e9, "speedup": t_nat / t,
+        rec["presets"].append({"scheme": "split", "bits": 7, "slices": d, "ms": t, "tflops": flop / t / 1e9,
+                               "speedup": t_nat / t,
                                "max_rel_err": err[0], "median_rel_err": err[1], "passes": p.info()["passes"]})
         log(args, f"  longK d={d}: {t:.1f} ms x{t_nat / t:.2f} err {err} passes {p.info()['passes']}")
+        p.close()
+    for mods in crt_list(args):
+        p = oz.Plan(M, N, K, 0, mods, rounding=oz.ROUND_NEAREST, crt=True)
+        t = time_steps(torch, lambda: p(A, B, C), 3, 1)
+        err = rel_errors(np, C[ri, ci].cpu().numpy(), exact)
+        rec["presets"].append({"scheme": "crt", "moduli": mods, "bits": p.info()["bits"], "ms": t,
+                               "tflops": flop / t / 1e9, "speedup": t_nat / t, "max_rel_err": err[0],
+                               "median_rel_err": err[1], "passes": 1})
+        log(args, f"  longK crt {mods} moduli ({p.info()['bits']} bits): {t:.1f} ms x{t_nat / t:.2f} err {err}")
         p.close()
     log(args, f"  longK native {t_nat:.1f} ms err {nat_err}")
     del A, B, C
@@ -419,10 +478,13 @@ def main():
         return 0
     main_res, (Ah, Bh) = measure(args, torch, np, oz, n, args.steps, args.warmup, args.block, slices)
     m = main_res["matched"]
-    d = m["slices"]
-    prods = d * (d + 1) // 2
-    achieved = prods * 2.0 * n ** 3 / m["gemm_ms"] / 1e9
-    traffic = load_traffic(d, n)
+    crt = m.get("scheme") == "crt"
+    d = m["moduli"] if crt else m["slices"]
+    achieved = preset_ops(m, n) / m["gemm_ms"] / 1e9
+    traffic = load_traffic("crt" if crt else d, n)
+    kname = (f"CRT modulus GEMM (k_crt_gemm_lds<4, 4>), {d} moduli" if crt else f"fused slice GEMM, d={d}")
+    cfg = ({"scheme": "crt", "moduli": d, "bits": m["bits"], "rounding": "nearest"} if crt
+           else {"scheme": "split", "bits": 7, "slices": d})
     line = {
         "metric": "effective FP64 TFLOP/s vs native DGEMM on MI355X at matched max rel err vs MPFR",
         "value": m["tflops"],
@@ -438,7 +500,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"square DGEMM M=N=K={n}, entries (U-0.5)*exp({args.phi}*N(0,1)), "
                                f"seed {args.seed}, device-resident inputs",
-                   "bits": 7, "slices": d, "matched": m["matched"],
+                   **cfg, "matched": m["matched"],
                    "error_reference": "correctly rounded exact product on a sampled "
                                       f"{args.block}x{args.block} block"},
         "native_dgemm": main_res["native"],
@@ -446,12 +508,12 @@ def main():
         "presets": main_res["presets"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS,
                      "unit": "TFLOP/s", "ops": "int8 MAC = 2 ops", "frac": achieved / INT8_PEAK_TOPS,
-                     "traffic": traffic, "kernel": f"fused slice GEMM, d={d}",
+                     "traffic": traffic, "kernel": kname,
                      "avg_launch_ms": m["gemm_ms"]},
     }
     if not args.no_cpu:
         try:
-            line["cpu_baseline"] = cpu_baseline(args, np, Ah, Bh, d, n)
+            line["cpu_baseline"] = cpu_baseline(args, np, Ah, Bh, d, n, crt)
         except Exception as e:  # the baseline is reported, never required
             line["cpu_baseline"] = {"error": repr(e)}
     if not args.no_target and n != 8192:

@@ -110,8 +110,9 @@ int32_t ozaie_dgemm(ozaie_plan plan, int transA, int transB, const double *A, in
 
 /* Milliseconds of the last ozaie_dgemm on `plan` (needs OZAIE_OPT_PHASE_TIMING = 1 when
  * the plan was created): out[0] exponent max + split, out[1] slice GEMM passes (the
- * fused GEMM kernel, or all bin-group passes), out[2] whole call.  Synchronizes on the
- * events of that call.                                                                 */
+ * fused GEMM kernel, or all bin-group passes; in CRT mode the modulus GEMM kernel
+ * alone), out[2] whole call (CRT: including the rebuild).  Synchronizes on the events
+ * of that call.                                                                        */
 int32_t ozaie_plan_phase_times(ozaie_plan plan, float *out3);
 
 /* ---- grouped products (many independent C_i = A_i B_i of uneven shape)

@@ -138,8 +138,9 @@ struct ozaie_plan_s {
     // staging for host pointers (reference-compatible entry point)
     double *dA = nullptr, *dB = nullptr, *dC = nullptr;
     hipStream_t stream = nullptr;
-    // optional phase timing: start, after split, after GEMM passes
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    // optional phase timing: start, after split, after the GEMM kernels, end (CRT: after
+    // the rebuild, which follows the modulus GEMM; otherwise the same point as ev[2])
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool timed = false;
 };
 
@@ -227,7 +228,7 @@ int plan_configure_crt(ozaie_plan p, int64_t M, int64_t N, int64_t K, int beta, 
     crt_consts(n, beta, p->cc);
     p->Mp = round_up(M, kCrtRowPad);
     p->Np = round_up(N, kCrtRowPad);
-    p->Kp = round_up(K, 4 * kBlk);   // the modulus GEMM prefetches four blocks
+    p->Kp = round_up(K, kCrtPF * kBlk);   // whole prefetch rings of the modulus GEMM
     p->Kb = p->Kp / kBlk;
     p->kcb = p->Kb;
     p->nchunks = 1;
@@ -321,6 +322,7 @@ int run_passes(ozaie_plan p, double *C, int64_t ldc, int epi, int32_t *bins, hip
         cg.SA = p->SA; cg.SB = p->SB; cg.Mp = p->Mp; cg.Np = p->Np; cg.Kb = p->Kb; cg.R = p->R;
         cg.variant = p->family == KERNEL_DIRECT ? 1 : 0;
         HIPCHK(launch_crt_gemm(cg, p->cc, st));
+        if (p->timed) HIPCHK(hipEventRecord(p->ev[2], st));
         if (epi == EPI_C)
             HIPCHK(launch_crt_rebuild(p->R, p->Mp, p->Np, p->M, p->N, p->Er, p->Ec, C, ldc, p->rounding, p->cc, st));
         return OZAIE_OK;
@@ -355,7 +357,8 @@ int plan_execute(ozaie_plan p, int transA, int transB, const double *A, int64_t 
     int rc = run_split(p, view_A(p, transA, A, lda), view_B(p, transB, B, ldb), st);
     if (!rc && p->timed) HIPCHK(hipEventRecord(p->ev[1], st));
     if (!rc) rc = run_passes(p, C, ldc, EPI_C, nullptr, st);
-    if (!rc && p->timed) HIPCHK(hipEventRecord(p->ev[2], st));
+    if (!rc && p->timed && !p->crt) HIPCHK(hipEventRecord(p->ev[2], st));
+    if (!rc && p->timed) HIPCHK(hipEventRecord(p->ev[3], st));
     if (cur != p->device) (void)hipSetDevice(cur);
     return rc;
 }
@@ -521,10 +524,10 @@ int32_t ozaie_dgemm(ozaie_plan plan, int transA, int transB, const double *A, in
 int32_t ozaie_plan_phase_times(ozaie_plan plan, float *out3) {
     if (!plan || !out3) return OZAIE_ERR_ARG;
     if (!plan->timed) { g_last_error = "plan created without OZAIE_OPT_PHASE_TIMING"; return OZAIE_ERR_ARG; }
-    HIPCHK(hipEventSynchronize(plan->ev[2]));
+    HIPCHK(hipEventSynchronize(plan->ev[3]));
     HIPCHK(hipEventElapsedTime(&out3[0], plan->ev[0], plan->ev[1]));
     HIPCHK(hipEventElapsedTime(&out3[1], plan->ev[1], plan->ev[2]));
-    HIPCHK(hipEventElapsedTime(&out3[2], plan->ev[0], plan->ev[2]));
+    HIPCHK(hipEventElapsedTime(&out3[2], plan->ev[0], plan->ev[3]));
     return OZAIE_OK;
 }
 

@@ -112,6 +112,7 @@ __host__ __device__ inline void band_map(int lin, int ntm, int ntn, int &tm, int
 // rebuilt with the Chinese remainder theorem).  Constants are computed on the host.
 constexpr int kCrtMaxMod = 16;
 constexpr int kCrtRowPad = 256;   // the modulus GEMM's workgroup tile is 256 x 256
+constexpr int kCrtPF = 8;         // contraction blocks in flight in the modulus GEMM (K pads to 32 kCrtPF)
 struct CrtConsts {
     int n, beta;
     double p[kCrtMaxMod];      // modulus

@@ -1537,7 +1537,6 @@ hipError_t launch_crt_split(const OperandView &op, const int32_t *E, const CrtCo
 // multiple of PF; the loads past the end re-read the last block, in bounds).  Epilogue:
 // each int32 sum reduced mod p in FP64 (|sum| < 2^31 is exact) and four rows of one column
 // packed into a word of R[j][row / 4][col].
-constexpr int kCrtPF = 4;
 
 template <int WM, int WN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
@@ -1618,7 +1617,7 @@ __device__ __forceinline__ void lds_barrier() {
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-template <int WM, int WN>
+template <int WM, int WN, int PF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_crt_gemm_lds(CrtGemm g, CrtConsts c) {
     constexpr int BA = 2 * WM, BB = 2 * WN, NB = BA + BB;   // blocks per contraction block
@@ -1641,8 +1640,8 @@ void k_crt_gemm_lds(CrtGemm g, CrtConsts c) {
         const int bi = w + 4 * i;   // block bi of the contraction block: A rows, then B columns
         src[i] = (bi < BA ? A0 + (int64_t)bi * rowstride : B0 + (int64_t)(bi - BA) * rowstride) + lane * 16;
     }
-    const int64_t nk = g.Kb;   // multiple of kCrtPF
-    v4i ring[kCrtPF][PER];
+    const int64_t nk = g.Kb;   // multiple of PF
+    v4i ring[PF][PER];
     auto fetch = [&](int s, int64_t kb) {
         kb = kb < nk ? kb : nk - 1;
 #pragma unroll
@@ -1654,16 +1653,16 @@ void k_crt_gemm_lds(CrtGemm g, CrtConsts c) {
 #pragma unroll
         for (int ni = 0; ni < WN; ni++) acc[mi][ni] = v16i{};
 #pragma unroll
-    for (int s = 0; s < kCrtPF; s++) fetch(s, s);
-    for (int64_t t = 0; t < nk; t += kCrtPF) {
+    for (int s = 0; s < PF; s++) fetch(s, s);
+    for (int64_t t = 0; t < nk; t += PF) {
 #pragma unroll
-        for (int s = 0; s < kCrtPF; s++) {
+        for (int s = 0; s < PF; s++) {
             int8_t *st = smem + ((s & 1) * NB) * kBlkBytes;
 #pragma unroll
             for (int i = 0; i < PER; i++)
                 *reinterpret_cast<v4i *>(st + (w + 4 * i) * kBlkBytes + lane * 16) = ring[s][i];
             lds_barrier();
-            fetch(s, t + kCrtPF + s);
+            fetch(s, t + PF + s);
             v4i af[WM], bf[WN];
 #pragma unroll
             for (int mi = 0; mi < WM; mi++)
@@ -1711,7 +1710,8 @@ hipError_t launch_crt_gemm(const CrtGemm &g, const CrtConsts &c, hipStream_t st)
     if (g.variant == 1)
         hipLaunchKernelGGL((k_crt_gemm<WM, WN>), dim3((unsigned)tiles, (unsigned)c.n), dim3(256), 0, st, g, c);
     else
-        hipLaunchKernelGGL((k_crt_gemm_lds<WM, WN>), dim3((unsigned)tiles, (unsigned)c.n), dim3(256), 0, st, g, c);
+        hipLaunchKernelGGL((k_crt_gemm_lds<WM, WN, kCrtPF>), dim3((unsigned)tiles, (unsigned)c.n), dim3(256), 0, st,
+                           g, c);
     return hipGetLastError();
 }
 
@@ -1740,7 +1740,8 @@ __device__ __forceinline__ double crt_to_double(i128 t, int e2, int rounding) {
 }
 
 // Thread per (4 rows, 1 column): the n residue words, then per row S = sum g W (mod
-// 2^128), k = nearest(sum g f), C' = S - k M read as signed.
+// 2^128), k = nearest(sum g f), C' = S - k M read as signed.  S is summed in four 32-bit
+// limb columns (sum_j g_j W_j[i] < 16 * 255 * 2^32 fits 64 bits) with one carry pass.
 __global__ __launch_bounds__(256) void k_crt_rebuild(const uint32_t *__restrict__ R, int64_t Mp, int64_t Np,
                                                      int64_t M, int64_t N, const int32_t *__restrict__ Er,
                                                      const int32_t *__restrict__ Ec, double *__restrict__ C,
@@ -1757,14 +1758,18 @@ __global__ __launch_bounds__(256) void k_crt_rebuild(const uint32_t *__restrict_
     for (int e = 0; e < 4; e++) {
         const int64_t row = row4 * 4 + e;
         if (row >= M) break;
-        u128 S = 0;
+        uint64_t l0 = 0, l1 = 0, l2 = 0, l3 = 0;
         double tf = 0.0;
 #pragma unroll
         for (int j = 0; j < kCrtMaxMod; j++) {   // planes j >= n hold zero words
-            const uint32_t gv = (wd[j] >> (8 * e)) & 0xFFu;
-            S += (u128)gv * (((u128)c.W[j][1] << 64) | c.W[j][0]);
+            const uint64_t gv = (wd[j] >> (8 * e)) & 0xFFu;
+            l0 += gv * (uint32_t)c.W[j][0];
+            l1 += gv * (uint32_t)(c.W[j][0] >> 32);
+            l2 += gv * (uint32_t)c.W[j][1];
+            l3 += gv * (uint32_t)(c.W[j][1] >> 32);
             tf += (double)gv * c.f[j];
         }
+        u128 S = (u128)l0 + ((u128)l1 << 32) + ((u128)l2 << 64) + ((u128)l3 << 96);
         S -= (u128)(uint64_t)(int64_t)rint(tf) * Mv;
         C[row * ldc + col] = crt_to_double((i128)S, Er[row] + ec - 2044 - 2 * c.beta, rounding);
     }
