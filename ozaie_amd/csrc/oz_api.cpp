@@ -119,6 +119,9 @@ void crt_consts(int n, int beta, CrtConsts &c) {
     }
     c.Mw[0] = (uint64_t)M;
     c.Mw[1] = (uint64_t)(M >> 64);
+    const unsigned __int128 Mn = (unsigned __int128)0 - M;
+    c.Mneg[0] = (uint64_t)Mn;
+    c.Mneg[1] = (uint64_t)(Mn >> 64);
 }
 }  // namespace oz
 

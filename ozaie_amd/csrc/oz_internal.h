@@ -122,6 +122,7 @@ struct CrtConsts {
     double f[kCrtMaxMod];      // fl(W_p / M)
     uint64_t W[kCrtMaxMod][2]; // W_p = (M/p) ((M/p)^-1 mod p) mod M, (lo, hi)
     uint64_t Mw[2];            // M = prod p, (lo, hi)
+    uint64_t Mneg[2];          // 2^128 - M, (lo, hi)
 };
 // Modulus list, bits per operand for (n, K) (or -1), and the constants.
 int crt_modulus(int i);

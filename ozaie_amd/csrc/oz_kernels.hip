@@ -1739,9 +1739,11 @@ __device__ __forceinline__ double crt_to_double(i128 t, int e2, int rounding) {
     return neg ? -v : v;
 }
 
-// Thread per (4 rows, 1 column): the n residue words, then per row S = sum g W (mod
-// 2^128), k = nearest(sum g f), C' = S - k M read as signed.  S is summed in four 32-bit
-// limb columns (sum_j g_j W_j[i] < 16 * 255 * 2^32 fits 64 bits) with one carry pass.
+// Thread per (4 rows, 1 column): the n residue words, then per row k = nearest(sum g f)
+// and C' = sum g W + k (2^128 - M) (mod 2^128) read as signed, summed in four 32-bit limb
+// columns (each below 17 * 2^44, so 64-bit) with one carry pass.  Nearest rounding uses
+// the correctly rounded 128-bit integer conversion (then the same power-of-two scaling as
+// the truncating path).
 __global__ __launch_bounds__(256) void k_crt_rebuild(const uint32_t *__restrict__ R, int64_t Mp, int64_t Np,
                                                      int64_t M, int64_t N, const int32_t *__restrict__ Er,
                                                      const int32_t *__restrict__ Ec, double *__restrict__ C,
@@ -1753,13 +1755,16 @@ __global__ __launch_bounds__(256) void k_crt_rebuild(const uint32_t *__restrict_
 #pragma unroll
     for (int j = 0; j < kCrtMaxMod; j++) wd[j] = j < c.n ? R[((int64_t)j * (Mp / 4) + row4) * Np + col] : 0u;
     const int ec = Ec[col];
-    const u128 Mv = ((u128)c.Mw[1] << 64) | c.Mw[0];
 #pragma unroll
     for (int e = 0; e < 4; e++) {
         const int64_t row = row4 * 4 + e;
         if (row >= M) break;
-        uint64_t l0 = 0, l1 = 0, l2 = 0, l3 = 0;
         double tf = 0.0;
+#pragma unroll
+        for (int j = 0; j < kCrtMaxMod; j++) tf += (double)((wd[j] >> (8 * e)) & 0xFFu) * c.f[j];
+        const uint64_t k = (uint64_t)(int64_t)rint(tf);
+        uint64_t l0 = k * (uint32_t)c.Mneg[0], l1 = k * (uint32_t)(c.Mneg[0] >> 32);
+        uint64_t l2 = k * (uint32_t)c.Mneg[1], l3 = k * (uint32_t)(c.Mneg[1] >> 32);
 #pragma unroll
         for (int j = 0; j < kCrtMaxMod; j++) {   // planes j >= n hold zero words
             const uint64_t gv = (wd[j] >> (8 * e)) & 0xFFu;
@@ -1767,11 +1772,10 @@ __global__ __launch_bounds__(256) void k_crt_rebuild(const uint32_t *__restrict_
             l1 += gv * (uint32_t)(c.W[j][0] >> 32);
             l2 += gv * (uint32_t)c.W[j][1];
             l3 += gv * (uint32_t)(c.W[j][1] >> 32);
-            tf += (double)gv * c.f[j];
         }
-        u128 S = (u128)l0 + ((u128)l1 << 32) + ((u128)l2 << 64) + ((u128)l3 << 96);
-        S -= (u128)(uint64_t)(int64_t)rint(tf) * Mv;
-        C[row * ldc + col] = crt_to_double((i128)S, Er[row] + ec - 2044 - 2 * c.beta, rounding);
+        const u128 S = (u128)l0 + ((u128)l1 << 32) + ((u128)l2 << 64) + ((u128)l3 << 96);
+        const int e2 = Er[row] + ec - 2044 - 2 * c.beta;
+        C[row * ldc + col] = rounding ? ldexp((double)(i128)S, e2) : crt_to_double((i128)S, e2, 0);
     }
 }
 
