@@ -451,6 +451,12 @@ def autograd_section(args, torch, np, oz, n=2048, slices=9):
            "effective_tflops": 3 * 2.0 * n ** 3 / t_emu / 1e9,
            "grad_rel_err_vs_fp64": {"dA": rel(A.grad, dA_ref), "dB": rel(B.grad, dB_ref)}}
     log(args, f"  autograd: native {t_nat:.2f} ms emulated {t_emu:.2f} ms {rec['grad_rel_err_vs_fp64']}")
+    for mods in crt_list(args):
+        t_crt = time_steps(torch, lambda: step(lambda a, b: ozaki_matmul(a, b, 0, mods, crt=True)), 3, 2)
+        rec[f"crt{mods}"] = {"emulated_ms": t_crt, "speedup": t_nat / t_crt,
+                             "effective_tflops": 3 * 2.0 * n ** 3 / t_crt / 1e9,
+                             "grad_rel_err_vs_fp64": {"dA": rel(A.grad, dA_ref), "dB": rel(B.grad, dB_ref)}}
+        log(args, f"  autograd crt {mods}: {t_crt:.2f} ms {rec[f'crt{mods}']['grad_rel_err_vs_fp64']}")
     del A, B, W
     torch.cuda.empty_cache()
     return rec

@@ -14,8 +14,9 @@ def gen(torch, m, n, phi, g):
         phi * torch.randn(m, n, dtype=torch.float64, device="cuda", generator=g))
 
 
-@pytest.mark.parametrize("M,N,K", [(2048, 2048, 2048), (300, 170, 530)])
-def test_gradients_match_fp64_matmul(gpu, M, N, K):
+@pytest.mark.parametrize("M,N,K,crt", [(2048, 2048, 2048, False), (300, 170, 530, False),
+                                       (2048, 2048, 2048, True), (300, 170, 530, True)])
+def test_gradients_match_fp64_matmul(gpu, M, N, K, crt):
     import torch
     from ozaie_amd.autograd import ozaki_matmul
     g = torch.Generator(device="cuda")
@@ -23,7 +24,7 @@ def test_gradients_match_fp64_matmul(gpu, M, N, K):
     A = gen(torch, M, K, 0.5, g).requires_grad_(True)
     B = gen(torch, K, N, 0.5, g).requires_grad_(True)
     W = torch.randn(M, N, dtype=torch.float64, device="cuda", generator=g)
-    C = ozaki_matmul(A, B)
+    C = ozaki_matmul(A, B, bits=0, slices=16, crt=True) if crt else ozaki_matmul(A, B)
     (C * W).sum().backward()
     with torch.no_grad():
         Cref = A @ B

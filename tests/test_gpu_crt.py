@@ -52,7 +52,7 @@ def test_crt_stages_bit_exact(gpu, M, N, K, n):
 
 
 @pytest.mark.parametrize("M,N,K,n,phi", [(64, 48, 200, 16, 1.0), (300, 260, 513, 16, 2.0), (40, 520, 64, 16, 4.0),
-                                         (256, 256, 1024, 9, 1.0), (17, 19, 3000, 3, 0.5)])
+                                         (256, 256, 1024, 9, 1.0), (17, 19, 3000, 3, 0.5), (12, 9, 40000, 16, 2.0)])
 @pytest.mark.parametrize("rounding", [oz.ROUND_TRUNCATE, oz.ROUND_NEAREST])
 def test_crt_C_bit_exact(gpu, M, N, K, n, phi, rounding):
     import torch
