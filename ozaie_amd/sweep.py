@@ -18,13 +18,15 @@ import json
 import os
 import sys
 
-MODES = {"accurate": None, "fast": (7, 8), "fast3": (7, 3), "accurate7": (7, 7), "matched": (7, 9)}
+MODES = {"accurate": None, "fast": (7, 8), "fast3": (7, 3), "accurate7": (7, 7), "matched": (7, 9), "crt": "crt"}
 
 
 def mode_bd(mode, K):
     import ozaie_amd as oz
     if mode == "accurate":
         return oz.mode_slices(oz.Mode.accurate, K)
+    if mode.startswith("crt"):          # crt (16 moduli) or crtN
+        return 0, int(mode[3:] or 16)
     if mode.startswith("b") and "d" in mode:          # e.g. b7d9
         b, d = mode[1:].split("d")
         return int(b), int(d)
@@ -90,7 +92,8 @@ def accuracy(args):
                 nat_err = rel_err(np, nat, exact)
                 for m in todo:
                     b, d = mode_bd(m, K)
-                    C = oz.Plan(K, K, K, b, d)(A, B)
+                    crt = m.startswith("crt")
+                    C = oz.Plan(K, K, K, b, d, rounding=oz.ROUND_NEAREST if crt else oz.ROUND_TRUNCATE, crt=crt)(A, B)
                     torch.cuda.synchronize()
                     err = rel_err(np, C[rows, cols].cpu().numpy(), exact)
                     append(args.out, {"kind": "accuracy", "K": K, "phi": phi, "seed": seed, "mode": m,
@@ -129,7 +132,8 @@ def perf(args):
             if record_key({"kind": "perf", "size": n, "mode": m, "phi": args.phi}) in done:
                 continue
             b, d = mode_bd(m, n)
-            p = oz.Plan(n, n, n, b, d)
+            crt = m.startswith("crt")
+            p = oz.Plan(n, n, n, b, d, rounding=oz.ROUND_NEAREST if crt else oz.ROUND_TRUNCATE, crt=crt)
             t = med_ms(lambda: p(A, B, C))
             p.close()
             append(args.out, {"kind": "perf", "size": n, "phi": args.phi, "mode": m, "bits": b, "slices": d,
@@ -144,11 +148,11 @@ def main(argv=None):
     a.add_argument("--phis", default="0.1,1,2,4")
     a.add_argument("--seeds", type=int, default=3)
     a.add_argument("--ks", default="128,256,512,1024,2048")
-    a.add_argument("--modes", default="accurate,fast,fast3,accurate7,matched")
+    a.add_argument("--modes", default="accurate,fast,fast3,accurate7,matched,crt")
     a.add_argument("--samples", type=int, default=1000, help="sampled entries of C per run")
     p = sub.add_parser("perf")
     p.add_argument("--sizes", default="1024,2048,4096")
-    p.add_argument("--modes", default="fast3,accurate7,fast,matched")
+    p.add_argument("--modes", default="fast3,accurate7,fast,matched,crt")
     p.add_argument("--phi", type=float, default=1.0)
     p.add_argument("--iters", type=int, default=10)
     p.add_argument("--warmup", type=int, default=10)

@@ -150,6 +150,7 @@ def test_sweep_mode_table_and_resume_keys(tmp_path):
     assert sweep.mode_bd("accurate", 128) == (3, 29)
     assert sweep.mode_bd("fast", 4096) == (7, 8)
     assert sweep.mode_bd("b6d11", 512) == (6, 11)
+    assert sweep.mode_bd("crt", 4096) == (0, 16) and sweep.mode_bd("crt12", 4096) == (0, 12)
     out = tmp_path / "r.json"
     rec = {"kind": "accuracy", "K": 128, "phi": 1.0, "seed": 0, "mode": "fast", "max_rel_err": 1e-10}
     sweep.append(str(out), rec)
