@@ -1,0 +1,581 @@
+// oz_device.h -- device-side building blocks shared by the kernel translation units
+// (oz_kernels.hip: single products; oz_grouped.hip: grouped and persistent launches;
+// oz_crt.hip: the CRT mode): exponent-field maxima, the split of FP64 elements into int8
+// slice blocks, the epilogue arithmetic (exact 128-bit T and its conversion to FP64), the
+// XCD-aware tile map and the fused slice-GEMM tile bodies.  Everything here is inline.
+#pragma once
+#include "oz_internal.h"
+
+namespace oz {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+typedef __int128 i128;
+typedef unsigned __int128 u128;
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ __forceinline__ int field_of(double x) {
+    const uint32_t hi = (uint32_t)__double2hiint(x);
+    return (int)((hi << 1) >> 21);
+}
+
+// ------------------------------------------------------------------ max exponent bodies
+// Contraction axis contiguous (sk == 1): one wave per row, coalesced along k.
+__device__ __forceinline__ void maxexp_kc_row(const double *__restrict__ X, int64_t sr, int64_t R, int64_t K,
+                                              int32_t *__restrict__ E, int64_t r, int lane) {
+    int m = 0;
+    if (r < R) {
+        const double *row = X + r * sr;
+        int64_t k = lane;
+        for (; k + 192 < K; k += 256) {
+            const double a0 = row[k], a1 = row[k + 64], a2 = row[k + 128], a3 = row[k + 192];
+            m = max(m, max(max(field_of(a0), field_of(a1)), max(field_of(a2), field_of(a3))));
+        }
+        for (; k < K; k += 64) m = max(m, field_of(row[k]));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+    if (lane == 0) E[r] = m;
+}
+
+// Row index contiguous (sr == 1): thread per row, a slab of k per block row, combined
+// with atomicMax into E (zero-filled beforehand).
+__device__ __forceinline__ void maxexp_rc_slab(const double *__restrict__ X, int64_t sk, int64_t R, int64_t K,
+                                               int32_t *__restrict__ E, int64_t r, int64_t k0, int64_t kper) {
+    if (r >= R) return;
+    const int64_t k1 = min(K, k0 + kper);
+    int m = 0;
+    int64_t k = k0;
+    for (; k + 3 < k1; k += 4) {
+        const double a0 = X[r + k * sk], a1 = X[r + (k + 1) * sk];
+        const double a2 = X[r + (k + 2) * sk], a3 = X[r + (k + 3) * sk];
+        m = max(m, max(max(field_of(a0), field_of(a1)), max(field_of(a2), field_of(a3))));
+    }
+    for (; k < k1; k++) m = max(m, field_of(X[r + k * sk]));
+    if (m > 0) atomicMax(E + r, m);
+}
+
+// ------------------------------------------------------------------ split
+// One block handles 32 rows x 128 contraction indices (four slice blocks).  The FP64
+// tile is staged through LDS so that the global reads are coalesced along whichever
+// axis is contiguous; each thread then owns 16 consecutive k of one row and writes one
+// 16-byte lane chunk per slice plane (a wave writes one whole 1 KiB block per plane).
+constexpr int kSplitLds = 128 * 33 * 8;  // bytes of the FP64 staging tile (either orientation)
+
+// Digits of 16 elements (mantissas mx with the implicit bit, bias = row exponent minus
+// element exponent, sign bits negm) into one 16-byte chunk per slice plane at dst.
+// With b*d <= 64 the whole digit string Q = floor(M_x 2^(bd - 53 - bias)) fits 64 bits and
+// digit j is the b-bit field of Q at bit b(d-1-j): one bit-field extract per digit.  Four
+// digits share a word; the negative elements' bytes are negated without carries between
+// bytes (every digit is below 128, so 0x80 - x fits a byte: -x = (0x80 - x) ^ 0x80).
+// Wider strings shift the mantissa per digit.
+__device__ __forceinline__ void emit_digits16(uint64_t (&mx)[16], const int (&bias)[16], uint32_t negm, int b,
+                                              int d, int8_t *__restrict__ dst, int64_t plane) {
+    uint32_t nbm[4];
+#pragma unroll
+    for (int gi = 0; gi < 4; gi++) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int h = 0; h < 4; h++) m |= ((negm >> (4 * gi + h)) & 1u) ? (0xFFu << (8 * h)) : 0u;
+        nbm[gi] = m;
+    }
+    if (b * d <= 64) {
+        uint32_t qlo[16], qhi[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int sh = 53 + bias[i] - b * d;   // Q = mx >> sh (mx << -sh below zero)
+            const uint64_t q = sh >= 64 ? 0ull : (sh >= 0 ? mx[i] >> sh : mx[i] << (-sh));
+            qlo[i] = (uint32_t)q;
+            qhi[i] = (uint32_t)(q >> 32);
+        }
+        const uint32_t mask = (1u << b) - 1u;
+        // one plane: x(i) is element i's digit; negative elements' bytes negated
+        auto plane_out = [&](int j, auto x) {
+            uint32_t w[4];
+#pragma unroll
+            for (int gi = 0; gi < 4; gi++) {
+                const uint32_t p = x(4 * gi) | (x(4 * gi + 1) << 8) | (x(4 * gi + 2) << 16) | (x(4 * gi + 3) << 24);
+                const uint32_t pn = (0x80808080u - p) ^ 0x80808080u;
+                w[gi] = (pn & nbm[gi]) | (p & ~nbm[gi]);
+            }
+            *reinterpret_cast<uint4 *>(dst + (int64_t)j * plane) = make_uint4(w[0], w[1], w[2], w[3]);
+        };
+        int j = 0;
+        for (; j < d && b * (d - 1 - j) >= 32; j++) {   // field inside the high word
+            const uint32_t off = (uint32_t)(b * (d - 1 - j) - 32);
+            plane_out(j, [&](int i) { return __builtin_amdgcn_ubfe(qhi[i], off, (uint32_t)b); });
+        }
+        for (; j < d; j++) {                              // field at bit < 32 (may straddle)
+            const uint32_t sj = (uint32_t)(b * (d - 1 - j));
+            plane_out(j, [&](int i) { return __builtin_amdgcn_alignbit(qhi[i], qlo[i], sj) & mask; });
+        }
+        return;
+    }
+    const uint64_t mask = (1ull << b) - 1ull;
+    for (int j = 0; j < d; j++) {
+        uint32_t w[4];
+#pragma unroll
+        for (int gi = 0; gi < 4; gi++) {
+            uint32_t acc = 0;
+#pragma unroll
+            for (int h = 0; h < 4; h++) {
+                const int i = 4 * gi + h;
+                const int sh = 53 + bias[i] - b * (j + 1);
+                uint64_t v;
+                if (sh >= 64) v = 0;
+                else if (sh >= 0) v = mx[i] >> sh;
+                else v = (sh <= -64) ? 0ull : (mx[i] << (-sh));
+                int dig = (int)(v & mask);
+                dig = ((negm >> i) & 1u) ? -dig : dig;
+                acc |= (uint32_t)(dig & 0xFF) << (8 * h);
+            }
+            w[gi] = acc;
+        }
+        *reinterpret_cast<uint4 *>(dst + (int64_t)j * plane) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
+// The three steps of a tile of 32 rows x TK contraction indices (TK = 128 or 256),
+// separable so that a caller can overlap the global reads of the next tile with the digit
+// extraction of the current one.
+template <bool KC, int TK = 128>
+__host__ __device__ constexpr int split_lds_bytes() { return KC ? 32 * (TK + 1) * 8 : TK * 33 * 8; }
+
+template <bool KC, int TK = 128>
+__device__ __forceinline__ void split_gather(const double *__restrict__ X, int64_t sr, int64_t sk, int64_t R,
+                                             int64_t K, int64_t rb, int64_t kt, double (&v)[TK / 8]) {
+    const int64_t r0 = rb * 32, k0 = kt * TK;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < TK / 8; i++) {
+        const int c = t + 256 * i;
+        int row, kk;
+        if (KC) { row = c / TK; kk = c % TK; }
+        else { kk = c >> 5; row = c & 31; }
+        const int64_t r = r0 + row, k = k0 + kk;
+        v[i] = (r < R && k < K) ? (KC ? X[r * sr + k] : X[k * sk + r]) : 0.0;
+    }
+}
+
+template <bool KC, int TK = 128>
+__device__ __forceinline__ void split_stage(const double (&v)[TK / 8], double *tile) {
+    constexpr int LDW = KC ? (TK + 1) : (32 + 1);
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < TK / 8; i++) {
+        const int c = t + 256 * i;
+        if (KC) tile[(c / TK) * LDW + (c % TK)] = v[i];
+        else tile[(c >> 5) * LDW + (c & 31)] = v[i];
+    }
+}
+
+// Digits of the staged tile: thread (row, q) owns 16 consecutive k of one row and writes
+// one 16-byte lane chunk per slice plane (q = t / 32 + 8 p for the TK / 128 passes p).
+template <bool KC, int TK = 128>
+__device__ __forceinline__ void split_emit(const double *tile, int64_t R, const int32_t *__restrict__ Erow, int b,
+                                           int d, int8_t *__restrict__ S, int64_t Rb, int64_t Kb, int64_t rb,
+                                           int64_t kt) {
+    constexpr int LDW = KC ? (TK + 1) : (32 + 1);
+    const int t = threadIdx.x;
+    const int64_t r0 = rb * 32;
+    const int row = t & 31;
+    const int emax = (r0 + row < R) ? Erow[row] : 0;
+    const int64_t plane = Rb * Kb * (int64_t)kBlkBytes;
+#pragma unroll
+    for (int pass = 0; pass < TK / 128; pass++) {
+        const int q = (t >> 5) + 8 * pass;   // k in [16q, 16q + 16) of the tile
+        const int64_t kb = kt * (TK / 32) + (q >> 1);
+        if (kb >= Kb) break;  // no barrier below
+        uint64_t mx[16];
+        int bias[16];
+        uint32_t negm = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const double v = KC ? tile[row * LDW + 16 * q + i] : tile[(16 * q + i) * LDW + row];
+            const uint64_t u = (uint64_t)__double_as_longlong(v);
+            const int e = (int)((u >> 52) & 0x7FF);
+            mx[i] = (u & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
+            bias[i] = emax - e;
+            negm |= (uint32_t)(u >> 63) << i;
+        }
+        int8_t *dst = S + (rb * Kb + kb) * (int64_t)kBlkBytes + (q & 1) * 512 + row * 16;
+        emit_digits16(mx, bias, negm, b, d, dst, plane);
+    }
+}
+
+template <bool KC>
+__device__ __forceinline__ void split_tile(const double *__restrict__ X, int64_t sr, int64_t sk, int64_t R,
+                                           int64_t K, const int32_t *__restrict__ Erow, int b, int d,
+                                           int8_t *__restrict__ S, int64_t Rb, int64_t Kb, int64_t rb,
+                                           int64_t kt, double *tile) {
+    double v[16];
+    split_gather<KC>(X, sr, sk, R, K, rb, kt, v);
+    split_stage<KC>(v, tile);
+    __syncthreads();
+    split_emit<KC>(tile, R, Erow, b, d, S, Rb, Kb, rb, kt);
+}
+
+// ------------------------------------------------------------------ epilogue math
+__device__ __forceinline__ int clz128(u128 a) {
+    const uint64_t h = (uint64_t)(a >> 64), l = (uint64_t)a;
+    if (h) return __builtin_clzll(h);
+    if (l) return 64 + __builtin_clzll(l);
+    return 128;
+}
+
+// Exact integer T (scaled by 2^(es - 2044 - b(d+1))) to FP64.
+// rounding 0: sign * trunc53(|T|) via the bit arithmetic of include/oz_aie.h:124-146
+// (no over/underflow handling, as in the reference); rounding 1: nearest-even.
+__device__ __forceinline__ double t_to_double(i128 t, int es, int b, int d, int rounding) {
+    if (t == 0) return 0.0;
+    const uint64_t sgn = t < 0 ? (1ull << 63) : 0ull;
+    const u128 a = t < 0 ? (u128)(-t) : (u128)t;
+    const int nb = 128 - clz128(a);
+    if (rounding == 0) {
+        const int shift = nb - 53;
+        uint64_t lu = shift > 0 ? (uint64_t)(a >> shift) : (uint64_t)(a << (-shift));
+        const int64_t ex = nb - b * (d + 1) - 1;
+        lu -= 0x0010000000000000ull;
+        lu += sgn + ((uint64_t)(ex + 1023) << 52) + ((uint64_t)(int64_t)(es - 2044) << 52);
+        return __longlong_as_double((long long)lu);
+    }
+    uint64_t m;
+    int e2;
+    if (nb <= 53) {
+        m = (uint64_t)a;
+        e2 = 0;
+    } else {
+        const int sh = nb - 53;
+        u128 q = a >> sh;
+        const u128 rem = a - (q << sh);
+        const u128 half = (u128)1 << (sh - 1);
+        if (rem > half || (rem == half && (q & 1))) q += 1;
+        m = (uint64_t)q;
+        e2 = sh;
+    }
+    const double v = ldexp((double)m, e2 + es - 2044 - b * (d + 1));
+    return sgn ? -v : v;
+}
+
+// Store one element: bins k0 .. k0+NB-1 of this pass (values vals[kk]); es is the sum of
+// the element's row and column exponent fields.
+template <int NB>
+__device__ __forceinline__ void epilogue_elem(const GemmLaunch &g, int epi, int row, int col,
+                                              const int32_t (&vals)[NB], int k0, int es) {
+    if (row >= g.M || col >= g.N) return;
+    if (epi == EPI_BINS) {
+#pragma unroll
+        for (int kk = 0; kk < NB; kk++)
+            if (k0 + kk < g.d) g.bins[((int64_t)(k0 + kk) * g.M + row) * g.N + col] = vals[kk];
+        return;
+    }
+    i128 T = 0;
+#pragma unroll
+    for (int kk = 0; kk < NB; kk++) {
+        const int k = k0 + kk;
+        if (k < g.d) T += (i128)vals[kk] << (g.b * (g.d - 1 - k));
+    }
+    i128 *tp = g.T + (int64_t)row * g.ldT + col;
+    if (!g.first) T += *tp;
+    if (g.last)
+        g.C[(int64_t)row * g.ldc + col] = t_to_double(T, es, g.b, g.d, g.rounding);
+    else
+        *tp = T;
+}
+
+// Exact T = sum_k v_k 2^(B(D-1-k)) for a compile-time digit width B with all D bins of one
+// pass: the bins are summed in 64-bit chunks of 31/B + 1 consecutive bins (relative
+// shifts up to 31 bits keep a chunk of int32 terms inside int64), and the chunks are
+// combined in 128-bit arithmetic, so every shift is a constant.
+template <int D, int B>
+__device__ __forceinline__ i128 bins_to_T(const int32_t (&v)[D]) {
+    constexpr int CH = 31 / B + 1;
+    i128 T = 0;
+#pragma unroll
+    for (int top = D - 1; top >= 0; top -= CH) {
+        int64_t c = 0;
+#pragma unroll
+        for (int i = 0; i < CH; i++)
+            if (top - i >= 0) c += (int64_t)v[top - i] << (B * i);
+        T += (i128)c << (B * (D - 1 - top));
+    }
+    return T;
+}
+
+// The same store for EPI_C when the pass holds every bin (k0 = 0, NB = d) and b = B.
+template <int D, int B>
+__device__ __forceinline__ void epilogue_elem_c(const GemmLaunch &g, int row, int col, const int32_t (&vals)[D],
+                                                int es) {
+    if (row >= g.M || col >= g.N) return;
+    i128 T = bins_to_T<D, B>(vals);
+    i128 *tp = g.T + (int64_t)row * g.ldT + col;
+    if (!g.first) T += *tp;
+    if (g.last)
+        g.C[(int64_t)row * g.ldc + col] = t_to_double(T, es, B, D, g.rounding);
+    else
+        *tp = T;
+}
+
+// Block -> tile mapping.  Workgroups are dispatched round-robin over the 8 XCDs; give
+// each XCD a contiguous range of tiles, and walk that range in bands of 8 tile-rows so
+// the concurrently running tiles of one XCD share A rows and B columns in its L2.
+__device__ __forceinline__ int xcd_remap(int bid, int total) {
+    const int q = total / 8, rem = total % 8;
+    const int xcd = bid % 8, idx = bid / 8;
+    return (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + idx;
+}
+
+__device__ __forceinline__ void tile_map(int bid, int ntm, int ntn, int &tm, int &tn) {
+    band_map(xcd_remap(bid, ntm * ntn), ntm, ntn, tm, tn);
+}
+
+template <int N>
+__device__ __forceinline__ void s_waitcnt_vm() {
+    // gfx9 encoding: vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt_hi[15:14]
+    constexpr int imm = (N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8);
+    __builtin_amdgcn_s_waitcnt(imm);
+}
+
+// ------------------------------------------------------------------ fused slice GEMM
+// Workgroup: 4 waves as 2 x 2, wave tile (32 WM) x (32 WN), workgroup tile
+// (64 WM) x (64 WN).  Per contraction block (32) every slice block of the tile is
+// copied HBM -> LDS with global_load_lds_dwordx4 (one 1 KiB block per wave
+// instruction) into one of two stages.  One barrier per block: after it, each wave
+// reads its d A-fragments and d*WN B-fragments of the current stage, then issues the
+// copies of the next block into the other stage (free: every wave has passed the
+// barrier, so all reads of that stage from the previous block have completed), then
+// runs the d(d+1)/2 * WM * WN MFMAs of the triangle into its d bin accumulators while
+// the copies are in flight.
+template <int D, int WM, int WN>
+struct FusedCfg {
+    static constexpr int BA = 2 * WM, BB = 2 * WN;
+    static constexpr int NBLK = D * (BA + BB);
+    static constexpr int STAGE = NBLK * kBlkBytes;
+    static constexpr int NLD = (NBLK + 3) / 4;  // load slots per wave per stage
+    static constexpr int NMIN = NBLK / 4;       // loads every wave issues per stage
+};
+
+// Epilogue of one wave: bins leave the accumulators four elements at a time through a
+// per-wave LDS scratch (each lane reads back only its own values) and each element's bins
+// become T and FP64 C.  The scratch of a wave holds NB * 256 bin words followed by the
+// exponent fields of the wave's 64 rows and 64 columns (er_v / ec_v: this lane's row
+// rbase + lane and column cbase + lane, loaded by the caller before its main loop).
+template <int NB, int WM, int WN>
+__host__ __device__ constexpr int wave_scratch_words() { return NB * 256 + 128; }
+
+template <int NB, int WM, int WN, int B>
+__device__ __forceinline__ void wave_epilogue_t(const GemmLaunch &g, int epi, int k0, v16i (&acc)[NB][WM][WN],
+                                                int32_t *scratch, int lane, int rbase, int cbase) {
+    const int32_t *ex = scratch + NB * 256;
+#pragma unroll
+    for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+        for (int ni = 0; ni < WN; ni++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+#pragma unroll
+                for (int k = 0; k < NB; k++)
+#pragma unroll
+                    for (int e = 0; e < 4; e++) scratch[(k * 4 + e) * 64 + lane] = acc[k][mi][ni][4 * q + e];
+                __builtin_amdgcn_sched_barrier(0);
+                const int lcol = ni * 32 + (lane & 31);
+                const int ecol = ex[64 + lcol];
+                if constexpr (B > 0) {
+                    // four independent elements per step
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const int lrow = mi * 32 + q * 8 + (lane >> 5) * 4 + e;
+                        int32_t vals[NB];
+#pragma unroll
+                        for (int k = 0; k < NB; k++) vals[k] = scratch[(k * 4 + e) * 64 + lane];
+                        epilogue_elem_c<NB, B>(g, rbase + lrow, cbase + lcol, vals, ex[lrow] + ecol);
+                    }
+                } else {
+#pragma unroll 1
+                    for (int e = 0; e < 4; e++) {
+                        const int lrow = mi * 32 + q * 8 + (lane >> 5) * 4 + e;
+                        int32_t vals[NB];
+#pragma unroll
+                        for (int k = 0; k < NB; k++) vals[k] = scratch[(k * 4 + e) * 64 + lane];
+                        epilogue_elem<NB>(g, epi, rbase + lrow, cbase + lcol, vals, k0, ex[lrow] + ecol);
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+}
+
+template <int NB, int WM, int WN>
+__device__ __forceinline__ void wave_epilogue(const GemmLaunch &g, int epi, int k0, v16i (&acc)[NB][WM][WN],
+                                              int32_t *scratch, int lane, int rbase, int cbase, int er_v,
+                                              int ec_v) {
+    scratch[NB * 256 + lane] = er_v;
+    scratch[NB * 256 + 64 + lane] = ec_v;
+    if (epi == EPI_C && k0 == 0 && NB == g.d && g.b == 7)
+        wave_epilogue_t<NB, WM, WN, 7>(g, epi, k0, acc, scratch, lane, rbase, cbase);
+    else
+        wave_epilogue_t<NB, WM, WN, 0>(g, epi, k0, acc, scratch, lane, rbase, cbase);
+}
+
+// Exponent fields this lane hands to the epilogue: row rbase + lane of A (lane < rows)
+// and column cbase + lane of B (lane < cols).
+__device__ __forceinline__ void load_tile_exps(const GemmLaunch &g, int epi, int lane, int rbase, int rows,
+                                               int cbase, int cols, int &er_v, int &ec_v) {
+    er_v = 0;
+    ec_v = 0;
+    if (epi != EPI_C) return;
+    if (lane < rows && rbase + lane < g.M) er_v = g.Er[rbase + lane];
+    if (lane < cols && cbase + lane < g.N) ec_v = g.Ec[cbase + lane];
+}
+
+template <int D, int WM, int WN>
+__host__ __device__ constexpr int lds_tile_bytes() { return 2 * FusedCfg<D, WM, WN>::STAGE; }
+
+template <int D, int WM, int WN>
+__device__ __forceinline__ void gemm_lds_tile(const GemmLaunch &g, int tm, int tn, int epi, int8_t *smem) {
+    using F = FusedCfg<D, WM, WN>;
+    constexpr int BA = F::BA, BB = F::BB, NBLK = F::NBLK, NLD = F::NLD, NMIN = F::NMIN;
+
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = w & 1, wn = w >> 1;
+    const int64_t rbA0 = (int64_t)tm * BA, rbB0 = (int64_t)tn * BB;
+    const int64_t planeA = g.RbA * g.Kb * kBlkBytes, planeB = g.RbB * g.Kb * kBlkBytes;
+    const int rbase = tm * 64 * WM + wm * 32 * WM, cbase = tn * 64 * WN + wn * 32 * WN;
+    int er_v, ec_v;
+    load_tile_exps(g, epi, lane, rbase, 32 * WM, cbase, 32 * WN, er_v, ec_v);
+
+    const int8_t *src[NLD];
+    int ldsoff[NLD];
+#pragma unroll
+    for (int it = 0; it < NLD; it++) {
+        const int blk = w + 4 * it;
+        const int bc = blk < NBLK ? blk : NBLK - 1;
+        if (bc < D * BA) {
+            const int j = bc / BA, mb = bc % BA;
+            src[it] = g.SA + j * planeA + (rbA0 + mb) * g.Kb * kBlkBytes;
+        } else {
+            const int bb = bc - D * BA;
+            const int j = bb / BB, nb = bb % BB;
+            src[it] = g.SB + j * planeB + (rbB0 + nb) * g.Kb * kBlkBytes;
+        }
+        src[it] += lane * 16;
+        ldsoff[it] = bc * kBlkBytes;
+    }
+    auto issue = [&](int stage, int64_t kb) {
+#pragma unroll
+        for (int it = 0; it < NLD; it++) {
+            if (it < NMIN || w + 4 * it < NBLK)
+                __builtin_amdgcn_global_load_lds((const void *)(src[it] + kb * kBlkBytes),
+                                                 (lds_void *)(smem + stage * F::STAGE + ldsoff[it]), 16, 0, 0);
+        }
+    };
+
+    v16i acc[D][WM][WN];
+#pragma unroll
+    for (int k = 0; k < D; k++)
+#pragma unroll
+        for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++) acc[k][mi][ni] = v16i{};
+
+    const int64_t kb0 = g.kb0, kb1 = g.kb1;
+    if (kb0 < kb1) issue(0, kb0);
+    for (int64_t kb = kb0; kb < kb1; kb++) {
+        const int s = (int)((kb - kb0) & 1);
+        s_waitcnt_vm<0>();
+        __syncthreads();
+        const int8_t *st = smem + s * F::STAGE;
+        v4i af[D][WM], bf[D][WN];
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+#pragma unroll
+            for (int mi = 0; mi < WM; mi++)
+                af[j][mi] = *reinterpret_cast<const v4i *>(st + (j * BA + wm * WM + mi) * kBlkBytes + lane * 16);
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++)
+                bf[j][ni] = *reinterpret_cast<const v4i *>(st + (D * BA + j * BB + wn * WN + ni) * kBlkBytes + lane * 16);
+        }
+        if (kb + 1 < kb1) issue(s ^ 1, kb + 1);
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; i + j < D; j++)
+#pragma unroll
+                for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+                    for (int ni = 0; ni < WN; ni++)
+                        acc[i + j][mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i][mi], bf[j][ni],
+                                                                                   acc[i + j][mi][ni], 0, 0, 0);
+    }
+    __syncthreads();
+
+    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * wave_scratch_words<D, WM, WN>();
+    wave_epilogue<D, WM, WN>(g, epi, 0, acc, scratch, lane, rbase, cbase, er_v, ec_v);
+}
+
+// Direct variant: no LDS in the main loop.  Each wave streams its own A and B
+// fragments HBM/L2 -> VGPR (one 1 KiB block per wave instruction, 16 bytes per lane),
+// double-buffered in registers, so there is no barrier and the compiler's per-register
+// wait counts overlap the loads of block t+1 with the MFMAs of block t.  Requires an
+// even number of contraction blocks per pass (the planner pads K to 64).
+template <int D>
+__host__ __device__ constexpr int direct_tile_bytes() { return 4 * 4 * (D * 256 + 128); }
+
+template <int D, int WM, int WN>
+__device__ __forceinline__ void gemm_direct_tile(const GemmLaunch &g, int tm, int tn, int epi, int8_t *smem) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = w & 1, wn = w >> 1;
+    const int64_t planeA = g.RbA * g.Kb * kBlkBytes, planeB = g.RbB * g.Kb * kBlkBytes;
+    const int64_t rbA = (int64_t)tm * 2 * WM + wm * WM, rbB = (int64_t)tn * 2 * WN + wn * WN;
+    const int rbase = tm * 64 * WM + wm * 32 * WM, cbase = tn * 64 * WN + wn * 32 * WN;
+    int er_v, ec_v;
+    load_tile_exps(g, epi, lane, rbase, 32 * WM, cbase, 32 * WN, er_v, ec_v);
+    const int8_t *pa = g.SA + rbA * g.Kb * kBlkBytes + lane * 16;
+    const int8_t *pb = g.SB + rbB * g.Kb * kBlkBytes + lane * 16;
+    const int64_t rowstride = g.Kb * kBlkBytes;
+
+    v4i a0[D][WM], b0[D][WN], a1[D][WM], b1[D][WN];
+    auto load = [&](v4i (&af)[D][WM], v4i (&bf)[D][WN], int64_t kb) {
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+#pragma unroll
+            for (int mi = 0; mi < WM; mi++)
+                af[j][mi] = *reinterpret_cast<const v4i *>(pa + j * planeA + mi * rowstride + kb * kBlkBytes);
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++)
+                bf[j][ni] = *reinterpret_cast<const v4i *>(pb + j * planeB + ni * rowstride + kb * kBlkBytes);
+        }
+    };
+    v16i acc[D][WM][WN];
+#pragma unroll
+    for (int k = 0; k < D; k++)
+#pragma unroll
+        for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++) acc[k][mi][ni] = v16i{};
+    auto mma = [&](const v4i (&af)[D][WM], const v4i (&bf)[D][WN]) {
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; i + j < D; j++)
+#pragma unroll
+                for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+                    for (int ni = 0; ni < WN; ni++)
+                        acc[i + j][mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i][mi], bf[j][ni],
+                                                                                   acc[i + j][mi][ni], 0, 0, 0);
+    };
+    const int64_t kb0 = g.kb0, nk = g.kb1 - g.kb0;   // nk is even
+    if (nk > 0) load(a0, b0, kb0);
+    for (int64_t t = 0; t < nk; t += 2) {
+        load(a1, b1, kb0 + t + 1);
+        mma(a0, b0);
+        if (t + 2 < nk) load(a0, b0, kb0 + t + 2);
+        mma(a1, b1);
+    }
+    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * wave_scratch_words<D, WM, WN>();
+    wave_epilogue<D, WM, WN>(g, epi, 0, acc, scratch, lane, rbase, cbase, er_v, ec_v);
+}
+
+}  // namespace oz
