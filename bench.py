@@ -420,6 +420,28 @@ def grouped(args, torch, np, oz, slices=(3, 7, 8, 9), count=512):
                       "speedup_vs_native_graph": entry[best]["speedup_vs_native_graph"]})
         rec["presets"].append(entry)
         oz.set_tune_table(d, [-1] * 9)
+    for mods in crt_list(args):
+        # CRT mode: per-phase launches, tile shape per size class from the CRT autotuner
+        t0 = time.time()
+        table = [int(x) for x in oz.autotune(0, mods, reps=2, crt=True)]
+        t_tune = time.time() - t0
+        grp = oz.Group(shapes, 0, mods, crt=True)
+        t = time_steps(torch, lambda: grp(As, Bs, Cs), 5, 3)
+        q = grp.query()
+        err = errs()
+        entry = {"scheme": "crt", "moduli": mods, "bits": "per problem: crt_beta(moduli, K)",
+                 "autotune_table": table, "autotune_s": t_tune, "best_mode": "multi",
+                 "multi": {"ms": t, "tflops": flop / t / 1e9, "speedup": t_nat / t,
+                           "speedup_vs_native_graph": (t_nat_graph / t) if t_nat_graph else None,
+                           "launches": q["launches"], "graph": q["graph"], "workgroups": q["workgroups"]},
+                 "ms": t, "tflops": flop / t / 1e9, "speedup": t_nat / t,
+                 "speedup_vs_native_graph": (t_nat_graph / t) if t_nat_graph else None,
+                 "max_rel_err": err[0], "median_rel_err": err[1]}
+        log(args, f"  grouped crt {mods}: {t:.2f} ms x{t_nat / t:.2f} err {err} launches {q['launches']}"
+                  f" table {table}")
+        grp.close()
+        rec["presets"].append(entry)
+        oz.set_tune_table(mods, [-1] * 9, crt=True)
     log(args, f"  grouped native {t_nat:.2f} ms (graph {t_nat_graph}) err {nat_err}")
     del As, Bs, Cs
     torch.cuda.empty_cache()
@@ -474,7 +496,7 @@ def main():
     import ozaie_amd as oz
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    slices = [int(x) for x in args.slices.split(",")]
+    slices = [int(x) for x in args.slices.split(",") if x.strip()]
     n = args.size
     if args.profile_run:
         profile_run(args, torch, oz, n, slices)

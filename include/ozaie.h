@@ -56,6 +56,8 @@ extern "C" {
 #define OZAIE_OPT_KERNEL 4      /* 0 auto, 1 bin-group kernel, 2 fused with LDS copies,
                                    3 fused with direct register loads, 4 fused pipelined  */
 #define OZAIE_OPT_PHASE_TIMING 5 /* 1: plans record HIP events around the split and GEMM    */
+#define OZAIE_OPT_CRT_GROUP_MODULI 6 /* moduli per workgroup of the grouped CRT modulus GEMM
+                                   (0 = automatic per tile shape); read at group creation */
 #define OZAIE_ROUND_TRUNCATE 0
 #define OZAIE_ROUND_NEAREST 1
 
@@ -129,10 +131,17 @@ int32_t ozaie_plan_phase_times(ozaie_plan plan, float *out3);
  * then every GEMM tile, balanced per XCD on the host) with one tile shape for the group.
  * flags bits 4..7: persistent tile configuration + 1 (0: per-d default; configuration
  * ids as for ozaie_autotune, direct-load tiles only).
+ * flags bit 3 (OZAIE_GROUP_CRT): the CRT mode for every problem (no reference
+ * counterpart): `slices` = moduli (2..16), `bits` = 0 for the largest exact bit count of
+ * each problem's K (crt_beta(moduli, K_i)) or a fixed count no larger than any problem's;
+ * K_i <= 65536.  Four phase launches plus one modulus-GEMM launch per tile shape (ids as
+ * for ozaie_autotune_crt; flags bits 4..7 = forced shape + 1), same graph capture; each
+ * C_i is bit-identical to a CRT plan of that problem.  Not combinable with bit 2.
  * Runs of one group must not overlap (they share the workspace and the control block). */
 #define OZAIE_GROUP_NEAREST 1
 #define OZAIE_GROUP_NO_GRAPH 2
 #define OZAIE_GROUP_PERSISTENT 4
+#define OZAIE_GROUP_CRT 8
 typedef struct ozaie_group_s *ozaie_group;
 int32_t ozaie_group_create(ozaie_group *group, int count, const int64_t *M, const int64_t *N,
                            const int64_t *K, int bits, int slices, int flags);
@@ -140,7 +149,7 @@ int32_t ozaie_group_create(ozaie_group *group, int count, const int64_t *M, cons
 int32_t ozaie_group_run(ozaie_group group, const double *const *A, const double *const *B,
                         double *const *C, void *stream);
 /* out[0] kernel launches per run, out[1] graph captured, out[2] per-problem fallback,
- * out[3] fused-GEMM workgroups per run.                                               */
+ * out[3] GEMM workgroups per run (fused-GEMM tiles; CRT: modulus-GEMM workgroups).   */
 int32_t ozaie_group_query(ozaie_group group, int32_t *out4);
 /* Tile configuration chosen for each problem (-1 in the per-problem fallback).        */
 int32_t ozaie_group_configs(ozaie_group group, int32_t *cfg_out);
@@ -163,6 +172,11 @@ int32_t ozaie_dgemm_grouped(int count, const int64_t *M, const int64_t *N, const
 int32_t ozaie_autotune(int bits, int slices, int reps, int32_t *table9);
 int32_t ozaie_tune_get(int slices, int32_t *table9);   /* -1 entries: untuned */
 int32_t ozaie_tune_set(int slices, const int32_t *table9);
+/* The same for CRT groups with `moduli` moduli (no reference counterpart); tile shape
+ * ids 0: 256x256, 1: 128x128, 2: 64x64 (modulus GEMM through LDS).                     */
+int32_t ozaie_autotune_crt(int moduli, int reps, int32_t *table9);
+int32_t ozaie_tune_get_crt(int moduli, int32_t *table9);
+int32_t ozaie_tune_set_crt(int moduli, const int32_t *table9);
 
 /* ---- inspection (reference full_dump equivalent, whole_array_iron_ozaki.py:485-558)  */
 

@@ -37,6 +37,7 @@ OPT_STRICT = 2
 OPT_KCHUNK = 3
 OPT_KERNEL = 4
 OPT_PHASE_TIMING = 5
+OPT_CRT_GROUP_MODULI = 6
 ROUND_TRUNCATE = 0
 ROUND_NEAREST = 1
 
@@ -58,7 +59,7 @@ EXPORTS = [
     "ozaie_paired_output", "ozaie_last_error", "ozaie_version", "ozaie_plan_phase_times",
     "ozaie_exact_entries", "ozaie_group_create", "ozaie_group_run", "ozaie_group_query",
     "ozaie_group_configs", "ozaie_group_status", "ozaie_group_destroy", "ozaie_dgemm_grouped", "ozaie_autotune",
-    "ozaie_tune_get", "ozaie_tune_set",
+    "ozaie_tune_get", "ozaie_tune_set", "ozaie_autotune_crt", "ozaie_tune_get_crt", "ozaie_tune_set_crt",
 ]
 
 
@@ -150,6 +151,12 @@ def _declare(L):
     L.ozaie_tune_get.restype = i32
     L.ozaie_tune_set.argtypes = [c_int, P]
     L.ozaie_tune_set.restype = i32
+    L.ozaie_autotune_crt.argtypes = [c_int, c_int, P]
+    L.ozaie_autotune_crt.restype = i32
+    L.ozaie_tune_get_crt.argtypes = [c_int, P]
+    L.ozaie_tune_get_crt.restype = i32
+    L.ozaie_tune_set_crt.argtypes = [c_int, P]
+    L.ozaie_tune_set_crt.restype = i32
     L.ozaie_last_error.argtypes = []
     L.ozaie_last_error.restype = ctypes.c_char_p
     L.ozaie_version.argtypes = []
@@ -377,22 +384,29 @@ def exact_entries(A, B, rows, cols):
 class Group:
     """Many independent products C_i = A_i @ B_i (float64 device tensors, row-major) with
     b bits and d slices each, replayed through one captured hipGraph.  persistent=True
-    runs the whole group as one persistent kernel launch (slice counts 3, 7, 8, 9)."""
+    runs the whole group as one persistent kernel launch (slice counts 3, 7, 8, 9).
+    crt=True runs every problem in the CRT mode with `slices` moduli and `bits` = 0 (the
+    largest exact bit count per problem) or a fixed bit count; crt_cfg forces a tile shape
+    (0: 256x256, 1: 128x128, 2: 64x64)."""
 
     GRAPH = 0
     NO_GRAPH = 2
     PERSISTENT = 4
+    CRT = 8
 
     def __init__(self, shapes, bits=7, slices=8, rounding=ROUND_TRUNCATE, graph=True, persistent=False,
-                 persistent_cfg=None):
+                 persistent_cfg=None, crt=False, crt_cfg=None):
         self.shapes = [(int(m), int(n), int(k)) for m, n, k in shapes]
         cnt = len(self.shapes)
         M = np.array([s[0] for s in self.shapes], dtype=np.int64)
         N = np.array([s[1] for s in self.shapes], dtype=np.int64)
         K = np.array([s[2] for s in self.shapes], dtype=np.int64)
-        flags = (1 if rounding == ROUND_NEAREST else 0) | (0 if graph else 2) | (4 if persistent else 0)
+        flags = ((1 if rounding == ROUND_NEAREST else 0) | (0 if graph else 2) | (4 if persistent else 0)
+                 | (8 if crt else 0))
         if persistent_cfg is not None:
             flags |= (int(persistent_cfg) + 1) << 4
+        if crt_cfg is not None:
+            flags |= (int(crt_cfg) + 1) << 4
         h = ctypes.c_void_p()
         _check(lib().ozaie_group_create(ctypes.byref(h), cnt, _ptr(M), _ptr(N), _ptr(K), int(bits),
                                         int(slices), flags), "ozaie_group_create")
@@ -445,22 +459,32 @@ class Group:
             pass
 
 
-def autotune(bits=7, slices=8, reps=3):
-    """Time the grouped tile configurations per size class; returns the 9-entry table."""
+def autotune(bits=7, slices=8, reps=3, crt=False):
+    """Time the grouped tile configurations per size class; returns the 9-entry table.
+    crt=True tunes the CRT groups with `slices` moduli (bits is ignored)."""
     out = np.zeros(9, dtype=np.int32)
-    _check(lib().ozaie_autotune(int(bits), int(slices), int(reps), _ptr(out)), "ozaie_autotune")
+    if crt:
+        _check(lib().ozaie_autotune_crt(int(slices), int(reps), _ptr(out)), "ozaie_autotune_crt")
+    else:
+        _check(lib().ozaie_autotune(int(bits), int(slices), int(reps), _ptr(out)), "ozaie_autotune")
     return out
 
 
-def tune_table(slices):
+def tune_table(slices, crt=False):
     out = np.zeros(9, dtype=np.int32)
-    _check(lib().ozaie_tune_get(int(slices), _ptr(out)), "ozaie_tune_get")
+    if crt:
+        _check(lib().ozaie_tune_get_crt(int(slices), _ptr(out)), "ozaie_tune_get_crt")
+    else:
+        _check(lib().ozaie_tune_get(int(slices), _ptr(out)), "ozaie_tune_get")
     return out
 
 
-def set_tune_table(slices, table):
+def set_tune_table(slices, table, crt=False):
     t = np.ascontiguousarray(table, dtype=np.int32)
-    _check(lib().ozaie_tune_set(int(slices), _ptr(t)), "ozaie_tune_set")
+    if crt:
+        _check(lib().ozaie_tune_set_crt(int(slices), _ptr(t)), "ozaie_tune_set_crt")
+    else:
+        _check(lib().ozaie_tune_set(int(slices), _ptr(t)), "ozaie_tune_set")
 
 
 def version():

@@ -31,6 +31,7 @@ int g_opt_strict = 0;
 int g_opt_kchunk = 0;
 int g_opt_kernel = 0;
 int g_opt_timing = 0;
+int g_opt_crt_mpw = 0;
 
 int set_hip_error(hipError_t e, const char *where) {
     g_last_error = std::string(where) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
@@ -461,6 +462,9 @@ int32_t ozaie_set_option(int option, int value) {
             if (value < 0 || value > 4) return OZAIE_ERR_ARG;
             g_opt_kernel = value; return OZAIE_OK;
         case OZAIE_OPT_PHASE_TIMING: g_opt_timing = value ? 1 : 0; return OZAIE_OK;
+        case OZAIE_OPT_CRT_GROUP_MODULI:
+            if (value < 0 || value > kCrtMaxMod) return OZAIE_ERR_ARG;
+            g_opt_crt_mpw = value; return OZAIE_OK;
         default: return OZAIE_ERR_ARG;
     }
 }
@@ -473,6 +477,7 @@ int32_t ozaie_get_option(int option) {
         case OZAIE_OPT_KCHUNK: return g_opt_kchunk;
         case OZAIE_OPT_KERNEL: return g_opt_kernel;
         case OZAIE_OPT_PHASE_TIMING: return g_opt_timing;
+        case OZAIE_OPT_CRT_GROUP_MODULI: return g_opt_crt_mpw;
         default: return -OZAIE_ERR_ARG;
     }
 }
@@ -615,8 +620,9 @@ int32_t ozaie_debug_crt(const double *A, const double *B, int64_t M, int64_t N, 
         for (int j = 0; j < moduli; j++)
             for (int64_t r = 0; r < M; r++)
                 for (int64_t c = 0; c < N; c++)
-                    G_out[((int64_t)j * M + r) * N + c] =
-                        (uint8_t)(hr[((int64_t)j * (p->Mp / 4) + r / 4) * p->Np + c] >> (8 * (r % 4)));
+                    G_out[((int64_t)j * M + r) * N + c] = (uint8_t)(
+                        ((int)(int8_t)(uint8_t)(hr[((int64_t)j * (p->Mp / 4) + r / 4) * p->Np + c] >> (8 * (r % 4))) +
+                         crt_modulus(j)) % crt_modulus(j));   // stored signed, reported in [0, p)
     }
     plan_free(p);
     return rc;

@@ -18,8 +18,8 @@ namespace oz {
 // (plane 0) the residue is the low byte of q in two's complement.
 template <bool KC>
 __device__ __forceinline__ void crt_emit(const double *tile, int64_t R, const int32_t *__restrict__ Erow,
-                                         const CrtConsts &c, int8_t *__restrict__ S, int64_t Rb, int64_t Kb,
-                                         int64_t rb, int64_t kt) {
+                                         const CrtConsts &c, int beta, int8_t *__restrict__ S, int64_t Rb,
+                                         int64_t Kb, int64_t rb, int64_t kt) {
     constexpr int TK = 128;
     constexpr int LDW = KC ? (TK + 1) : (32 + 1);
     const int t = threadIdx.x;
@@ -36,7 +36,7 @@ __device__ __forceinline__ void crt_emit(const double *tile, int64_t R, const in
         const uint64_t u = (uint64_t)__double_as_longlong(v);
         const int e = (int)((u >> 52) & 0x7FF);
         const uint64_t mx = (u & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
-        const int sh = 53 + emax - (e ? e : 1) - c.beta;   // subnormals: exponent 1 - 1023
+        const int sh = 53 + emax - (e ? e : 1) - beta;   // subnormals: exponent 1 - 1023
         const uint64_t Q = sh >= 64 ? 0ull : (sh >= 0 ? mx >> sh : mx << (-sh));
         const bool neg = (u >> 63) != 0;
         const double h = (double)(uint32_t)(Q >> 31), l = (double)(uint32_t)(Q & 0x7FFFFFFFu);
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void k_crt_split(const double *__restrict__ X,
     split_gather<KC>(X, sr, sk, R, K, blockIdx.y, blockIdx.x, v);
     split_stage<KC>(v, tile);
     __syncthreads();
-    crt_emit<KC>(tile, R, E + (int64_t)blockIdx.y * 32, c, S, Rb, Kb, blockIdx.y, blockIdx.x);
+    crt_emit<KC>(tile, R, E + (int64_t)blockIdx.y * 32, c, c.beta, S, Rb, Kb, blockIdx.y, blockIdx.x);
 }
 
 hipError_t launch_crt_split(const OperandView &op, const int32_t *E, const CrtConsts &c, int8_t *S, int64_t Rp,
@@ -93,12 +93,135 @@ hipError_t launch_crt_split(const OperandView &op, const int32_t *E, const CrtCo
     return hipGetLastError();
 }
 
+// Shared-fragment variant: the workgroup's 2 WM A and 2 WN B blocks of each contraction
+// block are loaded once per workgroup -- every thread brings PER x 16 bytes of whole
+// 1 KiB blocks into a register ring PF blocks deep -- and written to one of two LDS
+// stages; after a barrier each wave reads its WM + WN fragments from LDS.  Halves the
+// L2 -> CU traffic of the per-wave loads.  The barrier is a plain s_barrier after the
+// wave's own LDS writes have completed (no memory fence), so the global loads of the
+// ring stay in flight across it; the inline-asm memory clobber keeps the compiler from
+// moving LDS accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Contraction loop of one (64 WM) x (64 WN) tile.  setup: A0 / B0 = the tile's first row
+// block of one modulus plane of A / B (rows of rowstride bytes), nk contraction blocks;
+// prime: the first PF blocks into the ring; run: the products, leaving the ring consumed
+// (its last fetches re-read block nk - 1, in bounds); advance: move to another plane, for
+// the next prime.  TAIL: nk need not be a multiple of PF (the steps past nk are skipped;
+// the condition is uniform over the workgroup, so every wave meets the same barriers).
+template <int WM, int WN, int PF, bool TAIL>
+struct CrtTileLoop {
+    static constexpr int BA = 2 * WM, BB = 2 * WN, NB = BA + BB;   // blocks per contraction block
+    static constexpr int PER = NB / 4;                            // blocks each wave brings in
+    static_assert(NB % 4 == 0, "blocks must split over four waves");
+    const int8_t *src[PER];
+    v4i ring[PF][PER];
+    int64_t nk;
+    int lane, w, wm, wn;
+
+    __device__ __forceinline__ void setup(const int8_t *A0, const int8_t *B0, int64_t rowstride, int64_t nblk) {
+        nk = nblk;
+        lane = threadIdx.x & 63;
+        w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        wm = w & 1;
+        wn = w >> 1;
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int bi = w + 4 * i;   // block bi of the contraction block: A rows, then B columns
+            src[i] = (bi < BA ? A0 + (int64_t)bi * rowstride : B0 + (int64_t)(bi - BA) * rowstride) + lane * 16;
+        }
+    }
+    __device__ __forceinline__ void fetch(int s, int64_t kb) {
+        kb = kb < nk ? kb : nk - 1;
+#pragma unroll
+        for (int i = 0; i < PER; i++) ring[s][i] = *reinterpret_cast<const v4i *>(src[i] + kb * kBlkBytes);
+    }
+    __device__ __forceinline__ void prime() {
+#pragma unroll
+        for (int s = 0; s < PF; s++) fetch(s, s);
+    }
+    __device__ __forceinline__ void advance(int64_t offA, int64_t offB) {
+#pragma unroll
+        for (int i = 0; i < PER; i++) src[i] += (w + 4 * i < BA) ? offA : offB;
+    }
+    __device__ __forceinline__ void step(int s, int64_t t, int8_t *smem, v16i (&acc)[WM][WN]) {
+        int8_t *st = smem + ((s & 1) * NB) * kBlkBytes;
+#pragma unroll
+        for (int i = 0; i < PER; i++)
+            *reinterpret_cast<v4i *>(st + (w + 4 * i) * kBlkBytes + lane * 16) = ring[s][i];
+        lds_barrier();
+        fetch(s, t + PF + s);
+        v4i af[WM], bf[WN];
+#pragma unroll
+        for (int mi = 0; mi < WM; mi++)
+            af[mi] = *reinterpret_cast<const v4i *>(st + (wm * WM + mi) * kBlkBytes + lane * 16);
+#pragma unroll
+        for (int ni = 0; ni < WN; ni++)
+            bf[ni] = *reinterpret_cast<const v4i *>(st + (BA + wn * WN + ni) * kBlkBytes + lane * 16);
+#pragma unroll
+        for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++)
+                acc[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[mi], bf[ni], acc[mi][ni], 0, 0, 0);
+    }
+    __device__ __forceinline__ void run(int8_t *smem, v16i (&acc)[WM][WN]) {
+#pragma unroll
+        for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++) acc[mi][ni] = v16i{};
+        const int64_t nfull = TAIL ? nk - nk % PF : nk;
+        int64_t t = 0;
+        for (; t < nfull; t += PF) {
+#pragma unroll
+            for (int s = 0; s < PF; s++) step(s, t, smem, acc);
+        }
+        if (TAIL) {
+            const int64_t rest = nk - nfull;   // the ring holds blocks nfull .. nfull + PF - 1
+#pragma unroll
+            for (int s = 0; s < PF - 1; s++)
+                if (s < rest) step(s, t, smem, acc);
+        }
+    }
+};
+
+// Residues of the wave's (32 WM) x (32 WN) sums, four rows of one column per word of
+// Rj[row / 4][col] (row stride ldr words), one signed byte each: r = x - p rint(x / p),
+// congruent to x, |r| <= p / 2 (|x| <= 2^30, so x / p in FP64 is never within 1 / (2p) of
+// a half-integer for odd p, and is exact for p = 256).  CLIP: words whose four rows all
+// lie at or beyond M, or whose column lies at or beyond N, are not written.
+template <int WM, int WN, bool CLIP>
+__device__ __forceinline__ void crt_tile_store(const v16i (&acc)[WM][WN], double p, double inv, uint32_t *Rj,
+                                               int64_t ldr, int64_t tm, int64_t tn, int64_t M, int64_t N) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = w & 1, wn = w >> 1;
+    const int64_t rbase = tm * 64 * WM + wm * 32 * WM, cbase = tn * 64 * WN + wn * 32 * WN;
+#pragma unroll
+    for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+        for (int ni = 0; ni < WN; ni++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                uint32_t word = 0;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const double x = (double)acc[mi][ni][4 * q + e];
+                    const double r = __builtin_fma(-rint(x * inv), p, x);
+                    word |= ((uint32_t)(int)r & 0xFFu) << (8 * e);
+                }
+                const int64_t row4 = (rbase + mi * 32 + q * 8 + (lane >> 5) * 4) >> 2;
+                const int64_t col = cbase + ni * 32 + (lane & 31);
+                if (!CLIP || (row4 * 4 < M && col < N)) Rj[row4 * ldr + col] = word;
+            }
+}
+
 // One exact int8 GEMM per modulus (blockIdx.y), wave tile (32 WM) x (32 WN).  A contraction
 // block gives a wave only WM*WN MFMAs, too few to cover the L2 latency of its fragments,
 // so the fragments of the next PF blocks are in flight in a register ring (Kb is a
 // multiple of PF; the loads past the end re-read the last block, in bounds).  Epilogue:
-// each int32 sum reduced mod p in FP64 (|sum| < 2^31 is exact) and four rows of one column
-// packed into a word of R[j][row / 4][col].
+// crt_tile_store.
 
 template <int WM, int WN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
@@ -144,122 +267,28 @@ void k_crt_gemm(CrtGemm g, CrtConsts c) {
             load(s, t + kCrtPF + s);
         }
     }
-    const double p = c.p[j], inv = c.inv[j];
-    const int64_t rbase = (int64_t)tm * 64 * WM + wm * 32 * WM, cbase = (int64_t)tn * 64 * WN + wn * 32 * WN;
-    uint32_t *Rj = g.R + (int64_t)j * (g.Mp / 4) * g.Np;
-#pragma unroll
-    for (int mi = 0; mi < WM; mi++)
-#pragma unroll
-        for (int ni = 0; ni < WN; ni++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                uint32_t word = 0;
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const double x = (double)acc[mi][ni][4 * q + e];
-                    double r = __builtin_fma(-rint(x * inv), p, x);   // in [-p/2, p/2]
-                    r = r < 0.0 ? r + p : r;                           // in [0, p)
-                    word |= (uint32_t)r << (8 * e);
-                }
-                const int64_t row4 = (rbase + mi * 32 + q * 8 + (lane >> 5) * 4) >> 2;
-                const int64_t col = cbase + ni * 32 + (lane & 31);
-                Rj[row4 * g.Np + col] = word;
-            }
-}
-
-// Shared-fragment variant: the workgroup's 8 A and 8 B blocks of each contraction block
-// (16 KiB) are loaded once per workgroup -- every thread brings 4 x 16 bytes of whole
-// 1 KiB blocks into a register ring kCrtPF blocks deep -- and written to one of two LDS
-// stages; after a barrier each wave reads its 4 + 4 fragments from LDS.  Halves the
-// L2 -> CU traffic of the per-wave loads.  The barrier is a plain s_barrier after the
-// wave's own LDS writes have completed (no memory fence), so the global loads of the
-// ring stay in flight across it; the inline-asm memory clobber keeps the compiler from
-// moving LDS accesses across it.
-__device__ __forceinline__ void lds_barrier() {
-    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    crt_tile_store<WM, WN, false>(acc, c.p[j], c.inv[j], g.R + (int64_t)j * (g.Mp / 4) * g.Np, g.Np, tm, tn, g.Mp,
+                                  g.Np);
 }
 
 template <int WM, int WN, int PF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_crt_gemm_lds(CrtGemm g, CrtConsts c) {
-    constexpr int BA = 2 * WM, BB = 2 * WN, NB = BA + BB;   // blocks per contraction block
-    constexpr int PER = NB / 4;                            // blocks each wave brings in
-    static_assert(NB % 4 == 0, "blocks must split over four waves");
-    __shared__ __attribute__((aligned(1024))) int8_t smem[2 * NB * kBlkBytes];
+    __shared__ __attribute__((aligned(1024))) int8_t smem[2 * (2 * WM + 2 * WN) * kBlkBytes];
     const int j = blockIdx.y;
     const int ntm = (int)(g.Mp / (64 * WM)), ntn = (int)(g.Np / (64 * WN));
     int tm, tn;
     tile_map(blockIdx.x, ntm, ntn, tm, tn);
-    const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wm = w & 1, wn = w >> 1;
     const int64_t rowstride = g.Kb * kBlkBytes;
-    const int8_t *A0 = g.SA + (int64_t)j * (g.Mp / kBlk) * rowstride + (int64_t)tm * BA * rowstride;
-    const int8_t *B0 = g.SB + (int64_t)j * (g.Np / kBlk) * rowstride + (int64_t)tn * BB * rowstride;
-    const int8_t *src[PER];
-#pragma unroll
-    for (int i = 0; i < PER; i++) {
-        const int bi = w + 4 * i;   // block bi of the contraction block: A rows, then B columns
-        src[i] = (bi < BA ? A0 + (int64_t)bi * rowstride : B0 + (int64_t)(bi - BA) * rowstride) + lane * 16;
-    }
-    const int64_t nk = g.Kb;   // multiple of PF
-    v4i ring[PF][PER];
-    auto fetch = [&](int s, int64_t kb) {
-        kb = kb < nk ? kb : nk - 1;
-#pragma unroll
-        for (int i = 0; i < PER; i++) ring[s][i] = *reinterpret_cast<const v4i *>(src[i] + kb * kBlkBytes);
-    };
+    const int8_t *A0 = g.SA + (int64_t)j * (g.Mp / kBlk) * rowstride + (int64_t)tm * 2 * WM * rowstride;
+    const int8_t *B0 = g.SB + (int64_t)j * (g.Np / kBlk) * rowstride + (int64_t)tn * 2 * WN * rowstride;
     v16i acc[WM][WN];
-#pragma unroll
-    for (int mi = 0; mi < WM; mi++)
-#pragma unroll
-        for (int ni = 0; ni < WN; ni++) acc[mi][ni] = v16i{};
-#pragma unroll
-    for (int s = 0; s < PF; s++) fetch(s, s);
-    for (int64_t t = 0; t < nk; t += PF) {
-#pragma unroll
-        for (int s = 0; s < PF; s++) {
-            int8_t *st = smem + ((s & 1) * NB) * kBlkBytes;
-#pragma unroll
-            for (int i = 0; i < PER; i++)
-                *reinterpret_cast<v4i *>(st + (w + 4 * i) * kBlkBytes + lane * 16) = ring[s][i];
-            lds_barrier();
-            fetch(s, t + PF + s);
-            v4i af[WM], bf[WN];
-#pragma unroll
-            for (int mi = 0; mi < WM; mi++)
-                af[mi] = *reinterpret_cast<const v4i *>(st + (wm * WM + mi) * kBlkBytes + lane * 16);
-#pragma unroll
-            for (int ni = 0; ni < WN; ni++)
-                bf[ni] = *reinterpret_cast<const v4i *>(st + (BA + wn * WN + ni) * kBlkBytes + lane * 16);
-#pragma unroll
-            for (int mi = 0; mi < WM; mi++)
-#pragma unroll
-                for (int ni = 0; ni < WN; ni++)
-                    acc[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[mi], bf[ni], acc[mi][ni], 0, 0, 0);
-        }
-    }
-    const double p = c.p[j], inv = c.inv[j];
-    const int64_t rbase = (int64_t)tm * 64 * WM + wm * 32 * WM, cbase = (int64_t)tn * 64 * WN + wn * 32 * WN;
-    uint32_t *Rj = g.R + (int64_t)j * (g.Mp / 4) * g.Np;
-#pragma unroll
-    for (int mi = 0; mi < WM; mi++)
-#pragma unroll
-        for (int ni = 0; ni < WN; ni++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                uint32_t word = 0;
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const double x = (double)acc[mi][ni][4 * q + e];
-                    double r = __builtin_fma(-rint(x * inv), p, x);
-                    r = r < 0.0 ? r + p : r;
-                    word |= (uint32_t)r << (8 * e);
-                }
-                const int64_t row4 = (rbase + mi * 32 + q * 8 + (lane >> 5) * 4) >> 2;
-                const int64_t col = cbase + ni * 32 + (lane & 31);
-                Rj[row4 * g.Np + col] = word;
-            }
+    CrtTileLoop<WM, WN, PF, false> L;
+    L.setup(A0, B0, rowstride, g.Kb);
+    L.prime();
+    L.run(smem, acc);
+    crt_tile_store<WM, WN, false>(acc, c.p[j], c.inv[j], g.R + (int64_t)j * (g.Mp / 4) * g.Np, g.Np, tm, tn,
+                                  g.Mp, g.Np);
 }
 
 hipError_t launch_crt_gemm(const CrtGemm &g, const CrtConsts &c, hipStream_t st) {
@@ -301,26 +330,21 @@ __device__ __forceinline__ double crt_to_double(i128 t, int e2, int rounding) {
     return neg ? -v : v;
 }
 
-// Thread per (4 rows, 1 column): the n residue words, then per row k = nearest(sum g f)
-// and C' = sum g W + k (2^128 - M) (mod 2^128) read as signed, summed in four 32-bit limb
-// columns (each below 17 * 2^44, so 64-bit) with one carry pass.  Nearest rounding uses
-// the correctly rounded 128-bit integer conversion (then the same power-of-two scaling as
-// the truncating path).
-__global__ __launch_bounds__(256) void k_crt_rebuild(const uint32_t *__restrict__ R, int64_t Mp, int64_t Np,
-                                                     int64_t M, int64_t N, const int32_t *__restrict__ Er,
-                                                     const int32_t *__restrict__ Ec, double *__restrict__ C,
-                                                     int64_t ldc, int rounding, int64_t ncb, CrtConsts c) {
-    const int64_t row4 = blockIdx.x / ncb;
-    const int64_t col = (blockIdx.x % ncb) * 256 + threadIdx.x;
-    if (col >= N) return;
-    uint32_t wd[kCrtMaxMod];
-#pragma unroll
-    for (int j = 0; j < kCrtMaxMod; j++) wd[j] = j < c.n ? R[((int64_t)j * (Mp / 4) + row4) * Np + col] : 0u;
-    const int ec = Ec[col];
+// The four rows 4 row4 + e (e < rows) of one column from the n residue words wd[j], each
+// byte already flipped by 0x80: the stored signed residue r (congruent to the residue
+// product g, -128 <= r <= 127) read as the unsigned byte u = r + 128.  Since the weights
+// satisfy sum_j W_j = 1 (mod M), sum u W = C' + 128 (mod M); with t = sum u f and
+// k = nearest(t) (|C' + 128| / M is below 1/4 + 2^-100 and t is far closer than that to
+// sum u W / M), C' = sum u W - k M - 128 (mod 2^128), read as signed.  The sums run in four
+// 32-bit limb columns (each below 2^45, so 64-bit) with one carry pass.  Nearest
+// rounding uses the correctly rounded 128-bit integer conversion (then the same
+// power-of-two scaling as the truncating path).
+__device__ __forceinline__ void crt_rebuild_quad(const uint32_t (&wd)[kCrtMaxMod], const CrtConsts &c, int rows,
+                                                 const int32_t *__restrict__ Er4, int ec, int beta, int rounding,
+                                                 double *__restrict__ Cq, int64_t ldc) {
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-        const int64_t row = row4 * 4 + e;
-        if (row >= M) break;
+        if (e >= rows) break;
         double tf = 0.0;
 #pragma unroll
         for (int j = 0; j < kCrtMaxMod; j++) tf += (double)((wd[j] >> (8 * e)) & 0xFFu) * c.f[j];
@@ -335,10 +359,27 @@ __global__ __launch_bounds__(256) void k_crt_rebuild(const uint32_t *__restrict_
             l2 += gv * (uint32_t)c.W[j][1];
             l3 += gv * (uint32_t)(c.W[j][1] >> 32);
         }
-        const u128 S = (u128)l0 + ((u128)l1 << 32) + ((u128)l2 << 64) + ((u128)l3 << 96);
-        const int e2 = Er[row] + ec - 2044 - 2 * c.beta;
-        C[row * ldc + col] = rounding ? ldexp((double)(i128)S, e2) : crt_to_double((i128)S, e2, 0);
+        const u128 S = (u128)l0 + ((u128)l1 << 32) + ((u128)l2 << 64) + ((u128)l3 << 96) - (u128)128;
+        const int e2 = Er4[e] + ec - 2044 - 2 * beta;
+        Cq[e * ldc] = rounding ? ldexp((double)(i128)S, e2) : crt_to_double((i128)S, e2, 0);
     }
+}
+
+// Thread per (4 rows, 1 column) of R[j][Mp / 4][Np].
+__global__ __launch_bounds__(256) void k_crt_rebuild(const uint32_t *__restrict__ R, int64_t Mp, int64_t Np,
+                                                     int64_t M, int64_t N, const int32_t *__restrict__ Er,
+                                                     const int32_t *__restrict__ Ec, double *__restrict__ C,
+                                                     int64_t ldc, int rounding, int64_t ncb, CrtConsts c) {
+    const int64_t row4 = blockIdx.x / ncb;
+    const int64_t col = (blockIdx.x % ncb) * 256 + threadIdx.x;
+    if (col >= N) return;
+    uint32_t wd[kCrtMaxMod];
+#pragma unroll
+    for (int j = 0; j < kCrtMaxMod; j++)
+        wd[j] = j < c.n ? R[((int64_t)j * (Mp / 4) + row4) * Np + col] ^ 0x80808080u : 0u;
+    const int64_t rest = M - row4 * 4;
+    crt_rebuild_quad(wd, c, rest < 4 ? (int)rest : 4, Er + row4 * 4, Ec[col], c.beta, rounding,
+                     C + row4 * 4 * ldc + col, ldc);
 }
 
 hipError_t launch_crt_rebuild(const uint32_t *R, int64_t Mp, int64_t Np, int64_t M, int64_t N, const int32_t *Er,
@@ -350,6 +391,123 @@ hipError_t launch_crt_rebuild(const uint32_t *R, int64_t Mp, int64_t Np, int64_t
     if (blocks > 0x7fffffff) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_crt_rebuild, dim3((unsigned)blocks), dim3(256), 0, st, R, Mp, Np, M, N, Er, Ec, C, ldc,
                        rounding, ncb, c);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ grouped CRT
+// Many independent products in one launch per phase: exponent maxima (the grouped
+// split-scheme kernel), residues of every operand with its problem's bit count, one
+// modulus GEMM launch per tile shape over items {problem, tile row, tile col} x modulus,
+// and the rebuild over items {problem, first element}.
+
+__global__ __launch_bounds__(256) void k_grp_crt_split(const GroupOperand *__restrict__ ops,
+                                                       const CrtGroupProb *__restrict__ probs,
+                                                       const int4 *__restrict__ items, CrtConsts c) {
+    __shared__ double tile[kSplitLds / 8];
+    const int4 it = items[blockIdx.x];
+    const GroupOperand &o = ops[it.x];
+    const int beta = probs[it.x >> 1].beta;
+    double v[16];
+    if (o.sk == 1) {
+        split_gather<true>(o.X, o.sr, o.sk, o.R, o.K, it.y, it.z, v);
+        split_stage<true>(v, tile);
+        __syncthreads();
+        crt_emit<true>(tile, o.R, o.E + (int64_t)it.y * 32, c, beta, o.S, o.Rb, o.Kb, it.y, it.z);
+    } else {
+        split_gather<false>(o.X, o.sr, o.sk, o.R, o.K, it.y, it.z, v);
+        split_stage<false>(v, tile);
+        __syncthreads();
+        crt_emit<false>(tile, o.R, o.E + (int64_t)it.y * 32, c, beta, o.S, o.Rb, o.Kb, it.y, it.z);
+    }
+}
+
+hipError_t launch_group_crt_split(const GroupOperand *ops, const CrtGroupProb *probs, const int4 *items, int n,
+                                  const CrtConsts &c, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_grp_crt_split, dim3((unsigned)n), dim3(256), 0, st, ops, probs, items, c);
+    return hipGetLastError();
+}
+
+// Waves per SIMD the tile shape leaves room for: the 256 x 256 tile needs every register of
+// one wave (16 accumulator tiles + the ring), the smaller ones fit two and four.
+constexpr int crt_occupancy(int tiles) { return tiles >= 8 ? 1 : (tiles >= 2 ? 2 : 4); }
+
+// One workgroup per tile item and range of moduli [j0, j1) (grid y): the contraction loop
+// of each modulus in turn, the ring primed with the next modulus's first blocks before the
+// epilogue of the current one, so those loads are in flight during the residue stores.
+// Items {-1, ...} pad the host's per-XCD lists (the hardware deals blockIdx x to XCD x % 8)
+// and return at once.
+template <int WM, int WN>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(crt_occupancy(WM * WN), crt_occupancy(WM * WN))))
+void k_grp_crt_gemm(const CrtGroupProb *__restrict__ probs, const int4 *__restrict__ items, int mpw, CrtConsts c) {
+    __shared__ __attribute__((aligned(1024))) int8_t smem[2 * (2 * WM + 2 * WN) * kBlkBytes];
+    const int4 it = items[blockIdx.x];
+    if (it.x < 0) return;
+    const int j0 = blockIdx.y * mpw;
+    const int j1 = j0 + mpw < c.n ? j0 + mpw : c.n;
+    const CrtGroupProb &pr = probs[it.x];
+    const int64_t rowstride = pr.Kb * kBlkBytes;
+    const int64_t planeA = pr.RbA * rowstride, planeB = pr.RbB * rowstride;
+    const int64_t planeR = pr.RbA * (kBlk / 4) * pr.Np;
+    CrtTileLoop<WM, WN, kCrtPF, true> L;
+    L.setup(pr.SA + (int64_t)j0 * planeA + (int64_t)it.y * 2 * WM * rowstride,
+            pr.SB + (int64_t)j0 * planeB + (int64_t)it.z * 2 * WN * rowstride, rowstride, pr.Kb);
+    L.prime();
+    for (int j = j0; j < j1; j++) {
+        if (j > j0) lds_barrier();   // every wave is done reading the previous modulus's stages
+        v16i acc[WM][WN];
+        L.run(smem, acc);
+        if (j + 1 < j1) {
+            L.advance(planeA, planeB);
+            L.prime();
+        }
+        crt_tile_store<WM, WN, true>(acc, c.p[j], c.inv[j], pr.R + (int64_t)j * planeR, pr.Np, it.y, it.z, pr.M,
+                                     pr.N);
+    }
+}
+
+int crt_group_cfg_wm(int cfg) { return cfg == CCFG_44 ? 4 : (cfg == CCFG_22 ? 2 : 1); }
+int crt_group_cfg_wn(int cfg) { return crt_group_cfg_wm(cfg); }
+
+hipError_t launch_group_crt_gemm(const CrtGroupProb *probs, const int4 *items, int n, int cfg, int mpw,
+                                 const CrtConsts &c, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (c.n < 1 || c.n > kCrtMaxMod || mpw < 1) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)n, (unsigned)((c.n + mpw - 1) / mpw));
+    switch (cfg) {
+        case CCFG_44: hipLaunchKernelGGL((k_grp_crt_gemm<4, 4>), grid, dim3(256), 0, st, probs, items, mpw, c); break;
+        case CCFG_22: hipLaunchKernelGGL((k_grp_crt_gemm<2, 2>), grid, dim3(256), 0, st, probs, items, mpw, c); break;
+        case CCFG_11: hipLaunchKernelGGL((k_grp_crt_gemm<1, 1>), grid, dim3(256), 0, st, probs, items, mpw, c); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Items {problem, first element}: thread t handles element base + t of the problem's
+// ceil(M / 4) x N (row quad, column) grid.
+__global__ __launch_bounds__(256) void k_grp_crt_rebuild(const CrtGroupProb *__restrict__ probs,
+                                                         const int4 *__restrict__ items, int rounding, CrtConsts c) {
+    const int4 it = items[blockIdx.x];
+    const CrtGroupProb &pr = probs[it.x];
+    const uint32_t N = (uint32_t)pr.N;
+    const uint32_t idx = (uint32_t)it.y + threadIdx.x;
+    if (idx >= (uint32_t)((pr.M + 3) / 4) * N) return;
+    const uint32_t row4 = idx / N, col = idx - row4 * N;
+    const int64_t plane = pr.RbA * (kBlk / 4) * pr.Np;
+    const uint32_t *Rq = pr.R + (int64_t)row4 * pr.Np + col;
+    uint32_t wd[kCrtMaxMod];
+#pragma unroll
+    for (int j = 0; j < kCrtMaxMod; j++) wd[j] = j < c.n ? Rq[(int64_t)j * plane] ^ 0x80808080u : 0u;
+    const int rest = pr.M - 4 * (int)row4;
+    crt_rebuild_quad(wd, c, rest < 4 ? rest : 4, pr.Er + 4 * (int64_t)row4, pr.Ec[col], pr.beta, rounding,
+                     pr.C + 4 * (int64_t)row4 * pr.ldc + col, pr.ldc);
+}
+
+hipError_t launch_group_crt_rebuild(const CrtGroupProb *probs, const int4 *items, int n, int rounding,
+                                    const CrtConsts &c, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_grp_crt_rebuild, dim3((unsigned)n), dim3(256), 0, st, probs, items, rounding, c);
     return hipGetLastError();
 }
 

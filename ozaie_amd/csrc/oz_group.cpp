@@ -1,7 +1,8 @@
 // oz_group.cpp -- grouped emulated DGEMM: many independent problems of uneven shape in
 // one split launch and one fused-GEMM launch per tile configuration, captured once in a
 // hipGraph and replayed; plus the autotuner that picks the tile configuration of each
-// size class.
+// size class.  The CRT mode (flags bit 3) runs the same way with residue planes instead of
+// slices: exponent max, residues, one modulus-GEMM launch per tile shape, rebuild.
 //
 // The reference has no batched entry point (include/oz_aie.h handles one square product
 // per setup); this is new functionality on top of the same arithmetic.
@@ -55,6 +56,40 @@ int pick_cfg(int d, int64_t M, int64_t N, int64_t K, int force) {
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
+// CRT mode: g_tune_crt[moduli][size class] = CrtCfg, -1 = untuned.
+int g_tune_crt[kCrtMaxMod + 1][9];
+bool g_tune_crt_init = false;
+
+void tune_crt_init() {
+    if (g_tune_crt_init) return;
+    for (auto &row : g_tune_crt)
+        for (int &v : row) v = -1;
+    g_tune_crt_init = true;
+}
+
+// Untuned classes: the tile shape with the least padded work, the smaller tiles weighted
+// by their lower MFMA rate (fragments per MFMA through LDS rise as the tile shrinks).
+int default_crt_cfg(int64_t M, int64_t N) {
+    static const double rate[CCFG_COUNT] = {1.0, 0.75, 0.4};
+    int best = CCFG_44;
+    double best_cost = 0.0;
+    for (int c = 0; c < CCFG_COUNT; c++) {
+        const int64_t tm = 64 * crt_group_cfg_wm(c), tn = 64 * crt_group_cfg_wn(c);
+        const double cost = (double)round_up(M, tm) * (double)round_up(N, tn) / rate[c];
+        if (c == 0 || cost < best_cost) { best = c; best_cost = cost; }
+    }
+    return best;
+}
+
+int pick_crt_cfg(int n, int64_t M, int64_t N, int64_t K, int force) {
+    if (force >= 0 && force < CCFG_COUNT) return force;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_crt_init();
+    const int t = g_tune_crt[n][size_class(M, N, K)];
+    if (t >= 0 && t < CCFG_COUNT) return t;
+    return default_crt_cfg(M, N);
+}
+
 }  // namespace
 
 struct ozaie_group_s {
@@ -90,6 +125,16 @@ struct ozaie_group_s {
     char *pbuf = nullptr;                  // item lists + control block
     PersistLaunch PL{};
     hipStream_t last_stream = nullptr;
+    // CRT mode (flags bit 3): d = moduli, b = requested bits (0: largest per problem)
+    bool crt = false;
+    CrtConsts cc{};
+    std::vector<int> beta;
+    CrtGroupProb *d_cprobs = nullptr, *h_cprobs = nullptr;
+    int4 *d_items_crt[CCFG_COUNT] = {};
+    int n_crt[CCFG_COUNT] = {};
+    int4 *d_items_rebuild = nullptr;
+    int n_rebuild = 0;
+    int crt_mpw[CCFG_COUNT] = {};         // moduli per GEMM workgroup, per tile shape
 };
 
 namespace {
@@ -111,6 +156,11 @@ void group_free(ozaie_group g) {
         if (p) (void)hipFree(p);
     if (g->h_ops) (void)hipHostFree(g->h_ops);
     if (g->h_probs) (void)hipHostFree(g->h_probs);
+    if (g->d_cprobs) (void)hipFree(g->d_cprobs);
+    if (g->h_cprobs) (void)hipHostFree(g->h_cprobs);
+    for (auto &p : g->d_items_crt)
+        if (p) (void)hipFree(p);
+    if (g->d_items_rebuild) (void)hipFree(g->d_items_rebuild);
     delete g;
 }
 
@@ -252,9 +302,160 @@ int build_persist(ozaie_group g, const std::vector<int64_t> &Kp) {
     return OZAIE_OK;
 }
 
+// Work order of the modulus-GEMM tiles (one workgroup per tile): problems by decreasing
+// contraction length, each problem's tiles in band order cut into runs of 8, each run to
+// the XCD with the least work so far; then XCD x's list goes to blockIdx 8 i + x (the
+// hardware deals workgroups to XCDs round-robin), empty slots as problem -1.  Every XCD
+// gets about the same work, heaviest first, with neighbouring tiles of a problem (shared
+// rows of A and columns of B) on one XCD's L2.
+std::vector<int4> xcd_deal(const std::vector<int4> &tiles, const std::vector<int64_t> &Kp) {
+    constexpr int X = 8, RUN = 8;
+    if (tiles.empty()) return tiles;
+    std::vector<std::pair<int, int>> spans;   // [first, last) of each problem's tiles
+    for (size_t i = 0; i < tiles.size();) {
+        size_t e = i;
+        while (e < tiles.size() && tiles[e].x == tiles[i].x) e++;
+        spans.emplace_back((int)i, (int)e);
+        i = e;
+    }
+    std::stable_sort(spans.begin(), spans.end(), [&](const std::pair<int, int> &a, const std::pair<int, int> &b) {
+        return Kp[tiles[a.first].x] > Kp[tiles[b.first].x];
+    });
+    std::vector<std::vector<int4>> q(X);
+    std::vector<int64_t> load(X, 0);
+    for (const auto &sp : spans)
+        for (int i = sp.first; i < sp.second; i += RUN) {
+            const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            const int e = std::min(sp.second, i + RUN);
+            for (int t = i; t < e; t++) q[x].push_back(tiles[t]);
+            load[x] += (int64_t)(e - i) * Kp[tiles[i].x];
+        }
+    size_t len = 0;
+    for (const auto &v : q) len = std::max(len, v.size());
+    std::vector<int4> out(len * X, int4{-1, 0, 0, 0});
+    for (int x = 0; x < X; x++)
+        for (size_t i = 0; i < q[x].size(); i++) out[i * X + x] = q[x][i];
+    return out;
+}
+
+// CRT mode: per problem the largest exact bit count for its K (or the requested one), a
+// tile shape per size class, residue planes padded to the tile, and the item tables of
+// the four phases.
+int group_build_crt(ozaie_group g, int force_cfg) {
+    const int n = g->count, nm = g->d;
+    std::vector<int64_t> Mp(n), Np(n), Kp(n);
+    int64_t nE = 0, s_bytes = 0, r_bytes = 0;
+    g->beta.assign(n, 0);
+    for (int p = 0; p < n; p++) {
+        if (g->K[p] > 65536) {
+            set_error_msg("CRT mode needs K <= 65536 (exact int32 residue sums)");
+            return OZAIE_ERR_DIMS;
+        }
+        const int bmax = crt_beta(nm, g->K[p]);
+        const int beta = g->b ? g->b : bmax;
+        if (bmax < 1 || beta < 1 || beta > bmax) {
+            set_error_msg("CRT bits exceed the exact range of the moduli for a problem's K");
+            return OZAIE_ERR_ARG;
+        }
+        if ((g->M[p] + 3) / 4 * g->N[p] > INT32_MAX - 256) {
+            set_error_msg("grouped CRT problems must have ceil(M / 4) N < 2^31");
+            return OZAIE_ERR_DIMS;
+        }
+        g->beta[p] = beta;
+        const int c = pick_crt_cfg(nm, g->M[p], g->N[p], g->K[p], force_cfg);
+        g->cfg[p] = c;
+        Mp[p] = round_up(g->M[p], 64 * crt_group_cfg_wm(c));
+        Np[p] = round_up(g->N[p], 64 * crt_group_cfg_wn(c));
+        Kp[p] = round_up(g->K[p], kBlk);
+        nE += Mp[p] + Np[p];
+        s_bytes += (int64_t)nm * (Mp[p] + Np[p]) * Kp[p];
+        r_bytes += (int64_t)nm * Mp[p] * Np[p];
+    }
+    crt_consts(nm, 0, g->cc);   // per-problem bit counts travel in the descriptors
+    const int64_t e_bytes = round_up(nE * 4, 1024);
+    s_bytes = round_up(s_bytes, 1024);
+    GCHK(hipMalloc(&g->ws, (size_t)(e_bytes + s_bytes + r_bytes)));
+    g->E_all = reinterpret_cast<int32_t *>(g->ws);
+    g->n_E = nE;
+    GCHK(hipHostMalloc(&g->h_ops, sizeof(GroupOperand) * 2 * n));
+    GCHK(hipHostMalloc(&g->h_cprobs, sizeof(CrtGroupProb) * n));
+    std::memset(g->h_ops, 0, sizeof(GroupOperand) * 2 * n);
+    std::memset(g->h_cprobs, 0, sizeof(CrtGroupProb) * n);
+    std::vector<int4> imax, isplit, irebuild;
+    std::vector<int4> igemm[CCFG_COUNT];
+    int64_t eoff = 0, soff = e_bytes, roff = e_bytes + s_bytes;
+    for (int p = 0; p < n; p++) {
+        GroupOperand &a = g->h_ops[2 * p], &bo = g->h_ops[2 * p + 1];
+        a.R = g->M[p]; a.K = g->K[p]; a.Rp = Mp[p]; a.sr = g->K[p]; a.sk = 1;
+        bo.R = g->N[p]; bo.K = g->K[p]; bo.Rp = Np[p]; bo.sr = 1; bo.sk = g->N[p];
+        a.E = g->E_all + eoff; eoff += Mp[p];
+        bo.E = g->E_all + eoff; eoff += Np[p];
+        a.S = reinterpret_cast<int8_t *>(g->ws + soff); soff += (int64_t)nm * Mp[p] * Kp[p];
+        bo.S = reinterpret_cast<int8_t *>(g->ws + soff); soff += (int64_t)nm * Np[p] * Kp[p];
+        a.Rb = Mp[p] / kBlk; bo.Rb = Np[p] / kBlk;
+        a.Kb = bo.Kb = Kp[p] / kBlk;
+        for (int64_t q = 0; q < (Mp[p] + 3) / 4; q++) imax.push_back(int4{2 * p, (int)q, 0, 0});
+        const int64_t kper = 256;
+        for (int64_t cx = 0; cx < (g->N[p] + 255) / 256; cx++)
+            for (int64_t ky = 0; ky < (g->K[p] + kper - 1) / kper; ky++)
+                imax.push_back(int4{2 * p + 1, (int)cx, (int)ky, (int)kper});
+        for (int op = 0; op < 2; op++) {
+            const GroupOperand &o = g->h_ops[2 * p + op];
+            for (int64_t rb = 0; rb < o.Rb; rb++)
+                for (int64_t kt = 0; kt < (o.Kb + 3) / 4; kt++)
+                    isplit.push_back(int4{2 * p + op, (int)rb, (int)kt, 0});
+        }
+        CrtGroupProb &cp = g->h_cprobs[p];
+        cp.SA = a.S; cp.SB = bo.S; cp.Er = a.E; cp.Ec = bo.E;
+        cp.R = reinterpret_cast<uint32_t *>(g->ws + roff); roff += (int64_t)nm * Mp[p] * Np[p];
+        cp.RbA = a.Rb; cp.RbB = bo.Rb; cp.Kb = a.Kb; cp.Np = Np[p]; cp.ldc = g->N[p];
+        cp.M = (int)g->M[p]; cp.N = (int)g->N[p]; cp.beta = g->beta[p];
+        const int c = g->cfg[p];
+        const int ntm = (int)(Mp[p] / (64 * crt_group_cfg_wm(c))), ntn = (int)(Np[p] / (64 * crt_group_cfg_wn(c)));
+        for (int t = 0; t < ntm * ntn; t++) {
+            int tm, tn;
+            band_map(t, ntm, ntn, tm, tn);
+            igemm[c].push_back(int4{p, tm, tn, 0});
+        }
+        const int64_t quads = (g->M[p] + 3) / 4 * g->N[p];
+        for (int64_t base = 0; base < quads; base += 256) irebuild.push_back(int4{p, (int)base, 0, 0});
+    }
+    // Moduli per GEMM workgroup: all of them (the next modulus's loads overlap each
+    // epilogue) unless that leaves fewer than 8 workgroups per CU to balance, then fewer.
+    const int forced_mpw = ozaie_get_option(OZAIE_OPT_CRT_GROUP_MODULI);
+    for (int c = 0; c < CCFG_COUNT; c++) {
+        igemm[c] = xcd_deal(igemm[c], Kp);
+        int mpw = forced_mpw > 0 ? std::min(forced_mpw, nm)
+                                 : (int)std::max<int64_t>(1, std::min<int64_t>(nm, (int64_t)igemm[c].size() * nm / 2048));
+        mpw = (nm + (nm + mpw - 1) / mpw - 1) / ((nm + mpw - 1) / mpw);   // even split of the moduli
+        g->crt_mpw[c] = mpw;
+        g->tiles += (int64_t)igemm[c].size() * ((nm + mpw - 1) / mpw);
+    }
+    int rc;
+    GCHK(hipMalloc(&g->d_ops, sizeof(GroupOperand) * 2 * n));
+    GCHK(hipMalloc(&g->d_cprobs, sizeof(CrtGroupProb) * n));
+    GCHK(hipEventCreateWithFlags(&g->upload_done, hipEventDisableTiming));
+    GCHK(hipStreamCreateWithFlags(&g->cap_stream, hipStreamNonBlocking));
+    g->lastA.assign(n, nullptr);
+    g->lastB.assign(n, nullptr);
+    g->lastC.assign(n, nullptr);
+    if ((rc = upload(&g->d_items_max, imax))) return rc;
+    if ((rc = upload(&g->d_items_split, isplit))) return rc;
+    if ((rc = upload(&g->d_items_rebuild, irebuild))) return rc;
+    g->n_max = (int)imax.size();
+    g->n_split = (int)isplit.size();
+    g->n_rebuild = (int)irebuild.size();
+    for (int c = 0; c < CCFG_COUNT; c++) {
+        if ((rc = upload(&g->d_items_crt[c], igemm[c]))) return rc;
+        g->n_crt[c] = (int)igemm[c].size();
+    }
+    return OZAIE_OK;
+}
+
 // Workspace layout, descriptors and work-item tables; pointers to A, B, C are filled in
 // at run time.
 int group_build(ozaie_group g, int force_cfg) {
+    if (g->crt) return group_build_crt(g, force_cfg);
     const int n = g->count;
     const int64_t kmax = max_kchunk_blocks(g->b, g->d) * kBlk;
     std::vector<int64_t> Mp(n), Np(n), Kp(n);
@@ -348,6 +549,15 @@ int group_build(ozaie_group g, int force_cfg) {
 }
 
 int group_enqueue(ozaie_group g, hipStream_t st) {
+    if (g->crt) {
+        GCHK(launch_fill_i32(g->E_all, g->n_E, 0, st));
+        GCHK(launch_group_maxexp(g->d_ops, g->d_items_max, g->n_max, st));
+        GCHK(launch_group_crt_split(g->d_ops, g->d_cprobs, g->d_items_split, g->n_split, g->cc, st));
+        for (int c = 0; c < CCFG_COUNT; c++)
+            GCHK(launch_group_crt_gemm(g->d_cprobs, g->d_items_crt[c], g->n_crt[c], c, g->crt_mpw[c], g->cc, st));
+        GCHK(launch_group_crt_rebuild(g->d_cprobs, g->d_items_rebuild, g->n_rebuild, g->rounding, g->cc, st));
+        return OZAIE_OK;
+    }
     if (g->persist) {
         GCHK(launch_group_persistent(g->PL, g->d, g->pcfg, st));
         return OZAIE_OK;
@@ -376,7 +586,25 @@ int group_new(ozaie_group *out, int count, const int64_t *M, const int64_t *N, c
               int slices, int flags, int force_cfg) {
     *out = nullptr;
     if (count < 1 || !M || !N || !K) return OZAIE_ERR_ARG;
-    if (bits < 1 || bits > kMaxBits || slices < 1 || slices > kMaxSlices || bits * slices > 96) return OZAIE_ERR_ARG;
+    const bool crt = (flags & OZAIE_GROUP_CRT) != 0;
+    if (crt) {
+        if (slices < 2 || slices > kCrtMaxMod || bits < 0 || bits > 62) {
+            set_error_msg("CRT groups need 2 <= moduli <= 16 and 0 <= bits <= 62");
+            return OZAIE_ERR_ARG;
+        }
+        if (flags & OZAIE_GROUP_PERSISTENT) {
+            set_error_msg("the CRT mode has no persistent kernel");
+            return OZAIE_ERR_ARG;
+        }
+        const int want = ((flags >> 4) & 15) - 1;
+        if (want >= CCFG_COUNT) {
+            set_error_msg("no such CRT tile shape");
+            return OZAIE_ERR_ARG;
+        }
+        if (want >= 0) force_cfg = want;
+    } else if (bits < 1 || bits > kMaxBits || slices < 1 || slices > kMaxSlices || bits * slices > 96) {
+        return OZAIE_ERR_ARG;
+    }
     for (int p = 0; p < count; p++)
         if (M[p] < 1 || N[p] < 1 || K[p] < 1 || M[p] > INT32_MAX / 2 || N[p] > INT32_MAX / 2 || K[p] > INT32_MAX / 2)
             return OZAIE_ERR_DIMS;
@@ -396,8 +624,9 @@ int group_new(ozaie_group *out, int count, const int64_t *M, const int64_t *N, c
     int rc = OZAIE_OK;
     hipError_t e = hipGetDevice(&g->device);
     if (e != hipSuccess) rc = set_error(e, "hipGetDevice");
-    g->fallback = !group_cfg_supported(slices, GCFG_D11);
-    if (!g->fallback && (flags & 4)) {
+    g->crt = crt;
+    g->fallback = !crt && !group_cfg_supported(slices, GCFG_D11);
+    if (!crt && !g->fallback && (flags & 4)) {
         // persistent single launch: forced configuration if it has a persistent kernel
         const int want = ((flags >> 4) & 15) - 1 >= 0 ? ((flags >> 4) & 15) - 1 : force_cfg;
         g->pcfg = (want >= 0 && persist_cfg_supported(slices, want)) ? want : default_persist_cfg(slices);
@@ -470,11 +699,15 @@ int group_run(ozaie_group g, const double *const *A, const double *const *B, dou
         for (int p = 0; p < g->count; p++) {
             g->h_ops[2 * p].X = A[p];
             g->h_ops[2 * p + 1].X = B[p];
-            g->h_probs[p].C = C[p];
+            if (g->crt) g->h_cprobs[p].C = C[p];
+            else g->h_probs[p].C = C[p];
             g->lastA[p] = A[p]; g->lastB[p] = B[p]; g->lastC[p] = C[p];
         }
         GCHK(hipMemcpyAsync(g->d_ops, g->h_ops, sizeof(GroupOperand) * 2 * g->count, hipMemcpyHostToDevice, st));
-        GCHK(hipMemcpyAsync(g->d_probs, g->h_probs, sizeof(GemmLaunch) * g->count, hipMemcpyHostToDevice, st));
+        if (g->crt)
+            GCHK(hipMemcpyAsync(g->d_cprobs, g->h_cprobs, sizeof(CrtGroupProb) * g->count, hipMemcpyHostToDevice, st));
+        else
+            GCHK(hipMemcpyAsync(g->d_probs, g->h_probs, sizeof(GemmLaunch) * g->count, hipMemcpyHostToDevice, st));
         GCHK(hipEventRecord(g->upload_done, st));
         g->uploaded = true;
     }
@@ -484,6 +717,58 @@ int group_run(ozaie_group g, const double *const *A, const double *const *B, dou
         if (rc) return rc;
     }
     GCHK(hipGraphLaunch(g->exec, st));
+    return OZAIE_OK;
+}
+
+// Autotuner core: for each of the 9 size classes, enough copies of a representative shape
+// to fill the GPU a few times, one group per candidate configuration, timed over `reps`
+// graph replays after a capture + warm-up run; best[class] = the fastest candidate.
+template <typename Default, typename Make>
+int tune_classes(int reps, const std::vector<int> &cands, Default dflt, Make make, int32_t *best) {
+    if (reps < 1) reps = 3;
+    static const int64_t rep_mn[3] = {128, 512, 2048}, rep_k[3] = {128, 512, 2048};
+    for (int a = 0; a < 3; a++)
+        for (int kb = 0; kb < 3; kb++) {
+            const int64_t mn = rep_mn[a], kk = rep_k[kb];
+            const int64_t tiles1 = ((mn + 63) / 64) * ((mn + 63) / 64);
+            const int cnt = (int)std::max<int64_t>(1, std::min<int64_t>(64, 2048 / tiles1));
+            std::vector<int64_t> Ms(cnt, mn), Ns(cnt, mn), Ks(cnt, kk);
+            double *buf = nullptr;
+            const int64_t per = mn * kk;   // A and B are mn x kk / kk x mn
+            hipError_t e = hipMalloc(&buf, sizeof(double) * (size_t)(2 * per + mn * mn));
+            if (e != hipSuccess) return set_error(e, "hipMalloc");
+            e = launch_fill_random(buf, 2 * per, 1234u + a * 3 + kb, 0);
+            if (e != hipSuccess) { (void)hipFree(buf); return set_error(e, "fill"); }
+            std::vector<const double *> As(cnt, buf), Bs(cnt, buf + per);
+            std::vector<double *> Cs(cnt, buf + 2 * per);
+            float best_ms = 1e30f;
+            int best_cfg = dflt(mn);
+            for (int c : cands) {
+                ozaie_group g = nullptr;
+                int rc = make(&g, cnt, Ms.data(), Ns.data(), Ks.data(), c);
+                if (rc) { (void)hipFree(buf); return rc; }
+                hipEvent_t e0, e1;
+                (void)hipEventCreate(&e0);
+                (void)hipEventCreate(&e1);
+                rc = group_run(g, As.data(), Bs.data(), Cs.data(), 0);   // capture + warm-up
+                if (!rc) {
+                    (void)hipEventRecord(e0, 0);
+                    for (int r = 0; r < reps && !rc; r++) rc = group_run(g, As.data(), Bs.data(), Cs.data(), 0);
+                    (void)hipEventRecord(e1, 0);
+                    (void)hipEventSynchronize(e1);
+                    float ms = 0.f;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    if (!rc && ms < best_ms) { best_ms = ms; best_cfg = c; }
+                }
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                (void)hipDeviceSynchronize();
+                group_free(g);
+                if (rc) { (void)hipFree(buf); return rc; }
+            }
+            (void)hipFree(buf);
+            best[3 * a + kb] = best_cfg;
+        }
     return OZAIE_OK;
 }
 
@@ -513,6 +798,10 @@ int32_t ozaie_group_query(ozaie_group g, int32_t *out4) {
     int launches = 0;
     if (g->fallback) launches = g->count * 5;
     else if (g->persist) launches = 1;
+    else if (g->crt) {
+        launches = 4;
+        for (int c = 0; c < CCFG_COUNT; c++) launches += g->n_crt[c] > 0;
+    }
     else {
         launches = 3;
         for (int c = 0; c < GCFG_COUNT; c++) launches += g->n_gemm[c] > 0;
@@ -578,53 +867,15 @@ int32_t ozaie_tune_set(int slices, const int32_t *table9) {
 int32_t ozaie_autotune(int bits, int slices, int reps, int32_t *table9) {
     if (bits < 1 || bits > kMaxBits || slices < 1 || slices > kMaxSlices) return OZAIE_ERR_ARG;
     if (!group_cfg_supported(slices, GCFG_D11)) return OZAIE_ERR_ARG;
-    if (reps < 1) reps = 3;
-    static const int64_t rep_mn[3] = {128, 512, 2048}, rep_k[3] = {128, 512, 2048};
+    std::vector<int> cands;
+    for (int c = 0; c < GCFG_COUNT; c++)
+        if (group_cfg_supported(slices, c)) cands.push_back(c);
     int32_t best[9];
-    for (int a = 0; a < 3; a++)
-        for (int kb = 0; kb < 3; kb++) {
-            const int64_t mn = rep_mn[a], kk = rep_k[kb];
-            // enough copies of the representative shape to fill the GPU a few times
-            const int64_t tiles1 = ((mn + 63) / 64) * ((mn + 63) / 64);
-            const int cnt = (int)std::max<int64_t>(1, std::min<int64_t>(64, 2048 / tiles1));
-            std::vector<int64_t> Ms(cnt, mn), Ns(cnt, mn), Ks(cnt, kk);
-            double *buf = nullptr;
-            const int64_t per = mn * kk;   // A and B are mn x kk / kk x mn
-            hipError_t e = hipMalloc(&buf, sizeof(double) * (size_t)(2 * per + mn * mn));
-            if (e != hipSuccess) return set_error(e, "hipMalloc");
-            e = launch_fill_random(buf, 2 * per, 1234u + a * 3 + kb, 0);
-            if (e != hipSuccess) { (void)hipFree(buf); return set_error(e, "fill"); }
-            std::vector<const double *> As(cnt, buf), Bs(cnt, buf + per);
-            std::vector<double *> Cs(cnt, buf + 2 * per);
-            float best_ms = 1e30f;
-            int best_cfg = default_cfg(slices, mn, mn);
-            for (int c = 0; c < GCFG_COUNT; c++) {
-                if (!group_cfg_supported(slices, c)) continue;
-                ozaie_group g = nullptr;
-                int rc = group_new(&g, cnt, Ms.data(), Ns.data(), Ks.data(), bits, slices, 0, c);
-                if (rc) { (void)hipFree(buf); return rc; }
-                hipEvent_t e0, e1;
-                (void)hipEventCreate(&e0);
-                (void)hipEventCreate(&e1);
-                rc = group_run(g, As.data(), Bs.data(), Cs.data(), 0);   // capture + warm-up
-                if (!rc) {
-                    (void)hipEventRecord(e0, 0);
-                    for (int r = 0; r < reps && !rc; r++) rc = group_run(g, As.data(), Bs.data(), Cs.data(), 0);
-                    (void)hipEventRecord(e1, 0);
-                    (void)hipEventSynchronize(e1);
-                    float ms = 0.f;
-                    (void)hipEventElapsedTime(&ms, e0, e1);
-                    if (!rc && ms < best_ms) { best_ms = ms; best_cfg = c; }
-                }
-                (void)hipEventDestroy(e0);
-                (void)hipEventDestroy(e1);
-                (void)hipDeviceSynchronize();
-                group_free(g);
-                if (rc) { (void)hipFree(buf); return rc; }
-            }
-            (void)hipFree(buf);
-            best[3 * a + kb] = best_cfg;
-        }
+    int rc = tune_classes(reps, cands, [&](int64_t mn) { return default_cfg(slices, mn, mn); },
+                          [&](ozaie_group *g, int cnt, const int64_t *Ms, const int64_t *Ns, const int64_t *Ks, int c) {
+                              return group_new(g, cnt, Ms, Ns, Ks, bits, slices, 0, c);
+                          }, best);
+    if (rc) return rc;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         tune_init();
@@ -632,6 +883,44 @@ int32_t ozaie_autotune(int bits, int slices, int reps, int32_t *table9) {
     }
     if (table9)
         for (int c = 0; c < 9; c++) table9[c] = best[c];
+    return OZAIE_OK;
+}
+
+int32_t ozaie_autotune_crt(int moduli, int reps, int32_t *table9) {
+    if (moduli < 2 || moduli > kCrtMaxMod) return OZAIE_ERR_ARG;
+    std::vector<int> cands;
+    for (int c = 0; c < CCFG_COUNT; c++) cands.push_back(c);
+    int32_t best[9];
+    int rc = tune_classes(reps, cands, [&](int64_t mn) { return default_crt_cfg(mn, mn); },
+                          [&](ozaie_group *g, int cnt, const int64_t *Ms, const int64_t *Ns, const int64_t *Ks, int c) {
+                              return group_new(g, cnt, Ms, Ns, Ks, 0, moduli, OZAIE_GROUP_CRT, c);
+                          }, best);
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_crt_init();
+        for (int c = 0; c < 9; c++) g_tune_crt[moduli][c] = best[c];
+    }
+    if (table9)
+        for (int c = 0; c < 9; c++) table9[c] = best[c];
+    return OZAIE_OK;
+}
+
+int32_t ozaie_tune_get_crt(int moduli, int32_t *table9) {
+    if (!table9 || moduli < 2 || moduli > kCrtMaxMod) return OZAIE_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_crt_init();
+    for (int c = 0; c < 9; c++) table9[c] = g_tune_crt[moduli][c];
+    return OZAIE_OK;
+}
+
+int32_t ozaie_tune_set_crt(int moduli, const int32_t *table9) {
+    if (!table9 || moduli < 2 || moduli > kCrtMaxMod) return OZAIE_ERR_ARG;
+    for (int c = 0; c < 9; c++)
+        if (table9[c] >= CCFG_COUNT) return OZAIE_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_crt_init();
+    for (int c = 0; c < 9; c++) g_tune_crt[moduli][c] = table9[c];
     return OZAIE_OK;
 }
 

@@ -131,7 +131,8 @@ void crt_consts(int n, int beta, CrtConsts &c);
 // Residues S[j][rb][kb][1024] (int8, slice-block layout) of one operand.
 hipError_t launch_crt_split(const OperandView &op, const int32_t *E, const CrtConsts &c, int8_t *S, int64_t Rp,
                             int64_t Kp, hipStream_t st);
-// g_j = (A_j B_j^T) mod p_j for every modulus: R[j][Mp/4][Np] words of four rows (uint8 each).
+// g_j = A_j B_j^T mod p_j for every modulus: R[j][Mp/4][Np] words of four rows, one signed
+// byte each (congruent to g_j, |byte| <= p_j / 2).
 struct CrtGemm {
     const int8_t *SA, *SB;
     int64_t Mp, Np, Kb;
@@ -143,6 +144,32 @@ hipError_t launch_crt_gemm(const CrtGemm &g, const CrtConsts &c, hipStream_t st)
 hipError_t launch_crt_rebuild(const uint32_t *R, int64_t Mp, int64_t Np, int64_t M, int64_t N, const int32_t *Er,
                               const int32_t *Ec, double *C, int64_t ldc, int rounding, const CrtConsts &c,
                               hipStream_t st);
+
+// Grouped CRT: one descriptor per problem (C is filled in at run time).  Residue planes of A
+// hold RbA row blocks of Kb contraction blocks (RbA = Mp / 32), those of B RbB; R holds n
+// planes of (Mp / 4) x Np words.
+struct CrtGroupProb {
+    const int8_t *SA, *SB;
+    const int32_t *Er, *Ec;
+    uint32_t *R;
+    double *C;
+    int64_t RbA, RbB, Kb, Np, ldc;
+    int M, N, beta, pad;
+};
+// Tile shapes of the grouped modulus GEMM: 256 x 256, 128 x 128, 64 x 64 per workgroup.
+enum CrtCfg { CCFG_44 = 0, CCFG_22 = 1, CCFG_11 = 2, CCFG_COUNT = 3 };
+int crt_group_cfg_wm(int cfg);   // row blocks of 64 per tile
+int crt_group_cfg_wn(int cfg);   // column blocks of 64 per tile
+// Items: int4 {operand, row block, contraction tile, 0} (split; the problem of operand o
+// is o / 2), {problem, tile row, tile col, 0} (GEMM: workgroup x runs item x for moduli
+// [mpw y, mpw y + mpw); problem -1 = empty slot) and {problem, first (row quad, column)
+// element, 0, 0} (rebuild, 256 elements per item).
+hipError_t launch_group_crt_split(const GroupOperand *ops, const CrtGroupProb *probs, const int4 *items, int n,
+                                  const CrtConsts &c, hipStream_t st);
+hipError_t launch_group_crt_gemm(const CrtGroupProb *probs, const int4 *items, int n, int cfg, int mpw,
+                                 const CrtConsts &c, hipStream_t st);
+hipError_t launch_group_crt_rebuild(const CrtGroupProb *probs, const int4 *items, int n, int rounding,
+                                    const CrtConsts &c, hipStream_t st);
 
 // Synthetic (U - 0.5) 2^e data for timing runs.
 hipError_t launch_fill_random(double *p, int64_t n, uint32_t seed, hipStream_t st);

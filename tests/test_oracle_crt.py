@@ -180,3 +180,30 @@ def test_crt_error_bound_vs_exact_product():
                         for x, y in zip(A[r], B[:, c]))
             bound += abs(exact) * Fraction(2) ** -52
             assert abs(Fraction(float(C[r, c])) - exact) <= bound
+
+
+@pytest.mark.parametrize("n", [2, 9, 16])
+def test_rebuild_from_offset_symmetric_residues(n):
+    """The GPU stores each residue product as a signed byte r (r = g mod p, -128 <= r <= 127,
+    128 kept as -128 for p = 256) and rebuilds from u = r + 128: because sum_j W_j = 1 (mod M),
+    C' = sum u W - nearest(sum u f) M - 128 for every |C'| <= M / 4."""
+    ps = oracle.crt_moduli(n)
+    M = math.prod(ps)
+    W, f, M2 = oracle.crt_consts(n)
+    assert M2 == M and sum(W) % M == 1
+    rnd = random.Random(100 + n)
+    vals = [0, 1, -1, M // 4, -(M // 4), 128, -128] + [rnd.randrange(-(M // 4), M // 4 + 1) for _ in range(400)]
+    for v in vals:
+        u = []
+        for p in ps:
+            r = v % p
+            r = r - p if r > p // 2 else r            # symmetric, in [-(p-1)/2, (p-1)/2] or [-128, 128]
+            r = ((r & 0xFF) ^ 0x80) - 128             # stored as one byte, read back signed
+            assert (r - v) % p == 0
+            u.append(r + 128)
+        t = 0.0
+        for uj, fj in zip(u, f):
+            t += float(uj) * fj
+        k = round(t)
+        S = (sum(uj * wj for uj, wj in zip(u, W)) - k * M - 128) % 2 ** 128
+        assert (S - 2 ** 128 if S >= 2 ** 127 else S) == v
