@@ -117,12 +117,17 @@ void crt_consts(int n, int beta, CrtConsts &c) {
         c.f[i] = (double)w / (double)M;
         c.W[i][0] = (uint64_t)w;
         c.W[i][1] = (uint64_t)(w >> 64);
+        const uint32_t q = (uint32_t)std::floor(c.f[i] * 65536.0);   // < 2^16 since f < 1
+        for (int b = 0; b < 16; b++) c.Wd[i / 4][b] |= (uint32_t)((w >> (8 * b)) & 0xFF) << (8 * (i % 4));
+        c.Qd[0][i / 4] |= (q & 0xFFu) << (8 * (i % 4));
+        c.Qd[1][i / 4] |= (q >> 8) << (8 * (i % 4));
     }
     c.Mw[0] = (uint64_t)M;
     c.Mw[1] = (uint64_t)(M >> 64);
     const unsigned __int128 Mn = (unsigned __int128)0 - M;
     c.Mneg[0] = (uint64_t)Mn;
     c.Mneg[1] = (uint64_t)(Mn >> 64);
+    for (int b = 0; b < 16; b++) c.mneg8[b] = (uint32_t)((Mn >> (8 * b)) & 0xFF);
 }
 }  // namespace oz
 

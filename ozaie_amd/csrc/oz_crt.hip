@@ -333,33 +333,58 @@ __device__ __forceinline__ double crt_to_double(i128 t, int e2, int rounding) {
 // The four rows 4 row4 + e (e < rows) of one column from the n residue words wd[j], each
 // byte already flipped by 0x80: the stored signed residue r (congruent to the residue
 // product g, -128 <= r <= 127) read as the unsigned byte u = r + 128.  Since the weights
-// satisfy sum_j W_j = 1 (mod M), sum u W = C' + 128 (mod M); with t = sum u f and
-// k = nearest(t) (|C' + 128| / M is below 1/4 + 2^-100 and t is far closer than that to
-// sum u W / M), C' = sum u W - k M - 128 (mod 2^128), read as signed.  The sums run in four
-// 32-bit limb columns (each below 2^45, so 64-bit) with one carry pass.  Nearest
-// rounding uses the correctly rounded 128-bit integer conversion (then the same
-// power-of-two scaling as the truncating path).
+// satisfy sum_j W_j = 1 (mod M), S = sum_j u_j W_j = C' + 128 + K M for an integer K.
+// The words are transposed so that one word holds the bytes of four moduli of one row;
+// then byte dot products give the 16 byte limbs L_i = sum_j u_j w_ji (below 2^20) of S,
+// and K = floor((t + 2^15) / 2^16) with t = sum_j u_j q_j, q_j = floor(2^16 fl(W_j / M)):
+// t / 2^16 is within 0.0625 of S / M, which is within 1/4 + 128 / M of K.  Then
+// C' = sum_i (L_i + K m_i) 256^i - 128 (mod 2^128), m_i the bytes of 2^128 - M, read as
+// signed.  Nearest rounding uses the correctly rounded 128-bit integer conversion (then the
+// same power-of-two scaling as the truncating path).
 __device__ __forceinline__ void crt_rebuild_quad(const uint32_t (&wd)[kCrtMaxMod], const CrtConsts &c, int rows,
                                                  const int32_t *__restrict__ Er4, int ec, int beta, int rounding,
                                                  double *__restrict__ Cq, int64_t ldc) {
+    constexpr int NG = kCrtMaxMod / 4;
+    uint32_t x[NG][4];   // x[G][e]: bytes u_{4G + b} of row e, b = 0..3
+#pragma unroll
+    for (int G = 0; G < NG; G++) {
+        // 4 x 4 byte transpose (v_perm_b32: selector byte k < 4 picks byte k of the second
+        // operand, 4 <= k < 8 byte k - 4 of the first)
+        const uint32_t t0 = __builtin_amdgcn_perm(wd[4 * G + 1], wd[4 * G], 0x05010400u);
+        const uint32_t t1 = __builtin_amdgcn_perm(wd[4 * G + 1], wd[4 * G], 0x07030602u);
+        const uint32_t t2 = __builtin_amdgcn_perm(wd[4 * G + 3], wd[4 * G + 2], 0x05010400u);
+        const uint32_t t3 = __builtin_amdgcn_perm(wd[4 * G + 3], wd[4 * G + 2], 0x07030602u);
+        x[G][0] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
+        x[G][1] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
+        x[G][2] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
+        x[G][3] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
+    }
 #pragma unroll
     for (int e = 0; e < 4; e++) {
         if (e >= rows) break;
-        double tf = 0.0;
+        uint32_t tlo = 0, thi = 0;
 #pragma unroll
-        for (int j = 0; j < kCrtMaxMod; j++) tf += (double)((wd[j] >> (8 * e)) & 0xFFu) * c.f[j];
-        const uint64_t k = (uint64_t)(int64_t)rint(tf);
-        uint64_t l0 = k * (uint32_t)c.Mneg[0], l1 = k * (uint32_t)(c.Mneg[0] >> 32);
-        uint64_t l2 = k * (uint32_t)c.Mneg[1], l3 = k * (uint32_t)(c.Mneg[1] >> 32);
-#pragma unroll
-        for (int j = 0; j < kCrtMaxMod; j++) {   // planes j >= n hold zero words
-            const uint64_t gv = (wd[j] >> (8 * e)) & 0xFFu;
-            l0 += gv * (uint32_t)c.W[j][0];
-            l1 += gv * (uint32_t)(c.W[j][0] >> 32);
-            l2 += gv * (uint32_t)c.W[j][1];
-            l3 += gv * (uint32_t)(c.W[j][1] >> 32);
+        for (int G = 0; G < NG; G++) {
+            tlo = __builtin_amdgcn_udot4(x[G][e], c.Qd[0][G], tlo, false);
+            thi = __builtin_amdgcn_udot4(x[G][e], c.Qd[1][G], thi, false);
         }
-        const u128 S = (u128)l0 + ((u128)l1 << 32) + ((u128)l2 << 64) + ((u128)l3 << 96) - (u128)128;
+        const uint32_t k = (tlo + (thi << 8) + 32768u) >> 16;
+        uint32_t L[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            uint32_t acc = k * c.mneg8[i];
+#pragma unroll
+            for (int G = 0; G < NG; G++) acc = __builtin_amdgcn_udot4(x[G][e], c.Wd[G][i], acc, false);
+            L[i] = acc;   // below 2^21
+        }
+        uint64_t Q[4];
+#pragma unroll
+        for (int h = 0; h < 4; h++) {
+            const uint32_t a = L[4 * h] + (L[4 * h + 1] << 8);        // below 2^30
+            const uint32_t b = L[4 * h + 2] + (L[4 * h + 3] << 8);
+            Q[h] = (uint64_t)a + ((uint64_t)b << 16);                 // below 2^47
+        }
+        const u128 S = (u128)Q[0] + ((u128)Q[1] << 32) + ((u128)Q[2] << 64) + ((u128)Q[3] << 96) - (u128)128;
         const int e2 = Er4[e] + ec - 2044 - 2 * beta;
         Cq[e * ldc] = rounding ? ldexp((double)(i128)S, e2) : crt_to_double((i128)S, e2, 0);
     }

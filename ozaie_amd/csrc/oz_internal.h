@@ -123,6 +123,12 @@ struct CrtConsts {
     uint64_t W[kCrtMaxMod][2]; // W_p = (M/p) ((M/p)^-1 mod p) mod M, (lo, hi)
     uint64_t Mw[2];            // M = prod p, (lo, hi)
     uint64_t Mneg[2];          // 2^128 - M, (lo, hi)
+    // rebuild by byte dot products: Wd[G][i] = byte i of W_{4G + b} in byte b;
+    // Qd[h][G] = byte h of q_{4G + b} = floor(2^16 f_{4G + b}) in byte b; mneg8[i] = byte i
+    // of 2^128 - M (zero beyond the n moduli)
+    uint32_t Wd[kCrtMaxMod / 4][16];
+    uint32_t Qd[2][kCrtMaxMod / 4];
+    uint32_t mneg8[16];
 };
 // Modulus list, bits per operand for (n, K) (or -1), and the constants.
 int crt_modulus(int i);

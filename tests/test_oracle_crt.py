@@ -207,3 +207,32 @@ def test_rebuild_from_offset_symmetric_residues(n):
         k = round(t)
         S = (sum(uj * wj for uj, wj in zip(u, W)) - k * M - 128) % 2 ** 128
         assert (S - 2 ** 128 if S >= 2 ** 127 else S) == v
+
+
+@pytest.mark.parametrize("n", [2, 5, 9, 16])
+def test_rebuild_byte_limbs_and_fixed_point_quotient(n):
+    """The GPU rebuild's arithmetic against the exact CRT inverse: u = r + 128 per modulus,
+    L_i = sum_j u_j w_ji over the 16 byte limbs w_ji of W_j, the quotient
+    k = floor((sum_j u_j q_j + 2^15) / 2^16) with q_j = floor(2^16 fl(W_j / M)) (< 2^16), and
+    C' = (sum_i (L_i + k m_i) 256^i - 128) mod 2^128 with m_i the bytes of 2^128 - M."""
+    ps = oracle.crt_moduli(n)
+    W, f, M = oracle.crt_consts(n)
+    wb = [[(w >> (8 * i)) & 0xFF for i in range(16)] for w in W]
+    q = [int(math.floor(fj * 65536.0)) for fj in f]
+    assert all(0 <= qj < 65536 for qj in q)
+    mneg = [((2 ** 128 - M) >> (8 * i)) & 0xFF for i in range(16)]
+    rnd = random.Random(200 + n)
+    vals = [0, 1, -1, M // 4, -(M // 4), 127, -128] + [rnd.randrange(-(M // 4), M // 4 + 1) for _ in range(400)]
+    for v in vals:
+        u = []
+        for p in ps:
+            r = v % p
+            r = r - p if r > p // 2 else r
+            u.append((((r & 0xFF) ^ 0x80) - 128) + 128)
+        t = sum(uj * qj for uj, qj in zip(u, q))
+        assert t < 2 ** 28
+        k = (t + 32768) >> 16
+        L = [sum(u[j] * wb[j][i] for j in range(n)) + k * mneg[i] for i in range(16)]
+        assert max(L) < 2 ** 21
+        S = (sum(Li << (8 * i) for i, Li in enumerate(L)) - 128) % 2 ** 128
+        assert (S - 2 ** 128 if S >= 2 ** 127 else S) == v
