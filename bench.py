@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=12445)
     ap.add_argument("--slices", default="3,7,8,9", help="b=7 slice counts to measure")
     ap.add_argument("--crt", default="16", help="CRT-mode modulus counts to measure ('' for none)")
+    ap.add_argument("--crt-long", default="16,17",
+                    help="CRT-mode modulus counts for the long-K section (17: 192-bit rebuild, more bits)")
     ap.add_argument("--block", type=int, default=256, help="side of the sampled error block")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-target", action="store_true", help="skip the 8192^3 section")
@@ -308,16 +310,17 @@ def long_k(args, torch, np, oz, slices=(7, 8, 9)):
         err = rel_errors(np, C[ri, ci].cpu().numpy(), exact)
         rec["presets"].append({"scheme": "split", "bits": 7, "slices": d, "ms": t, "tflops": flop / t / 1e9,
                                "speedup": t_nat / t,
-                               "max_rel_err": err[0], "median_rel_err": err[1], "passes": p.info()["passes"]})
+                               "max_rel_err": err[0], "median_rel_err": err[1], "passes": p.info()["passes"],
+                               "matched": err[0] <= nat_err[0]})
         log(args, f"  longK d={d}: {t:.1f} ms x{t_nat / t:.2f} err {err} passes {p.info()['passes']}")
         p.close()
-    for mods in crt_list(args):
+    for mods in [int(x) for x in getattr(args, "crt_long", "16,17").split(",") if x.strip()]:
         p = oz.Plan(M, N, K, 0, mods, rounding=oz.ROUND_NEAREST, crt=True)
         t = time_steps(torch, lambda: p(A, B, C), 3, 1)
         err = rel_errors(np, C[ri, ci].cpu().numpy(), exact)
         rec["presets"].append({"scheme": "crt", "moduli": mods, "bits": p.info()["bits"], "ms": t,
                                "tflops": flop / t / 1e9, "speedup": t_nat / t, "max_rel_err": err[0],
-                               "median_rel_err": err[1], "passes": 1})
+                               "median_rel_err": err[1], "passes": 1, "matched": err[0] <= nat_err[0]})
         log(args, f"  longK crt {mods} moduli ({p.info()['bits']} bits): {t:.1f} ms x{t_nat / t:.2f} err {err}")
         p.close()
     log(args, f"  longK native {t_nat:.1f} ms err {nat_err}")

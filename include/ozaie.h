@@ -94,7 +94,7 @@ typedef struct ozaie_plan_s *ozaie_plan;
 
 /* Create a plan for C (M x N) = op(A) (M x K) * op(B) (K x N) with b bits and d slices.
  * flags: bit 0 -> nearest-even rounding (default truncation).
- * flags bit 2 (OZAIE_PLAN_CRT) -> CRT mode: `slices` = number of moduli (2..16), `bits`
+ * flags bit 2 (OZAIE_PLAN_CRT) -> CRT mode: `slices` = number of moduli (2..20), `bits`
  * = bits per operand (0 = the largest the moduli rebuild exactly for this K).  No
  * reference counterpart; K <= 65536.                                                   */
 #define OZAIE_PLAN_NEAREST 1
@@ -132,7 +132,7 @@ int32_t ozaie_plan_phase_times(ozaie_plan plan, float *out3);
  * flags bits 4..7: persistent tile configuration + 1 (0: per-d default; configuration
  * ids as for ozaie_autotune, direct-load tiles only).
  * flags bit 3 (OZAIE_GROUP_CRT): the CRT mode for every problem (no reference
- * counterpart): `slices` = moduli (2..16), `bits` = 0 for the largest exact bit count of
+ * counterpart): `slices` = moduli (2..20), `bits` = 0 for the largest exact bit count of
  * each problem's K (crt_beta(moduli, K_i)) or a fixed count no larger than any problem's;
  * K_i <= 65536.  Four phase launches plus one modulus-GEMM launch per tile shape (ids as
  * for ozaie_autotune_crt; flags bits 4..7 = forced shape + 1), same graph capture; each

@@ -75,8 +75,8 @@ def _setup(L):
     L.oro_crt_q.restype = i64
     L.oro_crt_split.argtypes = [P, i64, i64, i64, i64, P, i32, i32, P]
     L.oro_crt_products.argtypes = [P, P, i64, i64, i64, i32, P]
-    L.oro_crt_rebuild.argtypes = [P, i64, i32, ctypes.POINTER(i64), ctypes.POINTER(i64)]
-    L.oro_crt_to_double.argtypes = [i64, i64, i32, i32]
+    L.oro_crt_rebuild.argtypes = [P, i64, i32, P]
+    L.oro_crt_to_double.argtypes = [P, i32, i32]
     L.oro_crt_to_double.restype = ctypes.c_double
     L.oro_crt_dgemm.argtypes = [i64, i64, i64, P, i64, P, i64, P, i64, i32, i32]
     L.oro_crt_dgemm.restype = i32
@@ -243,11 +243,13 @@ def crt_beta(n, K):
 
 
 def crt_consts(n):
-    W = np.zeros(2 * n, dtype=np.uint64)
+    """(W_p as Python ints, fl(W_p / M) as float64, M)."""
+    W = np.zeros(3 * n, dtype=np.uint64)
     f = np.zeros(n, dtype=np.float64)
-    Mw = np.zeros(2, dtype=np.uint64)
+    Mw = np.zeros(3, dtype=np.uint64)
     lib().oro_crt_consts(int(n), _p(W), _p(f), _p(Mw))
-    return [int(W[2 * i]) | (int(W[2 * i + 1]) << 64) for i in range(n)], f, int(Mw[0]) | (int(Mw[1]) << 64)
+    word3 = lambda w: int(w[0]) | (int(w[1]) << 64) | (int(w[2]) << 128)
+    return [word3(W[3 * i:3 * i + 3]) for i in range(n)], f, word3(Mw)
 
 
 def crt_q(x, emax, beta):
@@ -283,18 +285,16 @@ def crt_products(SA, SB):
 def crt_rebuild(g):
     """Signed integer from the residues g[0..n) (Python int)."""
     g = np.ascontiguousarray(g, dtype=np.uint8)
-    hi = ctypes.c_int64()
-    lo = ctypes.c_int64()
-    lib().oro_crt_rebuild(_p(g), 1, len(g), ctypes.byref(hi), ctypes.byref(lo))
-    v = ((hi.value & (2**64 - 1)) << 64) | (lo.value & (2**64 - 1))
-    return v - 2**128 if v >= 2**127 else v
+    C3 = np.zeros(3, dtype=np.int64)
+    lib().oro_crt_rebuild(_p(g), 1, len(g), _p(C3))
+    v = sum((int(w) & (2**64 - 1)) << (64 * i) for i, w in enumerate(C3))
+    return v - 2**192 if v >= 2**191 else v
 
 
 def crt_to_double(v, e2, rounding=1):
-    v &= 2**128 - 1
-    hi, lo = v >> 64, v & (2**64 - 1)
-    to_s = lambda w: w - 2**64 if w >= 2**63 else w
-    return lib().oro_crt_to_double(to_s(hi), to_s(lo), int(e2), int(rounding))
+    v &= 2**192 - 1
+    C3 = np.array([((v >> (64 * i)) & (2**64 - 1)) for i in range(3)], dtype=np.uint64).view(np.int64)
+    return lib().oro_crt_to_double(_p(C3), int(e2), int(rounding))
 
 
 def crt_dgemm(A, B, n=16, rounding=1):

@@ -11,7 +11,8 @@
  *
  * The arithmetic, defined here and implemented by the GPU kernels:
  *   moduli      p = 256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211, 199,
- *               197, 193, 191 (the first n, 2 <= n <= 16; pairwise coprime); M = prod p.
+ *               197, 193, 191, 181, 179, 173, 167 (the first n, 2 <= n <= 20; pairwise
+ *               coprime); M = prod p (below 2^156).
  *   bits        beta = floor((L - 2 - ceil(log2 K)) / 2), L = floor(log2 M), at most
  *               62: then K * 2^(2 beta) <= M / 4, so |C'| <= M / 4.
  *   scaling     the row (A) / column (B) maximum exponent field E as in the split
@@ -21,9 +22,9 @@
  *   residues    r_p(x) = q(x) mod p in [-(p-1)/2, (p-1)/2] for odd p, [-128, 127] for
  *               p = 256: int8.
  *   products    g_p = (sum_k r_p(a_ik) r_p(b_kj)) mod p in [0, p) (int32 sums are exact).
- *   rebuild     W_p = (M/p) * ((M/p)^-1 mod p) mod M; S = sum_p g_p W_p (mod 2^128);
+ *   rebuild     W_p = (M/p) * ((M/p)^-1 mod p) mod M; S = sum_p g_p W_p (mod 2^192);
  *               t = sum_p g_p * fl(W_p / M) in double; k = nearest integer to t;
- *               C' = S - k M (mod 2^128) read as a signed 128-bit integer.  Because
+ *               C' = S - k M (mod 2^192) read as a signed 192-bit integer.  Because
  *               |C'| <= M/4 and the error of t is below 2^-30, k is exact and C' is the
  *               exact integer sum_k q(a_ik) q(b_kj).
  *   output      C = C' * 2^(E_r + E_c - 2044 - 2 beta), truncated toward zero or rounded
@@ -36,31 +37,72 @@
 #include <stdlib.h>
 #include <math.h>
 
-typedef __int128 i128;
 typedef unsigned __int128 u128;
 
-static const int crt_p[16] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211, 199, 197, 193, 191};
+#define CRT_MAXMOD 20
+static const int crt_p[CRT_MAXMOD] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+                                      223, 211, 199, 197, 193, 191, 181, 179, 173, 167};
 
 static inline uint64_t dbits(double x) { uint64_t u; memcpy(&u, &x, 8); return u; }
 
-int oro_crt_modulus(int i) { return (i >= 0 && i < 16) ? crt_p[i] : 0; }
+/* 192-bit unsigned integers, three little-endian 64-bit words, arithmetic mod 2^192 */
+typedef struct { uint64_t w[3]; } u192;
 
-static u128 crt_M(int n) {
-    u128 m = 1;
-    for (int i = 0; i < n; i++) m *= (u128)crt_p[i];
-    return m;
+static u192 u192_small(uint64_t v) { u192 r = {{v, 0, 0}}; return r; }
+
+static u192 u192_add(u192 a, u192 b) {
+    u192 r;
+    u128 c = 0;
+    for (int i = 0; i < 3; i++) { c += (u128)a.w[i] + b.w[i]; r.w[i] = (uint64_t)c; c >>= 64; }
+    return r;
 }
 
-static int floor_log2_u128(u128 v) {
-    int l = -1;
-    while (v) { v >>= 1; l++; }
-    return l;
+static u192 u192_neg(u192 a) {
+    u192 r;
+    u128 c = 1;
+    for (int i = 0; i < 3; i++) { c += (u128)(~a.w[i]); r.w[i] = (uint64_t)c; c >>= 64; }
+    return r;
+}
+
+static u192 u192_mul_small(u192 a, uint64_t m) {   /* m < 2^32 */
+    u192 r;
+    u128 c = 0;
+    for (int i = 0; i < 3; i++) { c += (u128)a.w[i] * m; r.w[i] = (uint64_t)c; c >>= 64; }
+    return r;
+}
+
+static uint64_t u192_divmod_small(u192 *a, uint64_t d) {   /* a /= d, returns a mod d */
+    u128 r = 0;
+    for (int i = 2; i >= 0; i--) {
+        r = (r << 64) | a->w[i];
+        a->w[i] = (uint64_t)(r / d);
+        r %= d;
+    }
+    return (uint64_t)r;
+}
+
+static int u192_bitlen(u192 a) {
+    for (int i = 2; i >= 0; i--)
+        if (a.w[i]) return 64 * i + 64 - __builtin_clzll(a.w[i]);
+    return 0;
+}
+
+static double u192_to_double(u192 a) {   /* within a few ulps; used only for W/M */
+    return ((double)a.w[2] * 18446744073709551616.0 + (double)a.w[1]) * 18446744073709551616.0 + (double)a.w[0];
+}
+
+int oro_crt_modulus(int i) { return (i >= 0 && i < CRT_MAXMOD) ? crt_p[i] : 0; }
+
+static u192 crt_M(int n) {
+    u192 m = u192_small(1);
+    for (int i = 0; i < n; i++) m = u192_mul_small(m, (uint64_t)crt_p[i]);
+    return m;
 }
 
 /* bits per operand for n moduli and contraction length K, or -1 if none fits */
 int oro_crt_beta(int n, int64_t K) {
-    if (n < 2 || n > 16 || K < 1) return -1;
-    int L = floor_log2_u128(crt_M(n));
+    if (n < 2 || n > CRT_MAXMOD || K < 1) return -1;
+    int L = u192_bitlen(crt_M(n)) - 1;
     int lk = 0;
     while (((int64_t)1 << lk) < K) lk++;
     int beta = (L - 2 - lk) / 2;
@@ -78,20 +120,16 @@ static int inv_mod(int a, int p) {   /* a^-1 mod p, gcd(a, p) = 1 */
     return t < 0 ? t + p : t;
 }
 
-/* W_p as (lo, hi) words, f_p = fl(W_p / M), M as (lo, hi). */
-void oro_crt_consts(int n, uint64_t *W, double *f, uint64_t *Mw) {
-    u128 M = crt_M(n);
-    for (int i = 0; i < n; i++) {
-        u128 Mi = M / (u128)crt_p[i];
-        int c = inv_mod((int)(Mi % (u128)crt_p[i]), crt_p[i]);
-        u128 w = Mi * (u128)c;   /* < M */
-        W[2 * i] = (uint64_t)w;
-        W[2 * i + 1] = (uint64_t)(w >> 64);
-        f[i] = (double)w / (double)M;
-    }
-    Mw[0] = (uint64_t)M;
-    Mw[1] = (uint64_t)(M >> 64);
+static u192 crt_W(int n, int i) {
+    u192 Mi = crt_M(n);
+    u192_divmod_small(&Mi, (uint64_t)crt_p[i]);
+    u192 tmp = Mi;
+    const int c = inv_mod((int)u192_divmod_small(&tmp, (uint64_t)crt_p[i]), crt_p[i]);
+    return u192_mul_small(Mi, (uint64_t)c);   /* < M */
 }
+
+/* W_p as three words each (low first), f_p = fl(W_p / M), M as three words. */
+void oro_crt_consts(int n, uint64_t *W, double *f, uint64_t *Mw);
 
 /* Signed integer q(x) of one element for row/column exponent field emax. */
 int64_t oro_crt_q(double x, int emax, int beta) {
@@ -139,46 +177,79 @@ void oro_crt_products(const int8_t *SA, const int8_t *SB, int64_t M, int64_t N, 
         }
 }
 
-/* Exact C' from the n residues g[i * stride] (as hi/lo words of a signed 128-bit value). */
-void oro_crt_rebuild(const uint8_t *g, int64_t stride, int n, int64_t *Chi, int64_t *Clo) {
-    uint64_t W[32], Mw[2];
-    double f[16];
-    oro_crt_consts(n, W, f, Mw);
-    u128 S = 0;
+static void rebuild_with(const u192 *W, const double *f, u192 M, const uint8_t *g, int64_t stride, int n,
+                         int64_t *C3) {
+    u192 S = u192_small(0);
     double t = 0.0;
     for (int i = 0; i < n; i++) {
-        u128 w = ((u128)W[2 * i + 1] << 64) | W[2 * i];
-        S += (u128)g[i * stride] * w;
+        S = u192_add(S, u192_mul_small(W[i], g[i * stride]));
         t += (double)g[i * stride] * f[i];
     }
-    int64_t k = (int64_t)nearbyint(t);
-    u128 M = ((u128)Mw[1] << 64) | Mw[0];
-    S -= (u128)k * M;
-    *Chi = (int64_t)(uint64_t)(S >> 64);
-    *Clo = (int64_t)(uint64_t)S;
+    const int64_t k = (int64_t)nearbyint(t);   /* 0 <= k < n * 256 */
+    S = u192_add(S, u192_neg(u192_mul_small(M, (uint64_t)k)));
+    for (int i = 0; i < 3; i++) C3[i] = (int64_t)S.w[i];
 }
 
-/* C' * 2^e2 to double: truncation (rounding 0) or nearest-even (1) to 53 bits. */
-double oro_crt_to_double(int64_t Chi, int64_t Clo, int e2, int rounding) {
-    i128 t = (i128)(((u128)(uint64_t)Chi << 64) | (uint64_t)Clo);
-    if (t == 0) return 0.0;
-    int neg = t < 0;
-    u128 a = neg ? (u128)(-t) : (u128)t;
-    int nb = floor_log2_u128(a) + 1;
+static void crt_tables(int n, u192 *W, double *f, u192 *M) {
+    *M = crt_M(n);
+    for (int i = 0; i < n; i++) {
+        W[i] = crt_W(n, i);
+        f[i] = u192_to_double(W[i]) / u192_to_double(*M);
+    }
+}
+
+void oro_crt_consts(int n, uint64_t *W, double *f, uint64_t *Mw) {
+    u192 Wt[CRT_MAXMOD], M;
+    crt_tables(n, Wt, f, &M);
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) W[3 * i + k] = Wt[i].w[k];
+    for (int k = 0; k < 3; k++) Mw[k] = M.w[k];
+}
+
+/* Exact C' from the n residues g[i * stride], as three words (low first) of a signed 192-bit
+ * integer. */
+void oro_crt_rebuild(const uint8_t *g, int64_t stride, int n, int64_t *C3) {
+    u192 W[CRT_MAXMOD], M;
+    double f[CRT_MAXMOD];
+    crt_tables(n, W, f, &M);
+    rebuild_with(W, f, M, g, stride, n, C3);
+}
+
+/* C' * 2^e2 to double (C' signed 192-bit, three words low first): truncation (rounding 0)
+ * or nearest-even (1) to 53 bits. */
+double oro_crt_to_double(const int64_t *C3, int e2, int rounding) {
+    u192 a = {{(uint64_t)C3[0], (uint64_t)C3[1], (uint64_t)C3[2]}};
+    const int neg = (int64_t)a.w[2] < 0;
+    if (neg) a = u192_neg(a);
+    const int nb = u192_bitlen(a);
+    if (nb == 0) return 0.0;
     uint64_t m;
     int sh = 0;
-    if (nb <= 53) m = (uint64_t)a;
-    else {
+    if (nb <= 53) {
+        m = a.w[0];
+    } else {
         sh = nb - 53;
-        u128 q = a >> sh;
-        if (rounding) {
-            u128 rem = a - (q << sh);
-            u128 half = (u128)1 << (sh - 1);
-            if (rem > half || (rem == half && (q & 1))) q += 1;
+        /* q = a >> sh (fits 53 bits), rem = a mod 2^sh compared with 2^(sh-1) */
+        u192 q = a;
+        for (int s = sh; s > 0;) {
+            const int step = s > 63 ? 63 : s;
+            q.w[0] = (q.w[0] >> step) | (q.w[1] << (64 - step));
+            q.w[1] = (q.w[1] >> step) | (q.w[2] << (64 - step));
+            q.w[2] >>= step;
+            s -= step;
         }
-        m = (uint64_t)q;
+        m = q.w[0];
+        if (rounding) {
+            /* bit sh-1 of a is the half bit; the bits below it are the sticky part */
+            const int hb = sh - 1;
+            const int half = (int)((a.w[hb / 64] >> (hb % 64)) & 1);
+            int sticky = 0;
+            for (int b = 0; b < hb; b++)
+                if ((a.w[b / 64] >> (b % 64)) & 1) { sticky = 1; break; }
+            if (half && (sticky || (m & 1))) m += 1;
+        }
     }
-    double v = ldexp((double)m, sh + e2);
+    const double v = ldexp((double)m, sh + e2);
     return neg ? -v : v;
 }
 
@@ -199,12 +270,15 @@ int oro_crt_dgemm(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
     oro_crt_split(A, M, K, lda, 1, Er, beta, n, SA);
     oro_crt_split(B, N, K, 1, ldb, Ec, beta, n, SB);
     oro_crt_products(SA, SB, M, N, K, n, G);
+    u192 Wt[CRT_MAXMOD], Mt;
+    double ft[CRT_MAXMOD];
+    crt_tables(n, Wt, ft, &Mt);
     #pragma omp parallel for schedule(static)
     for (int64_t r = 0; r < M; r++)
         for (int64_t c = 0; c < N; c++) {
-            int64_t hi, lo;
-            oro_crt_rebuild(G + r * N + c, M * N, n, &hi, &lo);
-            C[r * ldc + c] = oro_crt_to_double(hi, lo, Er[r] + Ec[c] - 2044 - 2 * beta, rounding);
+            int64_t C3[3];
+            rebuild_with(Wt, ft, Mt, G + r * N + c, M * N, n, C3);
+            C[r * ldc + c] = oro_crt_to_double(C3, Er[r] + Ec[c] - 2044 - 2 * beta, rounding);
         }
     free(Er); free(Ec); free(SA); free(SB); free(G);
     return 0;

@@ -190,7 +190,8 @@ def mode_slices(mode, K):
     return {Mode.fast: (7, 8), Mode.fast3: (7, 3), Mode.accurate7: (7, 7)}[mode]
 
 
-CRT_MODULI = (256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211, 199, 197, 193, 191)
+CRT_MODULI = (256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211, 199, 197, 193, 191,
+              181, 179, 173, 167)
 
 
 def crt_beta(moduli, K):
@@ -254,7 +255,7 @@ def _stream_handle(stream):
 
 class Plan:
     """A device plan for C (M x N) = op(A) (M x K) @ op(B) (K x N) with b bits, d slices.
-    crt=True: CRT mode, `slices` moduli (2..16) and `bits` per operand (0 = largest exact)."""
+    crt=True: CRT mode, `slices` moduli (2..20) and `bits` per operand (0 = largest exact)."""
 
     def __init__(self, M, N, K, bits=7, slices=8, rounding=ROUND_TRUNCATE, crt=False):
         self.M, self.N, self.K, self.bits, self.slices = int(M), int(N), int(K), int(bits), int(slices)

@@ -68,25 +68,58 @@ namespace oz {
 int set_error(hipError_t e, const char *where) { return set_hip_error(e, where); }
 void set_error_msg(const char *msg) { g_last_error = msg; }
 
-// CRT mode constants: the first n of 16 pairwise-coprime moduli <= 256 (256 = 2^8 and odd
-// numbers whose prime factors are all distinct), M = prod p < 2^126.
-static const int kCrtModuli[kCrtMaxMod] = {256, 255, 253, 251, 247, 241, 239, 233,
-                                           229, 227, 223, 211, 199, 197, 193, 191};
+// CRT mode constants: the first n of 20 pairwise-coprime moduli <= 256 (256 = 2^8 and odd
+// numbers whose prime factors are all distinct), M = prod p < 2^156, held in 192-bit
+// integers (three 64-bit words, low first, arithmetic mod 2^192).
+static const int kCrtModuli[kCrtMaxMod] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+                                           223, 211, 199, 197, 193, 191, 181, 179, 173, 167};
 
 int crt_modulus(int i) { return (i >= 0 && i < kCrtMaxMod) ? kCrtModuli[i] : 0; }
 
-static unsigned __int128 crt_M(int n) {
-    unsigned __int128 m = 1;
-    for (int i = 0; i < n; i++) m *= (unsigned __int128)kCrtModuli[i];
+namespace {
+struct Wide { uint64_t w[3] = {0, 0, 0}; };
+
+Wide wide_mul(const Wide &a, uint64_t m) {   // m < 2^32
+    Wide r;
+    unsigned __int128 c = 0;
+    for (int i = 0; i < 3; i++) { c += (unsigned __int128)a.w[i] * m; r.w[i] = (uint64_t)c; c >>= 64; }
+    return r;
+}
+
+uint64_t wide_divmod(Wide &a, uint64_t d) {   // a /= d, returns a mod d
+    unsigned __int128 r = 0;
+    for (int i = 2; i >= 0; i--) {
+        r = (r << 64) | a.w[i];
+        a.w[i] = (uint64_t)(r / d);
+        r %= d;
+    }
+    return (uint64_t)r;
+}
+
+int wide_bits(const Wide &a) {
+    for (int i = 2; i >= 0; i--)
+        if (a.w[i]) return 64 * i + 64 - __builtin_clzll(a.w[i]);
+    return 0;
+}
+
+double wide_double(const Wide &a) {
+    return ((double)a.w[2] * 18446744073709551616.0 + (double)a.w[1]) * 18446744073709551616.0 + (double)a.w[0];
+}
+
+uint32_t wide_byte(const Wide &a, int b) { return (uint32_t)((a.w[b / 8] >> (8 * (b % 8))) & 0xFF); }
+
+Wide crt_M(int n) {
+    Wide m;
+    m.w[0] = 1;
+    for (int i = 0; i < n; i++) m = wide_mul(m, (uint64_t)kCrtModuli[i]);
     return m;
 }
+}  // namespace
 
 // Largest beta with K 2^(2 beta) <= 2^(floor(log2 M) - 2) <= M / 4 (at most 62).
 int crt_beta(int n, int64_t K) {
     if (n < 2 || n > kCrtMaxMod || K < 1) return -1;
-    unsigned __int128 m = crt_M(n);
-    int L = -1;
-    while (m) { m >>= 1; L++; }
+    const int L = wide_bits(crt_M(n)) - 1;
     int lk = 0;
     while (((int64_t)1 << lk) < K) lk++;
     const int beta = std::min(62, (L - 2 - lk) / 2);
@@ -97,37 +130,34 @@ void crt_consts(int n, int beta, CrtConsts &c) {
     std::memset(&c, 0, sizeof(c));
     c.n = n;
     c.beta = beta;
-    const unsigned __int128 M = crt_M(n);
+    const Wide M = crt_M(n);
     for (int i = 0; i < n; i++) {
         const int p = kCrtModuli[i];
-        const unsigned __int128 Mi = M / (unsigned __int128)p;
+        Wide Mi = M;
+        wide_divmod(Mi, (uint64_t)p);
+        Wide tmp = Mi;
         // inverse of Mi mod p by the extended Euclidean algorithm
-        int t = 0, nt = 1, r = p, nr = (int)(Mi % (unsigned __int128)p);
+        int t = 0, nt = 1, r = p, nr = (int)wide_divmod(tmp, (uint64_t)p);
         while (nr) {
             const int qq = r / nr;
-            int tmp = t - qq * nt; t = nt; nt = tmp;
-            tmp = r - qq * nr; r = nr; nr = tmp;
+            int tt = t - qq * nt; t = nt; nt = tt;
+            tt = r - qq * nr; r = nr; nr = tt;
         }
         if (t < 0) t += p;
-        const unsigned __int128 w = Mi * (unsigned __int128)t;   // < M
+        const Wide w = wide_mul(Mi, (uint64_t)t);   // W_p < M
         c.p[i] = p;
         c.inv[i] = 1.0 / p;
         c.c31[i] = (double)((1ull << 31) % (uint64_t)p);
-        c.half[i] = p == 256 ? 127.0 : (double)((p - 1) / 2);
-        c.f[i] = (double)w / (double)M;
-        c.W[i][0] = (uint64_t)w;
-        c.W[i][1] = (uint64_t)(w >> 64);
-        const uint32_t q = (uint32_t)std::floor(c.f[i] * 65536.0);   // < 2^16 since f < 1
-        for (int b = 0; b < 16; b++) c.Wd[i / 4][b] |= (uint32_t)((w >> (8 * b)) & 0xFF) << (8 * (i % 4));
+        const double f = wide_double(w) / wide_double(M);
+        const uint32_t q = (uint32_t)std::floor(f * 65536.0);   // < 2^16 since f < 1
+        for (int b = 0; b < kCrtLimbs; b++) c.Wd[i / 4][b] |= wide_byte(w, b) << (8 * (i % 4));
         c.Qd[0][i / 4] |= (q & 0xFFu) << (8 * (i % 4));
         c.Qd[1][i / 4] |= (q >> 8) << (8 * (i % 4));
     }
-    c.Mw[0] = (uint64_t)M;
-    c.Mw[1] = (uint64_t)(M >> 64);
-    const unsigned __int128 Mn = (unsigned __int128)0 - M;
-    c.Mneg[0] = (uint64_t)Mn;
-    c.Mneg[1] = (uint64_t)(Mn >> 64);
-    for (int b = 0; b < 16; b++) c.mneg8[b] = (uint32_t)((Mn >> (8 * b)) & 0xFF);
+    Wide Mn;   // 2^192 - M
+    unsigned __int128 cy = 1;
+    for (int i = 0; i < 3; i++) { cy += (unsigned __int128)(~M.w[i]); Mn.w[i] = (uint64_t)cy; cy >>= 64; }
+    for (int b = 0; b < kCrtLimbs; b++) c.mneg8[b] = wide_byte(Mn, b);
 }
 }  // namespace oz
 
@@ -227,7 +257,7 @@ int plan_configure_crt(ozaie_plan p, int64_t M, int64_t N, int64_t K, int beta, 
     if (M < 1 || N < 1 || K < 1) { g_last_error = "dimensions must be >= 1"; return OZAIE_ERR_DIMS; }
     if (M > (1ll << 31) - 512 || N > (1ll << 31) - 512) { g_last_error = "dimensions too large"; return OZAIE_ERR_DIMS; }
     if (K > 65536) { g_last_error = "CRT mode needs K <= 65536 (exact int32 residue sums)"; return OZAIE_ERR_DIMS; }
-    if (n < 2 || n > kCrtMaxMod) { g_last_error = "CRT mode needs 2 <= moduli <= 16"; return OZAIE_ERR_ARG; }
+    if (n < 2 || n > kCrtMaxMod) { g_last_error = "CRT mode needs 2 <= moduli <= 20"; return OZAIE_ERR_ARG; }
     const int bmax = crt_beta(n, K);
     if (bmax < 1) { g_last_error = "no exact bit count for these moduli and K"; return OZAIE_ERR_ARG; }
     if (beta == 0) beta = bmax;
@@ -415,7 +445,7 @@ int32_t ozAIE_setup(int M, int N, int K, int mode) {
         case OZAIE_MODE_ACCURATE7: b = 7; d = 7; break;
         case OZAIE_MODE_CRT:
             if (g_plan) { plan_free(g_plan); g_plan = nullptr; }
-            return plan_new(&g_plan, M, N, K, 0, kCrtMaxMod, g_opt_rounding, true, true);
+            return plan_new(&g_plan, M, N, K, 0, kCrtDefaultMod, g_opt_rounding, true, true);
         default:
             g_last_error = "unknown mode";
             return OZAIE_ERR_ARG;

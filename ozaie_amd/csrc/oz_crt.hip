@@ -330,21 +330,62 @@ __device__ __forceinline__ double crt_to_double(i128 t, int e2, int rounding) {
     return neg ? -v : v;
 }
 
+// C' * 2^e2 for a signed 192-bit C' (three words, low first), truncated (rounding 0) or
+// nearest-even (1) to 53 bits.  Values below 2^127 take the 128-bit path directly; larger
+// ones are shifted right to 127 bits with the shifted-out bits OR-ed into bit 0 (a sticky
+// bit far below the rounding position, so both roundings are unchanged).
+__device__ __forceinline__ double crt_wide_to_double(uint64_t w0, uint64_t w1, uint64_t w2, int e2, int rounding) {
+    const bool neg = (int64_t)w2 < 0;
+    if (neg) {   // two's complement negation
+        w0 = ~w0; w1 = ~w1; w2 = ~w2;
+        w0 += 1;
+        const uint64_t c0 = w0 == 0;
+        w1 += c0;
+        w2 += (c0 & (w1 == 0));
+    }
+    u128 a;
+    int sh = 0;
+    if (w2 == 0 && (w1 >> 63) == 0) {
+        a = ((u128)w1 << 64) | w0;
+    } else {
+        const int nb = w2 ? 192 - __builtin_clzll(w2) : 128;
+        sh = nb - 127;   // 1 .. 65
+        bool sticky;
+        if (sh < 64) {
+            a = ((((u128)w2 << 64) | w1) << (64 - sh)) | (u128)(w0 >> sh);
+            sticky = (w0 << (64 - sh)) != 0;
+        } else if (sh == 64) {
+            a = ((u128)w2 << 64) | w1;
+            sticky = w0 != 0;
+        } else {
+            a = ((u128)w2 << 63) | (u128)(w1 >> 1);
+            sticky = w0 != 0 || (w1 & 1) != 0;
+        }
+        a |= (u128)sticky;
+    }
+    const i128 v = neg ? -(i128)a : (i128)a;
+    return rounding ? ldexp((double)v, e2 + sh) : crt_to_double(v, e2 + sh, 0);
+}
+
 // The four rows 4 row4 + e (e < rows) of one column from the n residue words wd[j], each
 // byte already flipped by 0x80: the stored signed residue r (congruent to the residue
 // product g, -128 <= r <= 127) read as the unsigned byte u = r + 128.  Since the weights
 // satisfy sum_j W_j = 1 (mod M), S = sum_j u_j W_j = C' + 128 + K M for an integer K.
 // The words are transposed so that one word holds the bytes of four moduli of one row;
-// then byte dot products give the 16 byte limbs L_i = sum_j u_j w_ji (below 2^20) of S,
-// and K = floor((t + 2^15) / 2^16) with t = sum_j u_j q_j, q_j = floor(2^16 fl(W_j / M)):
+// then byte dot products give the byte limbs L_i = sum_j u_j w_ji (below 2^22) of S, and
+// K = floor((t + 2^15) / 2^16) with t = sum_j u_j q_j, q_j = floor(2^16 fl(W_j / M)):
 // t / 2^16 is within 0.0625 of S / M, which is within 1/4 + 128 / M of K.  Then
-// C' = sum_i (L_i + K m_i) 256^i - 128 (mod 2^128), m_i the bytes of 2^128 - M, read as
-// signed.  Nearest rounding uses the correctly rounded 128-bit integer conversion (then the
-// same power-of-two scaling as the truncating path).
+// C' = sum_i (L_i + K m_i) 256^i - 128, m_i the bytes of 2^(8 NL) - M, modulo 2^128
+// (16 limbs) for n <= 16 moduli (|C'| <= M / 4 < 2^124) and 2^192 (24 limbs, WIDE) above,
+// read as signed.  Nearest rounding uses the correctly rounded 128-bit integer conversion
+// (then the same power-of-two scaling as the truncating path).
+template <bool WIDE>
 __device__ __forceinline__ void crt_rebuild_quad(const uint32_t (&wd)[kCrtMaxMod], const CrtConsts &c, int rows,
                                                  const int32_t *__restrict__ Er4, int ec, int beta, int rounding,
                                                  double *__restrict__ Cq, int64_t ldc) {
-    constexpr int NG = kCrtMaxMod / 4;
+    constexpr int NG = (WIDE ? kCrtMaxMod : kCrtNarrowMod) / 4;   // groups of four moduli
+    constexpr int NL = WIDE ? kCrtLimbs : 16;                      // byte limbs of C'
+    constexpr int NW = WIDE ? 20 : 16;                             // byte limbs of W_p (W_p < M < 2^156)
     uint32_t x[NG][4];   // x[G][e]: bytes u_{4G + b} of row e, b = 0..3
 #pragma unroll
     for (int G = 0; G < NG; G++) {
@@ -369,28 +410,43 @@ __device__ __forceinline__ void crt_rebuild_quad(const uint32_t (&wd)[kCrtMaxMod
             thi = __builtin_amdgcn_udot4(x[G][e], c.Qd[1][G], thi, false);
         }
         const uint32_t k = (tlo + (thi << 8) + 32768u) >> 16;
-        uint32_t L[16];
+        uint32_t L[NL];
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
+        for (int i = 0; i < NL; i++) {
             uint32_t acc = k * c.mneg8[i];
+            if (i < NW) {
 #pragma unroll
-            for (int G = 0; G < NG; G++) acc = __builtin_amdgcn_udot4(x[G][e], c.Wd[G][i], acc, false);
-            L[i] = acc;   // below 2^21
+                for (int G = 0; G < NG; G++) acc = __builtin_amdgcn_udot4(x[G][e], c.Wd[G][i], acc, false);
+            }
+            L[i] = acc;   // below 2^22
         }
-        uint64_t Q[4];
+        uint64_t Q[NL / 4];
 #pragma unroll
-        for (int h = 0; h < 4; h++) {
-            const uint32_t a = L[4 * h] + (L[4 * h + 1] << 8);        // below 2^30
-            const uint32_t b = L[4 * h + 2] + (L[4 * h + 3] << 8);
-            Q[h] = (uint64_t)a + ((uint64_t)b << 16);                 // below 2^47
+        for (int h = 0; h < NL / 4; h++) {
+            const uint32_t lo = L[4 * h] + (L[4 * h + 1] << 8);        // below 2^31
+            const uint32_t hi = L[4 * h + 2] + (L[4 * h + 3] << 8);
+            Q[h] = (uint64_t)lo + ((uint64_t)hi << 16);                // below 2^48
         }
-        const u128 S = (u128)Q[0] + ((u128)Q[1] << 32) + ((u128)Q[2] << 64) + ((u128)Q[3] << 96) - (u128)128;
         const int e2 = Er4[e] + ec - 2044 - 2 * beta;
-        Cq[e * ldc] = rounding ? ldexp((double)(i128)S, e2) : crt_to_double((i128)S, e2, 0);
+        if (!WIDE) {
+            const u128 S = (u128)Q[0] + ((u128)Q[1] << 32) + ((u128)Q[2] << 64) + ((u128)Q[3] << 96) - (u128)128;
+            Cq[e * ldc] = rounding ? ldexp((double)(i128)S, e2) : crt_to_double((i128)S, e2, 0);
+        } else {
+            const u128 s0 = (u128)Q[0] + ((u128)Q[1] << 32);
+            const u128 s1 = (s0 >> 64) + Q[2] + ((u128)Q[3] << 32);
+            const u128 s2 = (s1 >> 64) + Q[4] + ((u128)Q[5 % (NL / 4)] << 32);
+            uint64_t w0 = (uint64_t)s0, w1 = (uint64_t)s1, w2 = (uint64_t)s2;
+            const uint64_t b0 = w0 < 128u, b1 = b0 & (uint64_t)(w1 == 0);   // - 128 with borrows
+            w0 -= 128u;
+            w1 -= b0;
+            w2 -= b1;
+            Cq[e * ldc] = crt_wide_to_double(w0, w1, w2, e2, rounding);
+        }
     }
 }
 
 // Thread per (4 rows, 1 column) of R[j][Mp / 4][Np].
+template <bool WIDE>
 __global__ __launch_bounds__(256) void k_crt_rebuild(const uint32_t *__restrict__ R, int64_t Mp, int64_t Np,
                                                      int64_t M, int64_t N, const int32_t *__restrict__ Er,
                                                      const int32_t *__restrict__ Ec, double *__restrict__ C,
@@ -403,8 +459,8 @@ __global__ __launch_bounds__(256) void k_crt_rebuild(const uint32_t *__restrict_
     for (int j = 0; j < kCrtMaxMod; j++)
         wd[j] = j < c.n ? R[((int64_t)j * (Mp / 4) + row4) * Np + col] ^ 0x80808080u : 0u;
     const int64_t rest = M - row4 * 4;
-    crt_rebuild_quad(wd, c, rest < 4 ? (int)rest : 4, Er + row4 * 4, Ec[col], c.beta, rounding,
-                     C + row4 * 4 * ldc + col, ldc);
+    crt_rebuild_quad<WIDE>(wd, c, rest < 4 ? (int)rest : 4, Er + row4 * 4, Ec[col], c.beta, rounding,
+                           C + row4 * 4 * ldc + col, ldc);
 }
 
 hipError_t launch_crt_rebuild(const uint32_t *R, int64_t Mp, int64_t Np, int64_t M, int64_t N, const int32_t *Er,
@@ -414,8 +470,12 @@ hipError_t launch_crt_rebuild(const uint32_t *R, int64_t Mp, int64_t Np, int64_t
     const int64_t blocks = (Mp / 4) * ncb;
     if (blocks <= 0) return hipSuccess;
     if (blocks > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_crt_rebuild, dim3((unsigned)blocks), dim3(256), 0, st, R, Mp, Np, M, N, Er, Ec, C, ldc,
-                       rounding, ncb, c);
+    if (c.n > kCrtNarrowMod)
+        hipLaunchKernelGGL(k_crt_rebuild<true>, dim3((unsigned)blocks), dim3(256), 0, st, R, Mp, Np, M, N, Er, Ec, C,
+                           ldc, rounding, ncb, c);
+    else
+        hipLaunchKernelGGL(k_crt_rebuild<false>, dim3((unsigned)blocks), dim3(256), 0, st, R, Mp, Np, M, N, Er, Ec, C,
+                           ldc, rounding, ncb, c);
     return hipGetLastError();
 }
 
@@ -511,6 +571,7 @@ hipError_t launch_group_crt_gemm(const CrtGroupProb *probs, const int4 *items, i
 
 // Items {problem, first element}: thread t handles element base + t of the problem's
 // ceil(M / 4) x N (row quad, column) grid.
+template <bool WIDE>
 __global__ __launch_bounds__(256) void k_grp_crt_rebuild(const CrtGroupProb *__restrict__ probs,
                                                          const int4 *__restrict__ items, int rounding, CrtConsts c) {
     const int4 it = items[blockIdx.x];
@@ -525,14 +586,17 @@ __global__ __launch_bounds__(256) void k_grp_crt_rebuild(const CrtGroupProb *__r
 #pragma unroll
     for (int j = 0; j < kCrtMaxMod; j++) wd[j] = j < c.n ? Rq[(int64_t)j * plane] ^ 0x80808080u : 0u;
     const int rest = pr.M - 4 * (int)row4;
-    crt_rebuild_quad(wd, c, rest < 4 ? rest : 4, pr.Er + 4 * (int64_t)row4, pr.Ec[col], pr.beta, rounding,
-                     pr.C + 4 * (int64_t)row4 * pr.ldc + col, pr.ldc);
+    crt_rebuild_quad<WIDE>(wd, c, rest < 4 ? rest : 4, pr.Er + 4 * (int64_t)row4, pr.Ec[col], pr.beta, rounding,
+                           pr.C + 4 * (int64_t)row4 * pr.ldc + col, pr.ldc);
 }
 
 hipError_t launch_group_crt_rebuild(const CrtGroupProb *probs, const int4 *items, int n, int rounding,
                                     const CrtConsts &c, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_grp_crt_rebuild, dim3((unsigned)n), dim3(256), 0, st, probs, items, rounding, c);
+    if (c.n > kCrtNarrowMod)
+        hipLaunchKernelGGL(k_grp_crt_rebuild<true>, dim3((unsigned)n), dim3(256), 0, st, probs, items, rounding, c);
+    else
+        hipLaunchKernelGGL(k_grp_crt_rebuild<false>, dim3((unsigned)n), dim3(256), 0, st, probs, items, rounding, c);
     return hipGetLastError();
 }
 

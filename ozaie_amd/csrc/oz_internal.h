@@ -110,7 +110,10 @@ __host__ __device__ inline void band_map(int lin, int ntm, int ntn, int &tm, int
 
 // CRT mode (exact integer product from residue GEMMs modulo n <= 16 coprime moduli,
 // rebuilt with the Chinese remainder theorem).  Constants are computed on the host.
-constexpr int kCrtMaxMod = 16;
+constexpr int kCrtMaxMod = 20;
+constexpr int kCrtDefaultMod = 16;   // OZAIE_MODE_CRT: every product below 2^127 (128-bit rebuild)
+constexpr int kCrtNarrowMod = 16;    // up to this many moduli the rebuild runs in 128 bits, above in 192
+constexpr int kCrtLimbs = 24;        // byte limbs of the 192-bit rebuild
 constexpr int kCrtRowPad = 256;   // the modulus GEMM's workgroup tile is 256 x 256
 constexpr int kCrtPF = 8;         // contraction blocks in flight in the modulus GEMM (K pads to 32 kCrtPF)
 struct CrtConsts {
@@ -118,17 +121,13 @@ struct CrtConsts {
     double p[kCrtMaxMod];      // modulus
     double inv[kCrtMaxMod];    // 1 / p
     double c31[kCrtMaxMod];    // 2^31 mod p
-    double half[kCrtMaxMod];   // largest non-negative symmetric residue (127 for 256)
-    double f[kCrtMaxMod];      // fl(W_p / M)
-    uint64_t W[kCrtMaxMod][2]; // W_p = (M/p) ((M/p)^-1 mod p) mod M, (lo, hi)
-    uint64_t Mw[2];            // M = prod p, (lo, hi)
-    uint64_t Mneg[2];          // 2^128 - M, (lo, hi)
-    // rebuild by byte dot products: Wd[G][i] = byte i of W_{4G + b} in byte b;
-    // Qd[h][G] = byte h of q_{4G + b} = floor(2^16 f_{4G + b}) in byte b; mneg8[i] = byte i
-    // of 2^128 - M (zero beyond the n moduli)
-    uint32_t Wd[kCrtMaxMod / 4][16];
+    // rebuild by byte dot products: Wd[G][i] = byte i of W_{4G + b} in byte b, where
+    // W_p = (M/p) ((M/p)^-1 mod p) mod M; Qd[h][G] = byte h of q_{4G + b} in byte b, where
+    // q_p = floor(2^16 fl(W_p / M)); mneg8[i] = byte i of 2^192 - M (its low 16 bytes are
+    // those of 2^128 - M when M < 2^128).  Zero beyond the n moduli.
+    uint32_t Wd[kCrtMaxMod / 4][kCrtLimbs];
     uint32_t Qd[2][kCrtMaxMod / 4];
-    uint32_t mneg8[16];
+    uint32_t mneg8[kCrtLimbs];
 };
 // Modulus list, bits per operand for (n, K) (or -1), and the constants.
 int crt_modulus(int i);

@@ -69,7 +69,7 @@ def test_crt_group_replays_match_single_plans(gpu, cfg, rounding):
     grp.close()
 
 
-@pytest.mark.parametrize("n", [16, 9, 3])
+@pytest.mark.parametrize("n", [16, 9, 3, 17, 20])
 def test_crt_group_vs_oracle_small(gpu, n):
     import torch
     rng = np.random.default_rng(40 + n)
@@ -107,7 +107,7 @@ def test_crt_group_limits(gpu):
     with pytest.raises(oz.OzaieError):
         oz.Group([(64, 64, 70000)], 0, 16, crt=True)                 # beyond exact int32 residue sums
     with pytest.raises(oz.OzaieError):
-        oz.Group([(64, 64, 64)], 0, 17, crt=True)                    # more moduli than the table
+        oz.Group([(64, 64, 64)], 0, 21, crt=True)                    # more moduli than the table
     with pytest.raises(oz.OzaieError):
         oz.Group([(64, 64, 64), (64, 64, 4096)], oz.crt_beta(16, 64), 16, crt=True)   # bits too many for K=4096
     with pytest.raises(oz.OzaieError):

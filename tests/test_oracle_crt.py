@@ -34,12 +34,13 @@ def _sym(q, p):
 
 
 def test_moduli_and_constants():
-    ps = oracle.crt_moduli(16)
-    assert len(set(ps)) == 16 and all(2 <= p <= 256 for p in ps)
-    for i in range(16):
+    ps = oracle.crt_moduli(20)
+    assert len(set(ps)) == 20 and all(2 <= p <= 256 for p in ps)
+    for i in range(20):
         for j in range(i):
             assert math.gcd(ps[i], ps[j]) == 1
-    for n in (2, 5, 16):
+    assert oracle.crt_moduli(21)[20] == 0
+    for n in (2, 5, 16, 17, 20):
         W, f, M = oracle.crt_consts(n)
         assert M == math.prod(ps[:n])
         for i, w in enumerate(W):
@@ -49,7 +50,7 @@ def test_moduli_and_constants():
             assert abs(f[i] - w / M) <= 4 * 2.0 ** -53
 
 
-@pytest.mark.parametrize("n", [2, 3, 8, 12, 16])
+@pytest.mark.parametrize("n", [2, 3, 8, 12, 16, 17, 20])
 def test_beta_is_the_largest_safe_bit_count(n):
     M = math.prod(oracle.crt_moduli(n))
     L = M.bit_length() - 1
@@ -105,7 +106,7 @@ def test_split_residues_and_products():
         assert np.array_equal(G[i].astype(np.int64), ref)
 
 
-@pytest.mark.parametrize("n", [2, 7, 16])
+@pytest.mark.parametrize("n", [2, 7, 16, 17, 20])
 def test_rebuild_is_the_crt_inverse(n):
     ps = oracle.crt_moduli(n)
     M = math.prod(ps)
@@ -132,7 +133,7 @@ def _exact_C(A, B, n, rounding):
 
 
 @pytest.mark.parametrize("n,phi,shape", [(16, 1.0, (6, 50, 5)), (16, 4.0, (5, 129, 4)), (9, 0.5, (4, 64, 6)),
-                                         (3, 1.0, (3, 8, 3))])
+                                         (3, 1.0, (3, 8, 3)), (17, 2.0, (4, 70, 5)), (20, 1.0, (3, 40, 4))])
 def test_crt_dgemm_equals_exact_integer_product(n, phi, shape):
     M_, K, N_ = shape
     rng = np.random.default_rng(int(phi * 10) + n)
@@ -146,8 +147,8 @@ def test_crt_dgemm_equals_exact_integer_product(n, phi, shape):
 
 def test_conversion_rounding_closed_form():
     rnd = random.Random(5)
-    for _ in range(300):
-        v = rnd.randrange(-2 ** 126, 2 ** 126)
+    for _ in range(600):
+        v = rnd.randrange(-2 ** 126, 2 ** 126) if _ % 2 else rnd.randrange(-2 ** 190, 2 ** 190)
         e2 = rnd.randrange(-1100, -900)
         for rounding in (0, 1):
             got = oracle.crt_to_double(v, e2, rounding)
@@ -182,7 +183,7 @@ def test_crt_error_bound_vs_exact_product():
             assert abs(Fraction(float(C[r, c])) - exact) <= bound
 
 
-@pytest.mark.parametrize("n", [2, 9, 16])
+@pytest.mark.parametrize("n", [2, 9, 16, 17, 20])
 def test_rebuild_from_offset_symmetric_residues(n):
     """The GPU stores each residue product as a signed byte r (r = g mod p, -128 <= r <= 127,
     128 kept as -128 for p = 256) and rebuilds from u = r + 128: because sum_j W_j = 1 (mod M),
@@ -205,22 +206,25 @@ def test_rebuild_from_offset_symmetric_residues(n):
         for uj, fj in zip(u, f):
             t += float(uj) * fj
         k = round(t)
-        S = (sum(uj * wj for uj, wj in zip(u, W)) - k * M - 128) % 2 ** 128
-        assert (S - 2 ** 128 if S >= 2 ** 127 else S) == v
+        R = 2 ** (128 if n <= 16 else 192)
+        S = (sum(uj * wj for uj, wj in zip(u, W)) - k * M - 128) % R
+        assert (S - R if S >= R // 2 else S) == v
 
 
-@pytest.mark.parametrize("n", [2, 5, 9, 16])
+@pytest.mark.parametrize("n", [2, 5, 9, 16, 17, 20])
 def test_rebuild_byte_limbs_and_fixed_point_quotient(n):
     """The GPU rebuild's arithmetic against the exact CRT inverse: u = r + 128 per modulus,
-    L_i = sum_j u_j w_ji over the 16 byte limbs w_ji of W_j, the quotient
+    L_i = sum_j u_j w_ji over the byte limbs w_ji of W_j, the quotient
     k = floor((sum_j u_j q_j + 2^15) / 2^16) with q_j = floor(2^16 fl(W_j / M)) (< 2^16), and
-    C' = (sum_i (L_i + k m_i) 256^i - 128) mod 2^128 with m_i the bytes of 2^128 - M."""
+    C' = (sum_i (L_i + k m_i) 256^i - 128) mod 2^(8 nl) with m_i the bytes of 2^(8 nl) - M:
+    nl = 16 limbs (128 bits) for n <= 16 moduli, 24 (192 bits) above."""
     ps = oracle.crt_moduli(n)
     W, f, M = oracle.crt_consts(n)
-    wb = [[(w >> (8 * i)) & 0xFF for i in range(16)] for w in W]
+    nl = 16 if n <= 16 else 24
+    wb = [[(w >> (8 * i)) & 0xFF for i in range(nl)] for w in W]
     q = [int(math.floor(fj * 65536.0)) for fj in f]
     assert all(0 <= qj < 65536 for qj in q)
-    mneg = [((2 ** 128 - M) >> (8 * i)) & 0xFF for i in range(16)]
+    mneg = [((2 ** (8 * nl) - M) >> (8 * i)) & 0xFF for i in range(nl)]
     rnd = random.Random(200 + n)
     vals = [0, 1, -1, M // 4, -(M // 4), 127, -128] + [rnd.randrange(-(M // 4), M // 4 + 1) for _ in range(400)]
     for v in vals:
@@ -230,9 +234,10 @@ def test_rebuild_byte_limbs_and_fixed_point_quotient(n):
             r = r - p if r > p // 2 else r
             u.append((((r & 0xFF) ^ 0x80) - 128) + 128)
         t = sum(uj * qj for uj, qj in zip(u, q))
-        assert t < 2 ** 28
+        assert t < 2 ** 29
         k = (t + 32768) >> 16
-        L = [sum(u[j] * wb[j][i] for j in range(n)) + k * mneg[i] for i in range(16)]
-        assert max(L) < 2 ** 21
-        S = (sum(Li << (8 * i) for i, Li in enumerate(L)) - 128) % 2 ** 128
-        assert (S - 2 ** 128 if S >= 2 ** 127 else S) == v
+        L = [sum(u[j] * wb[j][i] for j in range(n)) + k * mneg[i] for i in range(nl)]
+        assert max(L) < 2 ** 22
+        R = 2 ** (8 * nl)
+        S = (sum(Li << (8 * i) for i, Li in enumerate(L)) - 128) % R
+        assert (S - R if S >= R // 2 else S) == v
