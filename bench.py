@@ -211,15 +211,21 @@ def load_traffic(d, n):
 def cpu_baseline(args, np, Ah, Bh, d, n, crt=False):
     import oracle
     run = (lambda A_: oracle.crt_dgemm(A_, Bh, d, 1)) if crt else (lambda A_: oracle.ozaki_dgemm(A_, Bh, 7, d))
-    # bounded sample: the first rows of C (the whole of B is split, as the GPU does); a
-    # 64-row probe sizes the sample to about OZAIE_CPU_SECONDS (default 10) of CPU work
-    target = float(os.environ.get("OZAIE_CPU_SECONDS", "10"))
+    # bounded sample: the first rows of C (the whole of B is split, as the GPU does); probes
+    # of 64 and 128 rows separate the fixed cost (B's split) from the per-row cost and size
+    # the sample to about OZAIE_CPU_SECONDS (default 12) of CPU work
+    target = float(os.environ.get("OZAIE_CPU_SECONDS", "12"))
     ms = max(8, min(n, int(os.environ.get("OZAIE_CPU_ROWS", "64"))))
     t0 = time.time()
     run(np.ascontiguousarray(Ah[:ms]))
     t = time.time() - t0
-    if "OZAIE_CPU_ROWS" not in os.environ and t < target:
-        ms = max(ms, min(n, int(ms * target / max(t, 1e-3))))
+    if "OZAIE_CPU_ROWS" not in os.environ and t < target and 2 * ms <= n:
+        t0 = time.time()
+        run(np.ascontiguousarray(Ah[:2 * ms]))
+        t2 = time.time() - t0
+        per_row = max((t2 - t) / ms, t2 / (2 * ms) * 0.05, 1e-7)
+        fixed = max(t - per_row * ms, 0.0)
+        ms = max(2 * ms, min(n, int((target - fixed) / per_row)))
         A_s = np.ascontiguousarray(Ah[:ms])
         t0 = time.time()
         run(A_s)
