@@ -399,9 +399,12 @@ int group_build_crt(ozaie_group g, int force_cfg) {
         for (int64_t cx = 0; cx < (g->N[p] + 255) / 256; cx++)
             for (int64_t ky = 0; ky < (g->K[p] + kper - 1) / kper; ky++)
                 imax.push_back(int4{2 * p + 1, (int)cx, (int)ky, (int)kper});
+        // residues of the row blocks that hold real rows only: the blocks padding a
+        // problem to its tile shape feed only outputs at or beyond M rows / N columns,
+        // which the modulus GEMM does not store and the rebuild does not read
         for (int op = 0; op < 2; op++) {
             const GroupOperand &o = g->h_ops[2 * p + op];
-            for (int64_t rb = 0; rb < o.Rb; rb++)
+            for (int64_t rb = 0; rb < (o.R + kBlk - 1) / kBlk; rb++)
                 for (int64_t kt = 0; kt < (o.Kb + 3) / 4; kt++)
                     isplit.push_back(int4{2 * p + op, (int)rb, (int)kt, 0});
         }
