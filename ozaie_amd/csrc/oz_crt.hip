@@ -187,10 +187,13 @@ struct CrtTileLoop {
 };
 
 // Residues of the wave's (32 WM) x (32 WN) sums, four rows of one column per word of
-// Rj[row / 4][col] (row stride ldr words), one signed byte each: r = x - p rint(x / p),
-// congruent to x, |r| <= p / 2 (|x| <= 2^30, so x / p in FP64 is never within 1 / (2p) of
-// a half-integer for odd p, and is exact for p = 256).  CLIP: words whose four rows all
-// lie at or beyond M, or whose column lies at or beyond N, are not written.
+// Rj[row / 4][col] (row stride ldr words), one signed byte each: r = x - p q with q the
+// integer nearest x / p, congruent to x, |r| <= p / 2.  q comes from fma(x, 1/p, 1.5 2^52),
+// which rounds the exact product to an integer (|x| <= 2^30, so x fl(1/p) is never within
+// 1 / (2p) of a half-integer for odd p, and is exact for p = 256: ties to even, as rint);
+// r + 1.5 2^52 holds r in two's complement in its low word, whose low bytes are packed with
+// v_perm_b32.  CLIP: words whose four rows all lie at or beyond M, or whose column lies at
+// or beyond N, are not written.
 template <int WM, int WN, bool CLIP>
 __device__ __forceinline__ void crt_tile_store(const v16i (&acc)[WM][WN], double p, double inv, uint32_t *Rj,
                                                int64_t ldr, int64_t tm, int64_t tn, int64_t M, int64_t N) {
@@ -204,13 +207,18 @@ __device__ __forceinline__ void crt_tile_store(const v16i (&acc)[WM][WN], double
         for (int ni = 0; ni < WN; ni++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                uint32_t word = 0;
+                constexpr double kMagic = 6755399441055744.0;   // 1.5 2^52
+                uint32_t lo[4];
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
                     const double x = (double)acc[mi][ni][4 * q + e];
-                    const double r = __builtin_fma(-rint(x * inv), p, x);
-                    word |= ((uint32_t)(int)r & 0xFFu) << (8 * e);
+                    const double qv = __builtin_fma(x, inv, kMagic) - kMagic;
+                    const double rm = __builtin_fma(-qv, p, x) + kMagic;
+                    lo[e] = (uint32_t)__double_as_longlong(rm);
                 }
+                const uint32_t word = __builtin_amdgcn_perm(__builtin_amdgcn_perm(lo[3], lo[2], 0x0c0c0400u),
+                                                            __builtin_amdgcn_perm(lo[1], lo[0], 0x0c0c0400u),
+                                                            0x05040100u);
                 const int64_t row4 = (rbase + mi * 32 + q * 8 + (lane >> 5) * 4) >> 2;
                 const int64_t col = cbase + ni * 32 + (lane & 31);
                 if (!CLIP || (row4 * 4 < M && col < N)) Rj[row4 * ldr + col] = word;
