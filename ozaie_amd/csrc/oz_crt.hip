@@ -13,9 +13,11 @@ namespace oz {
 // Residues of the staged tile: thread (row, q) owns 16 consecutive k of one row and
 // writes one 16-byte lane chunk per modulus plane.  With q = +-(hi 2^31 + lo), the value
 // x = +-(hi (2^31 mod p) + lo) is an exact double (|x| < 2^40) congruent to q, and
-// x - p rint(x / p) is its residue in [-(p-1)/2, (p-1)/2] for odd p: x / p is never within
-// 1 / (2p) of a half-integer, far above the rounding error of x * fl(1/p).  For p = 256
-// (plane 0) the residue is the low byte of q in two's complement.
+// x - p n, n the integer nearest x / p, is its residue in [-(p-1)/2, (p-1)/2] for odd p:
+// x / p is never within 1 / (2p) of a half-integer, far above the error of the product
+// x fl(1/p), which fma(x, fl(1/p), 1.5 2^52) rounds to an integer; the residue plus
+// 1.5 2^52 holds it in two's complement in its low word.  For p = 256 (plane 0) the residue
+// is the low byte of q in two's complement.
 template <bool KC>
 __device__ __forceinline__ void crt_emit(const double *tile, int64_t R, const int32_t *__restrict__ Erow,
                                          const CrtConsts &c, int beta, int8_t *__restrict__ S, int64_t Rb,
@@ -50,18 +52,20 @@ __device__ __forceinline__ void crt_emit(const double *tile, int64_t R, const in
     *reinterpret_cast<uint4 *>(dst) = make_uint4(low8[0], low8[1], low8[2], low8[3]);   // p = 256
     for (int j = 1; j < c.n; j++) {
         const double p = c.p[j], inv = c.inv[j], c31 = c.c31[j];
+        constexpr double kMagic = 6755399441055744.0;   // 1.5 2^52: rounds to an integer
         uint32_t w[4];
 #pragma unroll
         for (int gi = 0; gi < 4; gi++) {
-            uint32_t acc = 0;
+            uint32_t b[4];
 #pragma unroll
             for (int h = 0; h < 4; h++) {
                 const int i = 4 * gi + h;
                 const double x = __builtin_fma(hi[i], c31, lo[i]);
-                const double r = __builtin_fma(-rint(x * inv), p, x);
-                acc |= ((uint32_t)(int)r & 0xFFu) << (8 * h);
+                const double qv = __builtin_fma(x, inv, kMagic) - kMagic;
+                b[h] = (uint32_t)__double_as_longlong(__builtin_fma(-qv, p, x) + kMagic);   // residue, low word
             }
-            w[gi] = acc;
+            w[gi] = __builtin_amdgcn_perm(__builtin_amdgcn_perm(b[3], b[2], 0x0c0c0400u),
+                                          __builtin_amdgcn_perm(b[1], b[0], 0x0c0c0400u), 0x05040100u);
         }
         *reinterpret_cast<uint4 *>(dst + (int64_t)j * plane) = make_uint4(w[0], w[1], w[2], w[3]);
     }
