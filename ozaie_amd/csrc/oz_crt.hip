@@ -425,7 +425,7 @@ __device__ __forceinline__ void crt_rebuild_quad(const uint32_t (&wd)[kCrtMaxMod
         uint32_t L[NL];
 #pragma unroll
         for (int i = 0; i < NL; i++) {
-            uint32_t acc = k * c.mneg8[i];
+            uint32_t acc = __umul24(k, c.mneg8[i]);   // k < 2^13, byte: full-rate 24-bit product
             if (i < NW) {
 #pragma unroll
                 for (int G = 0; G < NG; G++) acc = __builtin_amdgcn_udot4(x[G][e], c.Wd[G][i], acc, false);
@@ -457,31 +457,44 @@ __device__ __forceinline__ void crt_rebuild_quad(const uint32_t (&wd)[kCrtMaxMod
     }
 }
 
-// Thread per (4 rows, 1 column) of R[j][Mp / 4][Np].
+// Thread per (4 rows, 4 consecutive columns) of R[j][Mp / 4][Np]: one 16-byte load per
+// modulus plane (Np is a multiple of 256, so the four words are in the row).
 template <bool WIDE>
 __global__ __launch_bounds__(256) void k_crt_rebuild(const uint32_t *__restrict__ R, int64_t Mp, int64_t Np,
                                                      int64_t M, int64_t N, const int32_t *__restrict__ Er,
                                                      const int32_t *__restrict__ Ec, double *__restrict__ C,
                                                      int64_t ldc, int rounding, int64_t ncb, CrtConsts c) {
     const int64_t row4 = blockIdx.x / ncb;
-    const int64_t col = (blockIdx.x % ncb) * 256 + threadIdx.x;
-    if (col >= N) return;
-    uint32_t wd[kCrtMaxMod];
+    const int64_t col0 = (blockIdx.x % ncb) * 1024 + 4 * (int64_t)threadIdx.x;
+    if (col0 >= N) return;
+    uint4 w4[kCrtMaxMod];
 #pragma unroll
     for (int j = 0; j < kCrtMaxMod; j++)
-        wd[j] = j < c.n ? R[((int64_t)j * (Mp / 4) + row4) * Np + col] ^ 0x80808080u : 0u;
+        w4[j] = j < c.n ? *reinterpret_cast<const uint4 *>(R + ((int64_t)j * (Mp / 4) + row4) * Np + col0)
+                        : make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u);
     const int64_t rest = M - row4 * 4;
-    crt_rebuild_quad<WIDE>(wd, c, rest < 4 ? (int)rest : 4, Er + row4 * 4, Ec[col], c.beta, rounding,
-                           C + row4 * 4 * ldc + col, ldc);
+    const int rows = rest < 4 ? (int)rest : 4;
+#pragma unroll
+    for (int cc = 0; cc < 4; cc++) {
+        if (col0 + cc >= N) break;
+        uint32_t wd[kCrtMaxMod];
+#pragma unroll
+        for (int j = 0; j < kCrtMaxMod; j++) {
+            const uint32_t v = cc == 0 ? w4[j].x : (cc == 1 ? w4[j].y : (cc == 2 ? w4[j].z : w4[j].w));
+            wd[j] = v ^ 0x80808080u;   // planes j >= n: zero words
+        }
+        crt_rebuild_quad<WIDE>(wd, c, rows, Er + row4 * 4, Ec[col0 + cc], c.beta, rounding,
+                               C + row4 * 4 * ldc + col0 + cc, ldc);
+    }
 }
 
 hipError_t launch_crt_rebuild(const uint32_t *R, int64_t Mp, int64_t Np, int64_t M, int64_t N, const int32_t *Er,
                               const int32_t *Ec, double *C, int64_t ldc, int rounding, const CrtConsts &c,
                               hipStream_t st) {
-    const int64_t ncb = (N + 255) / 256;
-    const int64_t blocks = (Mp / 4) * ncb;
+    const int64_t ncb = (N + 1023) / 1024;
+    const int64_t blocks = ((M + 3) / 4) * ncb;
     if (blocks <= 0) return hipSuccess;
-    if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+    if (blocks > 0x7fffffff || Np % 4) return hipErrorInvalidValue;
     if (c.n > kCrtNarrowMod)
         hipLaunchKernelGGL(k_crt_rebuild<true>, dim3((unsigned)blocks), dim3(256), 0, st, R, Mp, Np, M, N, Er, Ec, C,
                            ldc, rounding, ncb, c);
