@@ -594,25 +594,40 @@ hipError_t launch_group_crt_gemm(const CrtGroupProb *probs, const int4 *items, i
     return hipGetLastError();
 }
 
-// Items {problem, first element}: thread t handles element base + t of the problem's
-// ceil(M / 4) x N (row quad, column) grid.
+// Items {problem, first group}: thread t handles group base + t of the problem's
+// ceil(M / 4) x ceil(N / 4) grid of (row quad, four consecutive columns), one 16-byte load
+// per modulus plane (the padded row stride Np is a multiple of 64 words).
 template <bool WIDE>
 __global__ __launch_bounds__(256) void k_grp_crt_rebuild(const CrtGroupProb *__restrict__ probs,
                                                          const int4 *__restrict__ items, int rounding, CrtConsts c) {
     const int4 it = items[blockIdx.x];
     const CrtGroupProb &pr = probs[it.x];
-    const uint32_t N = (uint32_t)pr.N;
+    const uint32_t nc4 = (uint32_t)(pr.N + 3) / 4;
     const uint32_t idx = (uint32_t)it.y + threadIdx.x;
-    if (idx >= (uint32_t)((pr.M + 3) / 4) * N) return;
-    const uint32_t row4 = idx / N, col = idx - row4 * N;
+    if (idx >= (uint32_t)((pr.M + 3) / 4) * nc4) return;
+    const uint32_t row4 = idx / nc4;
+    const int col0 = 4 * (int)(idx - row4 * nc4);
     const int64_t plane = pr.RbA * (kBlk / 4) * pr.Np;
-    const uint32_t *Rq = pr.R + (int64_t)row4 * pr.Np + col;
-    uint32_t wd[kCrtMaxMod];
+    const uint32_t *Rq = pr.R + (int64_t)row4 * pr.Np + col0;
+    uint4 w4[kCrtMaxMod];
 #pragma unroll
-    for (int j = 0; j < kCrtMaxMod; j++) wd[j] = j < c.n ? Rq[(int64_t)j * plane] ^ 0x80808080u : 0u;
+    for (int j = 0; j < kCrtMaxMod; j++)
+        w4[j] = j < c.n ? *reinterpret_cast<const uint4 *>(Rq + (int64_t)j * plane)
+                        : make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u);
     const int rest = pr.M - 4 * (int)row4;
-    crt_rebuild_quad<WIDE>(wd, c, rest < 4 ? rest : 4, pr.Er + 4 * (int64_t)row4, pr.Ec[col], pr.beta, rounding,
-                           pr.C + 4 * (int64_t)row4 * pr.ldc + col, pr.ldc);
+    const int rows = rest < 4 ? rest : 4;
+#pragma unroll
+    for (int cc = 0; cc < 4; cc++) {
+        if (col0 + cc >= pr.N) break;
+        uint32_t wd[kCrtMaxMod];
+#pragma unroll
+        for (int j = 0; j < kCrtMaxMod; j++) {
+            const uint32_t v = cc == 0 ? w4[j].x : (cc == 1 ? w4[j].y : (cc == 2 ? w4[j].z : w4[j].w));
+            wd[j] = v ^ 0x80808080u;   // planes j >= n: zero words
+        }
+        crt_rebuild_quad<WIDE>(wd, c, rows, pr.Er + 4 * (int64_t)row4, pr.Ec[col0 + cc], pr.beta, rounding,
+                               pr.C + 4 * (int64_t)row4 * pr.ldc + col0 + cc, pr.ldc);
+    }
 }
 
 hipError_t launch_group_crt_rebuild(const CrtGroupProb *probs, const int4 *items, int n, int rounding,

@@ -420,8 +420,8 @@ int group_build_crt(ozaie_group g, int force_cfg) {
             band_map(t, ntm, ntn, tm, tn);
             igemm[c].push_back(int4{p, tm, tn, 0});
         }
-        const int64_t quads = (g->M[p] + 3) / 4 * g->N[p];
-        for (int64_t base = 0; base < quads; base += 256) irebuild.push_back(int4{p, (int)base, 0, 0});
+        const int64_t groups = (g->M[p] + 3) / 4 * ((g->N[p] + 3) / 4);   // (row quad, 4 columns)
+        for (int64_t base = 0; base < groups; base += 256) irebuild.push_back(int4{p, (int)base, 0, 0});
     }
     // Moduli per GEMM workgroup: all of them (the next modulus's loads overlap each
     // epilogue) unless that leaves fewer than 8 workgroups per CU to balance, then fewer.

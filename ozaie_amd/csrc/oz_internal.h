@@ -167,8 +167,8 @@ int crt_group_cfg_wm(int cfg);   // row blocks of 64 per tile
 int crt_group_cfg_wn(int cfg);   // column blocks of 64 per tile
 // Items: int4 {operand, row block, contraction tile, 0} (split; the problem of operand o
 // is o / 2), {problem, tile row, tile col, 0} (GEMM: workgroup x runs item x for moduli
-// [mpw y, mpw y + mpw); problem -1 = empty slot) and {problem, first (row quad, column)
-// element, 0, 0} (rebuild, 256 elements per item).
+// [mpw y, mpw y + mpw); problem -1 = empty slot) and {problem, first (row quad, four
+// columns) group, 0, 0} (rebuild, 256 groups per item).
 hipError_t launch_group_crt_split(const GroupOperand *ops, const CrtGroupProb *probs, const int4 *items, int n,
                                   const CrtConsts &c, hipStream_t st);
 hipError_t launch_group_crt_gemm(const CrtGroupProb *probs, const int4 *items, int n, int cfg, int mpw,
