@@ -318,27 +318,22 @@ hipError_t launch_crt_gemm(const CrtGemm &g, const CrtConsts &c, hipStream_t st)
     return hipGetLastError();
 }
 
-// C' * 2^e2 to double, truncated (rounding 0) or nearest-even (1) to 53 bits.
+// C' * 2^e2 to double, truncated (rounding 0) or nearest-even (1) to 53 bits, then scaled
+// by ldexp (as the oracle: one rounding to 53 bits, then the scaling).  |C'| is normalised
+// so its top bit is bit 127; the head m (64 bits) carries the value, the tail only a sticky
+// bit.  Nearest: the sticky bit OR-ed into bit 0 of m, far below the rounding position, and
+// fma(m_hi32, 2^32, m_lo32) rounds the exact m once.  Truncation: the low 11 bits of m
+// cleared, leaving 53 bits, which the same fma converts exactly.
 __device__ __forceinline__ double crt_to_double(i128 t, int e2, int rounding) {
-    if (t == 0) return 0.0;
     const bool neg = t < 0;
-    const u128 a = neg ? (u128)(-t) : (u128)t;
-    const int nb = 128 - clz128(a);
-    uint64_t m;
-    int sh = 0;
-    if (nb <= 53) {
-        m = (uint64_t)a;
-    } else {
-        sh = nb - 53;
-        u128 qv = a >> sh;
-        if (rounding) {
-            const u128 rem = a - (qv << sh);
-            const u128 hf = (u128)1 << (sh - 1);
-            if (rem > hf || (rem == hf && (qv & 1))) qv += 1;
-        }
-        m = (uint64_t)qv;
-    }
-    const double v = ldexp((double)m, sh + e2);
+    const u128 a = neg ? (u128)0 - (u128)t : (u128)t;
+    const uint64_t hi = (uint64_t)(a >> 64), lo = (uint64_t)a;
+    const int lz = hi ? __builtin_clzll(hi) : (lo ? 64 + __builtin_clzll(lo) : 127);
+    const u128 sa = a << lz;
+    const uint64_t m = (uint64_t)(sa >> 64), rest = (uint64_t)sa;
+    const uint64_t ms = rounding ? (m | (uint64_t)(rest != 0)) : (m & ~(uint64_t)0x7FF);
+    const double d = __builtin_fma((double)(uint32_t)(ms >> 32), 4294967296.0, (double)(uint32_t)ms);
+    const double v = ldexp(d, 64 - lz + e2);
     return neg ? -v : v;
 }
 
@@ -376,7 +371,7 @@ __device__ __forceinline__ double crt_wide_to_double(uint64_t w0, uint64_t w1, u
         a |= (u128)sticky;
     }
     const i128 v = neg ? -(i128)a : (i128)a;
-    return rounding ? ldexp((double)v, e2 + sh) : crt_to_double(v, e2 + sh, 0);
+    return crt_to_double(v, e2 + sh, rounding);
 }
 
 // The four rows 4 row4 + e (e < rows) of one column from the n residue words wd[j], each
@@ -412,37 +407,45 @@ __device__ __forceinline__ void crt_rebuild_quad(const uint32_t (&wd)[kCrtMaxMod
         x[G][2] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
         x[G][3] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
     }
+    uint32_t k[4], L[4][NL];
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-        if (e >= rows) break;
         uint32_t tlo = 0, thi = 0;
 #pragma unroll
         for (int G = 0; G < NG; G++) {
             tlo = __builtin_amdgcn_udot4(x[G][e], c.Qd[0][G], tlo, false);
             thi = __builtin_amdgcn_udot4(x[G][e], c.Qd[1][G], thi, false);
         }
-        const uint32_t k = (tlo + (thi << 8) + 32768u) >> 16;
-        uint32_t L[NL];
+        k[e] = (tlo + (thi << 8) + 32768u) >> 16;
+    }
+    // limb-outer order: each constant serves the four rows in turn
 #pragma unroll
-        for (int i = 0; i < NL; i++) {
-            uint32_t acc = __umul24(k, c.mneg8[i]);   // k < 2^13, byte: full-rate 24-bit product
-            if (i < NW) {
+    for (int i = 0; i < NL; i++) {
 #pragma unroll
-                for (int G = 0; G < NG; G++) acc = __builtin_amdgcn_udot4(x[G][e], c.Wd[G][i], acc, false);
+        for (int e = 0; e < 4; e++) L[e][i] = __umul24(k[e], c.mneg8[i]);   // k < 2^13, byte
+        if (i < NW) {
+#pragma unroll
+            for (int G = 0; G < NG; G++) {
+                const uint32_t wdi = c.Wd[G][i];
+#pragma unroll
+                for (int e = 0; e < 4; e++) L[e][i] = __builtin_amdgcn_udot4(x[G][e], wdi, L[e][i], false);
             }
-            L[i] = acc;   // below 2^22
         }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        if (e >= rows) break;
         uint64_t Q[NL / 4];
 #pragma unroll
         for (int h = 0; h < NL / 4; h++) {
-            const uint32_t lo = L[4 * h] + (L[4 * h + 1] << 8);        // below 2^31
-            const uint32_t hi = L[4 * h + 2] + (L[4 * h + 3] << 8);
-            Q[h] = (uint64_t)lo + ((uint64_t)hi << 16);                // below 2^48
+            const uint32_t lo = L[e][4 * h] + (L[e][4 * h + 1] << 8);        // below 2^31
+            const uint32_t hi = L[e][4 * h + 2] + (L[e][4 * h + 3] << 8);
+            Q[h] = (uint64_t)lo + ((uint64_t)hi << 16);                      // below 2^48
         }
         const int e2 = Er4[e] + ec - 2044 - 2 * beta;
         if (!WIDE) {
             const u128 S = (u128)Q[0] + ((u128)Q[1] << 32) + ((u128)Q[2] << 64) + ((u128)Q[3] << 96) - (u128)128;
-            Cq[e * ldc] = rounding ? ldexp((double)(i128)S, e2) : crt_to_double((i128)S, e2, 0);
+            Cq[e * ldc] = crt_to_double((i128)S, e2, rounding);
         } else {
             const u128 s0 = (u128)Q[0] + ((u128)Q[1] << 32);
             const u128 s1 = (s0 >> 64) + Q[2] + ((u128)Q[3] << 32);
