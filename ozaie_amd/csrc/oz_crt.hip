@@ -460,23 +460,19 @@ __device__ __forceinline__ void crt_rebuild_quad(const uint32_t (&wd)[kCrtMaxMod
     }
 }
 
-// Thread per (4 rows, 4 consecutive columns) of R[j][Mp / 4][Np]: one 16-byte load per
-// modulus plane (Np is a multiple of 256, so the four words are in the row).
+// Four rows 4 row4 + e (e < rows) of the four consecutive columns col0 + cc (col0 + cc < N):
+// one 16-byte load per modulus plane (planes `plane` words apart), then crt_rebuild_quad
+// per column.  Row quad row4 of the planes starts at Rq; Er4 / Cq at the quad's first row.
 template <bool WIDE>
-__global__ __launch_bounds__(256) void k_crt_rebuild(const uint32_t *__restrict__ R, int64_t Mp, int64_t Np,
-                                                     int64_t M, int64_t N, const int32_t *__restrict__ Er,
-                                                     const int32_t *__restrict__ Ec, double *__restrict__ C,
-                                                     int64_t ldc, int rounding, int64_t ncb, CrtConsts c) {
-    const int64_t row4 = blockIdx.x / ncb;
-    const int64_t col0 = (blockIdx.x % ncb) * 1024 + 4 * (int64_t)threadIdx.x;
-    if (col0 >= N) return;
+__device__ __forceinline__ void crt_rebuild_cols(const uint32_t *__restrict__ Rq, int64_t plane, int64_t col0,
+                                                 int64_t N, int rows, const CrtConsts &c,
+                                                 const int32_t *__restrict__ Er4, const int32_t *__restrict__ Ec,
+                                                 int beta, int rounding, double *__restrict__ Cq, int64_t ldc) {
     uint4 w4[kCrtMaxMod];
 #pragma unroll
     for (int j = 0; j < kCrtMaxMod; j++)
-        w4[j] = j < c.n ? *reinterpret_cast<const uint4 *>(R + ((int64_t)j * (Mp / 4) + row4) * Np + col0)
+        w4[j] = j < c.n ? *reinterpret_cast<const uint4 *>(Rq + (int64_t)j * plane + col0)
                         : make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u);
-    const int64_t rest = M - row4 * 4;
-    const int rows = rest < 4 ? (int)rest : 4;
 #pragma unroll
     for (int cc = 0; cc < 4; cc++) {
         if (col0 + cc >= N) break;
@@ -486,9 +482,23 @@ __global__ __launch_bounds__(256) void k_crt_rebuild(const uint32_t *__restrict_
             const uint32_t v = cc == 0 ? w4[j].x : (cc == 1 ? w4[j].y : (cc == 2 ? w4[j].z : w4[j].w));
             wd[j] = v ^ 0x80808080u;   // planes j >= n: zero words
         }
-        crt_rebuild_quad<WIDE>(wd, c, rows, Er + row4 * 4, Ec[col0 + cc], c.beta, rounding,
-                               C + row4 * 4 * ldc + col0 + cc, ldc);
+        crt_rebuild_quad<WIDE>(wd, c, rows, Er4, Ec[col0 + cc], beta, rounding, Cq + col0 + cc, ldc);
     }
+}
+
+// Thread per (4 rows, 4 consecutive columns) of R[j][Mp / 4][Np] (Np is a multiple of 256,
+// so the four words are in the row).
+template <bool WIDE>
+__global__ __launch_bounds__(256) void k_crt_rebuild(const uint32_t *__restrict__ R, int64_t Mp, int64_t Np,
+                                                     int64_t M, int64_t N, const int32_t *__restrict__ Er,
+                                                     const int32_t *__restrict__ Ec, double *__restrict__ C,
+                                                     int64_t ldc, int rounding, int64_t ncb, CrtConsts c) {
+    const int64_t row4 = blockIdx.x / ncb;
+    const int64_t col0 = (blockIdx.x % ncb) * 1024 + 4 * (int64_t)threadIdx.x;
+    if (col0 >= N) return;
+    const int64_t rest = M - row4 * 4;
+    crt_rebuild_cols<WIDE>(R + row4 * Np, (Mp / 4) * Np, col0, N, rest < 4 ? (int)rest : 4, c, Er + row4 * 4, Ec,
+                           c.beta, rounding, C + row4 * 4 * ldc, ldc);
 }
 
 hipError_t launch_crt_rebuild(const uint32_t *R, int64_t Mp, int64_t Np, int64_t M, int64_t N, const int32_t *Er,
@@ -598,8 +608,8 @@ hipError_t launch_group_crt_gemm(const CrtGroupProb *probs, const int4 *items, i
 }
 
 // Items {problem, first group}: thread t handles group base + t of the problem's
-// ceil(M / 4) x ceil(N / 4) grid of (row quad, four consecutive columns), one 16-byte load
-// per modulus plane (the padded row stride Np is a multiple of 64 words).
+// ceil(M / 4) x ceil(N / 4) grid of (row quad, four consecutive columns) (the padded row
+// stride Np is a multiple of 64 words).
 template <bool WIDE>
 __global__ __launch_bounds__(256) void k_grp_crt_rebuild(const CrtGroupProb *__restrict__ probs,
                                                          const int4 *__restrict__ items, int rounding, CrtConsts c) {
@@ -609,28 +619,11 @@ __global__ __launch_bounds__(256) void k_grp_crt_rebuild(const CrtGroupProb *__r
     const uint32_t idx = (uint32_t)it.y + threadIdx.x;
     if (idx >= (uint32_t)((pr.M + 3) / 4) * nc4) return;
     const uint32_t row4 = idx / nc4;
-    const int col0 = 4 * (int)(idx - row4 * nc4);
-    const int64_t plane = pr.RbA * (kBlk / 4) * pr.Np;
-    const uint32_t *Rq = pr.R + (int64_t)row4 * pr.Np + col0;
-    uint4 w4[kCrtMaxMod];
-#pragma unroll
-    for (int j = 0; j < kCrtMaxMod; j++)
-        w4[j] = j < c.n ? *reinterpret_cast<const uint4 *>(Rq + (int64_t)j * plane)
-                        : make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u);
+    const int64_t col0 = 4 * (int64_t)(idx - row4 * nc4);
     const int rest = pr.M - 4 * (int)row4;
-    const int rows = rest < 4 ? rest : 4;
-#pragma unroll
-    for (int cc = 0; cc < 4; cc++) {
-        if (col0 + cc >= pr.N) break;
-        uint32_t wd[kCrtMaxMod];
-#pragma unroll
-        for (int j = 0; j < kCrtMaxMod; j++) {
-            const uint32_t v = cc == 0 ? w4[j].x : (cc == 1 ? w4[j].y : (cc == 2 ? w4[j].z : w4[j].w));
-            wd[j] = v ^ 0x80808080u;   // planes j >= n: zero words
-        }
-        crt_rebuild_quad<WIDE>(wd, c, rows, pr.Er + 4 * (int64_t)row4, pr.Ec[col0 + cc], pr.beta, rounding,
-                               pr.C + 4 * (int64_t)row4 * pr.ldc + col0 + cc, pr.ldc);
-    }
+    crt_rebuild_cols<WIDE>(pr.R + (int64_t)row4 * pr.Np, pr.RbA * (kBlk / 4) * pr.Np, col0, pr.N, rest < 4 ? rest : 4,
+                           c, pr.Er + 4 * (int64_t)row4, pr.Ec, pr.beta, rounding,
+                           pr.C + 4 * (int64_t)row4 * pr.ldc, pr.ldc);
 }
 
 hipError_t launch_group_crt_rebuild(const CrtGroupProb *probs, const int4 *items, int n, int rounding,
