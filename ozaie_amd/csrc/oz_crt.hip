@@ -97,31 +97,48 @@ hipError_t launch_crt_split(const OperandView &op, const int32_t *E, const CrtCo
     return hipGetLastError();
 }
 
-// Shared-fragment variant: the workgroup's 2 WM A and 2 WN B blocks of each contraction
-// block are loaded once per workgroup -- every thread brings PER x 16 bytes of whole
-// 1 KiB blocks into a register ring PF blocks deep -- and written to one of two LDS
-// stages; after a barrier each wave reads its WM + WN fragments from LDS.  Halves the
-// L2 -> CU traffic of the per-wave loads.  The barrier is a plain s_barrier after the
-// wave's own LDS writes have completed (no memory fence), so the global loads of the
-// ring stay in flight across it; the inline-asm memory clobber keeps the compiler from
-// moving LDS accesses across it.
+// Barrier between the waves of a workgroup after the wave's own LDS reads have completed
+// (no memory fence, so global loads stay in flight across it); the inline-asm memory
+// clobber keeps the compiler from moving LDS accesses across it.
 __device__ __forceinline__ void lds_barrier() {
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// Contraction loop of one (64 WM) x (64 WN) tile.  setup: A0 / B0 = the tile's first row
-// block of one modulus plane of A / B (rows of rowstride bytes), nk contraction blocks;
-// prime: the first PF blocks into the ring; run: the products, leaving the ring consumed
-// (its last fetches re-read block nk - 1, in bounds); advance: move to another plane, for
-// the next prime.  TAIL: nk need not be a multiple of PF (the steps past nk are skipped;
-// the condition is uniform over the workgroup, so every wave meets the same barriers).
-template <int WM, int WN, int PF, bool TAIL>
+constexpr int kCrtStages = 4;   // LDS stages of the modulus GEMM's contraction loop
+
+// Contraction loop of one (64 WM) x (64 WN) tile, the workgroup's 2 WM A and 2 WN B blocks
+// of each contraction block shared through LDS.  The global loads write LDS themselves
+// (global_load_lds_dwordx4: lane l's 16 bytes land at the wave-uniform LDS address + 16 l,
+// which is a block's own layout), so there is no register staging and no LDS store; each
+// wave brings PER whole 1 KiB blocks of a contraction block.  S stages keep blocks
+// b + 1 .. b + S - 1 in flight while block b is multiplied, and the fragments of block b + 1
+// are read from LDS while the MFMAs of block b run:
+//   step b: wait for the wave's own loads of block b + 1 (loads complete in order, so the
+//           S - 2 later blocks stay in flight), barrier (block b + 1 is in LDS; every wave
+//           has read block b), issue block b + S into the stage block b used, read block
+//           b + 1, multiply block b (its fragments are in registers since step b - 1).
+// No load goes past block nk - 1: where fewer than S - 2 blocks follow, the wait is for
+// every load, so after the last step nothing is in flight and no wave reads LDS any more
+// (the last step reads nothing): prime() for another plane may follow at once.  The
+// conditions are uniform over the workgroup, so every wave meets the same barriers.
+// setup: A0 / B0 = the tile's first row block of one modulus plane of A / B (rows of
+// rowstride bytes), nk >= 1 contraction blocks; prime: issue the first S - 1 blocks;
+// run: the products; advance: move to another plane, for the next prime.
+template <int WM, int WN, int S>
 struct CrtTileLoop {
     static constexpr int BA = 2 * WM, BB = 2 * WN, NB = BA + BB;   // blocks per contraction block
     static constexpr int PER = NB / 4;                            // blocks each wave brings in
     static_assert(NB % 4 == 0, "blocks must split over four waves");
-    const int8_t *src[PER];
-    v4i ring[PF][PER];
+    static_assert(S >= 3 && S % 2 == 0, "two fragment sets alternate over an even number of stages");
+    static constexpr int VM = PER * (S - 2);   // loads of the wave still allowed in flight
+    static_assert(VM < 64, "vmcnt range");
+    // s_waitcnt vmcnt(VM) lgkmcnt(0) and vmcnt(0) lgkmcnt(0) as builtins: the compiler's own
+    // wait placement sees them (it does not read the inline asm of lds_barrier), so it knows
+    // the fragments of the previous step have arrived and adds no wait on the LDS reads
+    // issued after the barrier.
+    static constexpr int kWaitSteady = (VM & 15) | ((VM >> 4) << 14) | (7 << 4);
+    static constexpr int kWaitAll = 7 << 4;
+    const int8_t *src[PER];   // wave-uniform: first byte of each block row the wave brings
     int64_t nk;
     int lane, w, wm, wn;
 
@@ -134,58 +151,83 @@ struct CrtTileLoop {
 #pragma unroll
         for (int i = 0; i < PER; i++) {
             const int bi = w + 4 * i;   // block bi of the contraction block: A rows, then B columns
-            src[i] = (bi < BA ? A0 + (int64_t)bi * rowstride : B0 + (int64_t)(bi - BA) * rowstride) + lane * 16;
+            src[i] = bi < BA ? A0 + (int64_t)bi * rowstride : B0 + (int64_t)(bi - BA) * rowstride;
         }
-    }
-    __device__ __forceinline__ void fetch(int s, int64_t kb) {
-        kb = kb < nk ? kb : nk - 1;
-#pragma unroll
-        for (int i = 0; i < PER; i++) ring[s][i] = *reinterpret_cast<const v4i *>(src[i] + kb * kBlkBytes);
-    }
-    __device__ __forceinline__ void prime() {
-#pragma unroll
-        for (int s = 0; s < PF; s++) fetch(s, s);
     }
     __device__ __forceinline__ void advance(int64_t offA, int64_t offB) {
 #pragma unroll
         for (int i = 0; i < PER; i++) src[i] += (w + 4 * i < BA) ? offA : offB;
     }
-    __device__ __forceinline__ void step(int s, int64_t t, int8_t *smem, v16i (&acc)[WM][WN]) {
-        int8_t *st = smem + ((s & 1) * NB) * kBlkBytes;
+    template <bool CHECK = true>
+    __device__ __forceinline__ void issue(int8_t *smem, int slot, int64_t kb) {
+        if (CHECK && kb >= nk) return;
 #pragma unroll
         for (int i = 0; i < PER; i++)
-            *reinterpret_cast<v4i *>(st + (w + 4 * i) * kBlkBytes + lane * 16) = ring[s][i];
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(src[i] + kb * kBlkBytes + lane * 16),
+                (__attribute__((address_space(3))) void *)(smem + (slot * NB + w + 4 * i) * kBlkBytes), 16, 0, 0);
+    }
+    __device__ __forceinline__ void prime(int8_t *smem) {
+#pragma unroll
+        for (int s = 0; s < S - 1; s++) issue(smem, s, s);
+    }
+    // wait until block kb and all before it are in LDS, then barrier
+    template <bool CHECK = true>
+    __device__ __forceinline__ void sync(int64_t kb) {
+        if (!CHECK || kb + S - 2 < nk)
+            __builtin_amdgcn_s_waitcnt(kWaitSteady);
+        else
+            __builtin_amdgcn_s_waitcnt(kWaitAll);
         lds_barrier();
-        fetch(s, t + PF + s);
-        v4i af[WM], bf[WN];
+    }
+    __device__ __forceinline__ void read(const int8_t *st, v4i (&af)[WM], v4i (&bf)[WN]) {
 #pragma unroll
         for (int mi = 0; mi < WM; mi++)
             af[mi] = *reinterpret_cast<const v4i *>(st + (wm * WM + mi) * kBlkBytes + lane * 16);
 #pragma unroll
         for (int ni = 0; ni < WN; ni++)
             bf[ni] = *reinterpret_cast<const v4i *>(st + (BA + wn * WN + ni) * kBlkBytes + lane * 16);
+    }
+    // Step b (stage s = b mod S, fragment set s & 1).  CHECK = false: blocks b + S - 1 and
+    // b + S exist, so the step has no branch; the LDS reads and the MFMAs then share one
+    // basic block and the compiler adds no wait between them.  The read of block b + 1 is
+    // made even past the last block: it reads a stage (possibly while the next plane's
+    // first blocks land in it) into fragments that are never multiplied.
+    template <bool CHECK>
+    __device__ __forceinline__ void step(int s, int64_t b, int8_t *smem, v4i (&af)[2][WM], v4i (&bf)[2][WN],
+                                         v16i (&acc)[WM][WN]) {
+        const int f = s & 1;
+        __builtin_amdgcn_sched_barrier(0);
+        sync<CHECK>(b + 1);
+        issue<CHECK>(smem, s, b + S);
+        read(smem + ((s + 1) % S) * NB * kBlkBytes, af[f ^ 1], bf[f ^ 1]);
 #pragma unroll
         for (int mi = 0; mi < WM; mi++)
 #pragma unroll
             for (int ni = 0; ni < WN; ni++)
-                acc[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[mi], bf[ni], acc[mi][ni], 0, 0, 0);
+                acc[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[f][mi], bf[f][ni], acc[mi][ni], 0, 0, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, WM + WN, 0);   // the LDS reads first,
+        __builtin_amdgcn_sched_group_barrier(0x008, WM * WN, 0);   // then the MFMAs
+        __builtin_amdgcn_sched_barrier(0);
     }
     __device__ __forceinline__ void run(int8_t *smem, v16i (&acc)[WM][WN]) {
 #pragma unroll
         for (int mi = 0; mi < WM; mi++)
 #pragma unroll
             for (int ni = 0; ni < WN; ni++) acc[mi][ni] = v16i{};
-        const int64_t nfull = TAIL ? nk - nk % PF : nk;
+        v4i af[2][WM], bf[2][WN];
+        sync(0);
+        issue(smem, S - 1, S - 1);
+        read(smem, af[0], bf[0]);
         int64_t t = 0;
-        for (; t < nfull; t += PF) {
+        for (; t + 2 * S <= nk; t += S) {   // every step of the pass loads a block that exists
 #pragma unroll
-            for (int s = 0; s < PF; s++) step(s, t, smem, acc);
+            for (int s = 0; s < S; s++) step<false>(s, t + s, smem, af, bf, acc);
         }
-        if (TAIL) {
-            const int64_t rest = nk - nfull;   // the ring holds blocks nfull .. nfull + PF - 1
+        for (; t < nk; t += S) {
 #pragma unroll
-            for (int s = 0; s < PF - 1; s++)
-                if (s < rest) step(s, t, smem, acc);
+            for (int s = 0; s < S; s++)
+                if (t + s < nk) step<true>(s, t + s, smem, af, bf, acc);
         }
     }
 };
@@ -194,8 +236,9 @@ struct CrtTileLoop {
 // Rj[row / 4][col] (row stride ldr words), one signed byte each: r = x - p q with q the
 // integer nearest x / p, congruent to x, |r| <= p / 2.  q comes from fma(x, 1/p, 1.5 2^52),
 // which rounds the exact product to an integer (|x| <= 2^30, so x fl(1/p) is never within
-// 1 / (2p) of a half-integer for odd p, and is exact for p = 256: ties to even, as rint);
-// r + 1.5 2^52 holds r in two's complement in its low word, whose low bytes are packed with
+// 1 / (2p) of a half-integer for odd p, and is exact for p = 256: ties to even, as rint)
+// and holds q in two's complement in its low word.  |q| <= 2^30 / 167 < 2^23, so
+// r = x - p q is one 24-bit multiply-add in int32; the low bytes of four r are packed with
 // v_perm_b32.  CLIP: words whose four rows all lie at or beyond M, or whose column lies at
 // or beyond N, are not written.
 template <int WM, int WN, bool CLIP>
@@ -205,6 +248,7 @@ __device__ __forceinline__ void crt_tile_store(const v16i (&acc)[WM][WN], double
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = w & 1, wn = w >> 1;
     const int64_t rbase = tm * 64 * WM + wm * 32 * WM, cbase = tn * 64 * WN + wn * 32 * WN;
+    const int np = -(int)p;
 #pragma unroll
     for (int mi = 0; mi < WM; mi++)
 #pragma unroll
@@ -215,10 +259,9 @@ __device__ __forceinline__ void crt_tile_store(const v16i (&acc)[WM][WN], double
                 uint32_t lo[4];
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
-                    const double x = (double)acc[mi][ni][4 * q + e];
-                    const double qv = __builtin_fma(x, inv, kMagic) - kMagic;
-                    const double rm = __builtin_fma(-qv, p, x) + kMagic;
-                    lo[e] = (uint32_t)__double_as_longlong(rm);
+                    const int x = acc[mi][ni][4 * q + e];
+                    const int qi = (int)(uint32_t)__double_as_longlong(__builtin_fma((double)x, inv, kMagic));
+                    lo[e] = (uint32_t)(__mul24(qi, np) + x);
                 }
                 const uint32_t word = __builtin_amdgcn_perm(__builtin_amdgcn_perm(lo[3], lo[2], 0x0c0c0400u),
                                                             __builtin_amdgcn_perm(lo[1], lo[0], 0x0c0c0400u),
@@ -283,10 +326,10 @@ void k_crt_gemm(CrtGemm g, CrtConsts c) {
                                   g.Np);
 }
 
-template <int WM, int WN, int PF>
+template <int WM, int WN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_crt_gemm_lds(CrtGemm g, CrtConsts c) {
-    __shared__ __attribute__((aligned(1024))) int8_t smem[2 * (2 * WM + 2 * WN) * kBlkBytes];
+    __shared__ __attribute__((aligned(1024))) int8_t smem[kCrtStages * (2 * WM + 2 * WN) * kBlkBytes];
     const int j = blockIdx.y;
     const int ntm = (int)(g.Mp / (64 * WM)), ntn = (int)(g.Np / (64 * WN));
     int tm, tn;
@@ -295,9 +338,9 @@ void k_crt_gemm_lds(CrtGemm g, CrtConsts c) {
     const int8_t *A0 = g.SA + (int64_t)j * (g.Mp / kBlk) * rowstride + (int64_t)tm * 2 * WM * rowstride;
     const int8_t *B0 = g.SB + (int64_t)j * (g.Np / kBlk) * rowstride + (int64_t)tn * 2 * WN * rowstride;
     v16i acc[WM][WN];
-    CrtTileLoop<WM, WN, PF, false> L;
+    CrtTileLoop<WM, WN, kCrtStages> L;
     L.setup(A0, B0, rowstride, g.Kb);
-    L.prime();
+    L.prime(smem);
     L.run(smem, acc);
     crt_tile_store<WM, WN, false>(acc, c.p[j], c.inv[j], g.R + (int64_t)j * (g.Mp / 4) * g.Np, g.Np, tm, tn,
                                   g.Mp, g.Np);
@@ -313,8 +356,7 @@ hipError_t launch_crt_gemm(const CrtGemm &g, const CrtConsts &c, hipStream_t st)
     if (g.variant == 1)
         hipLaunchKernelGGL((k_crt_gemm<WM, WN>), dim3((unsigned)tiles, (unsigned)c.n), dim3(256), 0, st, g, c);
     else
-        hipLaunchKernelGGL((k_crt_gemm_lds<WM, WN, kCrtPF>), dim3((unsigned)tiles, (unsigned)c.n), dim3(256), 0, st,
-                           g, c);
+        hipLaunchKernelGGL((k_crt_gemm_lds<WM, WN>), dim3((unsigned)tiles, (unsigned)c.n), dim3(256), 0, st, g, c);
     return hipGetLastError();
 }
 
@@ -551,20 +593,20 @@ hipError_t launch_group_crt_split(const GroupOperand *ops, const CrtGroupProb *p
     return hipGetLastError();
 }
 
-// Waves per SIMD the tile shape leaves room for: the 256 x 256 tile needs every register of
-// one wave (16 accumulator tiles + the ring), the smaller ones fit two and four.
+// Waves per SIMD the tile shape leaves room for: the 256 x 256 tile's 16 accumulator tiles
+// take half the registers of one wave, the smaller ones fit two and four.
 constexpr int crt_occupancy(int tiles) { return tiles >= 8 ? 1 : (tiles >= 2 ? 2 : 4); }
 
 // One workgroup per tile item and range of moduli [j0, j1) (grid y): the contraction loop
-// of each modulus in turn, the ring primed with the next modulus's first blocks before the
-// epilogue of the current one, so those loads are in flight during the residue stores.
+// of each modulus in turn, the next modulus's first blocks issued before the epilogue of
+// the current one, so those loads are in flight during the residue stores.
 // Items {-1, ...} pad the host's per-XCD lists (the hardware deals blockIdx x to XCD x % 8)
 // and return at once.
 template <int WM, int WN>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(crt_occupancy(WM * WN), crt_occupancy(WM * WN))))
 void k_grp_crt_gemm(const CrtGroupProb *__restrict__ probs, const int4 *__restrict__ items, int mpw, CrtConsts c) {
-    __shared__ __attribute__((aligned(1024))) int8_t smem[2 * (2 * WM + 2 * WN) * kBlkBytes];
+    __shared__ __attribute__((aligned(1024))) int8_t smem[kCrtStages * (2 * WM + 2 * WN) * kBlkBytes];
     const int4 it = items[blockIdx.x];
     if (it.x < 0) return;
     const int j0 = blockIdx.y * mpw;
@@ -573,17 +615,16 @@ void k_grp_crt_gemm(const CrtGroupProb *__restrict__ probs, const int4 *__restri
     const int64_t rowstride = pr.Kb * kBlkBytes;
     const int64_t planeA = pr.RbA * rowstride, planeB = pr.RbB * rowstride;
     const int64_t planeR = pr.RbA * (kBlk / 4) * pr.Np;
-    CrtTileLoop<WM, WN, kCrtPF, true> L;
+    CrtTileLoop<WM, WN, kCrtStages> L;
     L.setup(pr.SA + (int64_t)j0 * planeA + (int64_t)it.y * 2 * WM * rowstride,
             pr.SB + (int64_t)j0 * planeB + (int64_t)it.z * 2 * WN * rowstride, rowstride, pr.Kb);
-    L.prime();
+    L.prime(smem);
     for (int j = j0; j < j1; j++) {
-        if (j > j0) lds_barrier();   // every wave is done reading the previous modulus's stages
         v16i acc[WM][WN];
         L.run(smem, acc);
-        if (j + 1 < j1) {
+        if (j + 1 < j1) {   // the next modulus's first blocks load during the epilogue
             L.advance(planeA, planeB);
-            L.prime();
+            L.prime(smem);
         }
         crt_tile_store<WM, WN, true>(acc, c.p[j], c.inv[j], pr.R + (int64_t)j * planeR, pr.Np, it.y, it.z, pr.M,
                                      pr.N);

@@ -115,7 +115,7 @@ constexpr int kCrtDefaultMod = 16;   // OZAIE_MODE_CRT: every product below 2^12
 constexpr int kCrtNarrowMod = 16;    // up to this many moduli the rebuild runs in 128 bits, above in 192
 constexpr int kCrtLimbs = 24;        // byte limbs of the 192-bit rebuild
 constexpr int kCrtRowPad = 256;   // the modulus GEMM's workgroup tile is 256 x 256
-constexpr int kCrtPF = 8;         // contraction blocks in flight in the modulus GEMM (K pads to 32 kCrtPF)
+constexpr int kCrtPF = 8;         // K pads to 32 kCrtPF (register ring depth of the per-wave modulus GEMM)
 struct CrtConsts {
     int n, beta;
     double p[kCrtMaxMod];      // modulus
