@@ -431,7 +431,7 @@ __device__ __forceinline__ double crt_wide_to_double(uint64_t w0, uint64_t w1, u
 template <bool WIDE>
 __device__ __forceinline__ void crt_rebuild_quad(const uint32_t (&wd)[kCrtMaxMod], const CrtConsts &c, int rows,
                                                  const int32_t *__restrict__ Er4, int ec, int beta, int rounding,
-                                                 double *__restrict__ Cq, int64_t ldc) {
+                                                 double (&out)[4]) {
     constexpr int NG = (WIDE ? kCrtMaxMod : kCrtNarrowMod) / 4;   // groups of four moduli
     constexpr int NL = WIDE ? kCrtLimbs : 16;                      // byte limbs of C'
     constexpr int NW = WIDE ? 20 : 16;                             // byte limbs of W_p (W_p < M < 2^156)
@@ -487,7 +487,7 @@ __device__ __forceinline__ void crt_rebuild_quad(const uint32_t (&wd)[kCrtMaxMod
         const int e2 = Er4[e] + ec - 2044 - 2 * beta;
         if (!WIDE) {
             const u128 S = (u128)Q[0] + ((u128)Q[1] << 32) + ((u128)Q[2] << 64) + ((u128)Q[3] << 96) - (u128)128;
-            Cq[e * ldc] = crt_to_double((i128)S, e2, rounding);
+            out[e] = crt_to_double((i128)S, e2, rounding);
         } else {
             const u128 s0 = (u128)Q[0] + ((u128)Q[1] << 32);
             const u128 s1 = (s0 >> 64) + Q[2] + ((u128)Q[3] << 32);
@@ -497,14 +497,18 @@ __device__ __forceinline__ void crt_rebuild_quad(const uint32_t (&wd)[kCrtMaxMod
             w0 -= 128u;
             w1 -= b0;
             w2 -= b1;
-            Cq[e * ldc] = crt_wide_to_double(w0, w1, w2, e2, rounding);
+            out[e] = crt_wide_to_double(w0, w1, w2, e2, rounding);
         }
     }
 }
 
 // Four rows 4 row4 + e (e < rows) of the four consecutive columns col0 + cc (col0 + cc < N):
 // one 16-byte load per modulus plane (planes `plane` words apart), then crt_rebuild_quad
-// per column.  Row quad row4 of the planes starts at Rq; Er4 / Cq at the quad's first row.
+// per column, and the 4 x 4 results stored by row, 32 bytes per row in two 16-byte stores
+// where the row's four columns exist and are 16-byte aligned (a wave then writes whole
+// cache lines at once; stores of one column each left lines partly written in L2, and
+// those reached HBM as partial writes).  Row quad row4 of the planes starts at Rq; Er4 /
+// Cq at the quad's first row.
 template <bool WIDE>
 __device__ __forceinline__ void crt_rebuild_cols(const uint32_t *__restrict__ Rq, int64_t plane, int64_t col0,
                                                  int64_t N, int rows, const CrtConsts &c,
@@ -515,6 +519,7 @@ __device__ __forceinline__ void crt_rebuild_cols(const uint32_t *__restrict__ Rq
     for (int j = 0; j < kCrtMaxMod; j++)
         w4[j] = j < c.n ? *reinterpret_cast<const uint4 *>(Rq + (int64_t)j * plane + col0)
                         : make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u);
+    double out[4][4];   // [column][row]
 #pragma unroll
     for (int cc = 0; cc < 4; cc++) {
         if (col0 + cc >= N) break;
@@ -524,7 +529,21 @@ __device__ __forceinline__ void crt_rebuild_cols(const uint32_t *__restrict__ Rq
             const uint32_t v = cc == 0 ? w4[j].x : (cc == 1 ? w4[j].y : (cc == 2 ? w4[j].z : w4[j].w));
             wd[j] = v ^ 0x80808080u;   // planes j >= n: zero words
         }
-        crt_rebuild_quad<WIDE>(wd, c, rows, Er4, Ec[col0 + cc], beta, rounding, Cq + col0 + cc, ldc);
+        crt_rebuild_quad<WIDE>(wd, c, rows, Er4, Ec[col0 + cc], beta, rounding, out[cc]);
+    }
+    const bool full = col0 + 3 < N;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        if (e >= rows) break;
+        double *dst = Cq + e * ldc + col0;
+        if (full && (reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
+            *reinterpret_cast<double2 *>(dst) = make_double2(out[0][e], out[1][e]);
+            *reinterpret_cast<double2 *>(dst + 2) = make_double2(out[2][e], out[3][e]);
+        } else {
+#pragma unroll
+            for (int cc = 0; cc < 4; cc++)
+                if (col0 + cc < N) dst[cc] = out[cc][e];
+        }
     }
 }
 
