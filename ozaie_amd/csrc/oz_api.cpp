@@ -341,17 +341,21 @@ GemmLaunch base_launch(const ozaie_plan p) {
     return g;
 }
 
-// Exponents + slices of both operands.
+// Exponents + slices of both operands, one operand after the other: the split's second
+// read of an operand follows its exponent pass directly, before the other operand streams
+// through the memory-side cache (256 MB).  Measured: 2-3% off the split-scheme split phase
+// at 4096^3, no change for the CRT residues or at 8192^3.
 int run_split(ozaie_plan p, const OperandView &va, const OperandView &vb, hipStream_t st) {
     HIPCHK(launch_maxexp(va, p->Er, p->Mp, st));
-    HIPCHK(launch_maxexp(vb, p->Ec, p->Np, st));
-    if (p->crt) {
+    if (p->crt)
         HIPCHK(launch_crt_split(va, p->Er, p->cc, p->SA, p->Mp, p->Kp, st));
+    else
+        HIPCHK(launch_split(va, p->Er, p->b, p->d, p->SA, p->Mp, p->Kp, st));
+    HIPCHK(launch_maxexp(vb, p->Ec, p->Np, st));
+    if (p->crt)
         HIPCHK(launch_crt_split(vb, p->Ec, p->cc, p->SB, p->Np, p->Kp, st));
-        return OZAIE_OK;
-    }
-    HIPCHK(launch_split(va, p->Er, p->b, p->d, p->SA, p->Mp, p->Kp, st));
-    HIPCHK(launch_split(vb, p->Ec, p->b, p->d, p->SB, p->Np, p->Kp, st));
+    else
+        HIPCHK(launch_split(vb, p->Ec, p->b, p->d, p->SB, p->Np, p->Kp, st));
     return OZAIE_OK;
 }
 

@@ -72,6 +72,34 @@ def test_crt_C_bit_exact(gpu, M, N, K, n, phi, rounding):
     p.close()
 
 
+@pytest.mark.parametrize("M,N,K,n", [(130, 67, 300, 16), (64, 128, 100, 16), (33, 36, 90, 20)])
+def test_crt_C_into_strided_view(gpu, M, N, K, n):
+    """C written through a view that starts 8 bytes past a 16-byte boundary, with a row
+    stride above N: equal to the contiguous result, and nothing outside the view written
+    (the rebuild stores four columns of a row with 16-byte stores only where they are
+    aligned, one double at a time elsewhere)."""
+    import torch
+    rng = np.random.default_rng(M + 3 * N + K)
+    A = gen_inputs(rng, M, K, 1.0)
+    B = gen_inputs(rng, K, N, 1.0)
+    p = oz.Plan(M, N, K, 0, n, rounding=oz.ROUND_NEAREST, crt=True)
+    ref = p(dev(A), dev(B))
+    torch.cuda.synchronize()
+    big = torch.full((M + 5, N + 7), float("nan"), dtype=torch.float64, device="cuda")
+    for r0, c0 in ((3, 1), (2, 2)):
+        big.fill_(float("nan"))
+        C = big[r0:r0 + M, c0:c0 + N]
+        assert C.stride(0) == N + 7
+        p(dev(A), dev(B), C)
+        torch.cuda.synchronize()
+        assert torch.equal(C, ref), (r0, c0)
+        outside = big.clone()
+        outside[r0:r0 + M, c0:c0 + N] = float("nan")
+        assert bool(torch.isnan(outside).all()), (r0, c0)
+    assert np.array_equal(bits(ref.cpu().numpy()), bits(oracle.crt_dgemm(A, B, n, oz.ROUND_NEAREST)))
+    p.close()
+
+
 def test_crt_transposes_and_explicit_bits(gpu):
     import torch
     rng = np.random.default_rng(11)
