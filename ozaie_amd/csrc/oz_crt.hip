@@ -97,13 +97,6 @@ hipError_t launch_crt_split(const OperandView &op, const int32_t *E, const CrtCo
     return hipGetLastError();
 }
 
-// Barrier between the waves of a workgroup after the wave's own LDS reads have completed
-// (no memory fence, so global loads stay in flight across it); the inline-asm memory
-// clobber keeps the compiler from moving LDS accesses across it.
-__device__ __forceinline__ void lds_barrier() {
-    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
-
 constexpr int kCrtStages = 4;   // LDS stages of the modulus GEMM's contraction loop
 
 // Contraction loop of one (64 WM) x (64 WN) tile, the workgroup's 2 WM A and 2 WN B blocks
@@ -118,9 +111,10 @@ constexpr int kCrtStages = 4;   // LDS stages of the modulus GEMM's contraction 
 //           has read block b), issue block b + S into the stage block b used, read block
 //           b + 1, multiply block b (its fragments are in registers since step b - 1).
 // No load goes past block nk - 1: where fewer than S - 2 blocks follow, the wait is for
-// every load, so after the last step nothing is in flight and no wave reads LDS any more
-// (the last step reads nothing): prime() for another plane may follow at once.  The
-// conditions are uniform over the workgroup, so every wave meets the same barriers.
+// every load, so after the last step nothing is in flight, and the only LDS read after
+// the last barrier is the last step's read of a stage into fragments that are never
+// multiplied: prime() for another plane may follow at once.  The conditions are uniform
+// over the workgroup, so every wave meets the same barriers.
 // setup: A0 / B0 = the tile's first row block of one modulus plane of A / B (rows of
 // rowstride bytes), nk >= 1 contraction blocks; prime: issue the first S - 1 blocks;
 // run: the products; advance: move to another plane, for the next prime.

@@ -329,6 +329,13 @@ __device__ __forceinline__ void tile_map(int bid, int ntm, int ntn, int &tm, int
     band_map(xcd_remap(bid, ntm * ntn), ntm, ntn, tm, tn);
 }
 
+// Barrier between the waves of a workgroup after the wave's own LDS reads have completed
+// (no memory fence, so global loads stay in flight across it); the inline-asm memory
+// clobber keeps the compiler from moving LDS accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 template <int N>
 __device__ __forceinline__ void s_waitcnt_vm() {
     // gfx9 encoding: vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt_hi[15:14]
