@@ -246,7 +246,38 @@ def cpu_baseline(args, np, Ah, Bh, d, n, crt=False):
     except Exception:
         rec["host_blas"] = "numpy default"
     rec["cpu_model"] = cpu_model()
+    try:
+        rec["reference_host_phases"] = reference_host_phases(np, n)
+    except Exception as e:
+        rec["reference_host_phases"] = {"error": repr(e)}
     return rec
+
+
+def reference_host_phases(np, n):
+    """The reference's host-only work, single-threaded as it runs there: the paired-int64 ->
+    FP64 conversion (include/oz_aie.h:92-150, the phase its test driver times as "CPU") over a
+    whole n x n C, on synthetic words of the paired layout (b = 7, d = 8, K = n; the cost does
+    not depend on the values), and the naive triple-loop DGEMM on the 256 x 512 x 128 example
+    shape.  Both through the oracle's C restatement."""
+    import oracle
+    rng = np.random.default_rng(1)
+    L = rng.integers(-(1 << 50), 1 << 50, size=(n, n), dtype=np.int64)
+    H = rng.integers(-(1 << 38), 1 << 38, size=(n, n), dtype=np.int64)
+    Hw = ((H.astype(np.uint64) & np.uint64((1 << 51) - 1)) | (np.uint64(2046) << np.uint64(51))
+          | ((H < 0).astype(np.uint64) << np.uint64(63)))
+    P = np.stack([L, Hw.view(np.int64)], axis=-1)
+    t0 = time.time()
+    oracle.convert_paired(P, n, 7, 8)
+    tc = time.time() - t0
+    del P, L, H, Hw
+    A = rng.standard_normal((256, 128))
+    B = rng.standard_normal((128, 512))
+    t0 = time.time()
+    oracle.naive_dgemm(A, B)
+    tn = time.time() - t0
+    return {"conversion": {"seconds": tc, "elements": n * n, "threads": 1,
+                           "what": f"paired int64 -> FP64 over {n}x{n} C (include/oz_aie.h:92-150)"},
+            "naive_dgemm_256x512x128": {"gflops": 2.0 * 256 * 512 * 128 / tn / 1e9, "threads": 1}}
 
 
 def cpu_model():
