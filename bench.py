@@ -21,6 +21,8 @@ The JSON line also carries:
   cpu_baseline  the oracle's CPU path of the matched preset (the port of the reference
                 rule, or of the CRT mode) on a row sample of the same workload, on this
                 host's cores
+  host_pointer  the matched preset through the reference-style entry point with host arrays
+                (PCIe copies of A, B and C inside the timed call); not `value`
   target_8192   the same measurements at 8192^3
 """
 import argparse
@@ -251,6 +253,34 @@ def cpu_baseline(args, np, Ah, Bh, d, n, crt=False):
     except Exception as e:
         rec["reference_host_phases"] = {"error": repr(e)}
     return rec
+
+
+def host_pointer(np, oz, Ah, Bh, n, crt, d, reps=5):
+    """The reference-style entry point with host arrays (ozAIE_setup + ozAIE_DGEMM, blocking,
+    as the reference is called) on the same inputs: its time includes copying A and B to the
+    GPU and C back from pageable numpy memory.  Median of `reps` calls after one warm call."""
+    if crt:
+        rc = oz.ozAIE_setup(n, n, n, oz.Mode.crt)
+    else:
+        rc = oz.ozAIE_setup_slices(n, n, n, 7, d)
+    if rc != oz.OK:
+        return {"error": f"setup status {rc}"}
+    A, B = np.ascontiguousarray(Ah), np.ascontiguousarray(Bh)
+    C = np.zeros((n, n))
+    oz.ozAIE_DGEMM(n, n, n, A, B, C)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        rc = oz.ozAIE_DGEMM(n, n, n, A, B, C)
+        ts.append(time.perf_counter() - t0)
+        if rc != oz.OK:
+            return {"error": f"DGEMM status {rc}"}
+    ts.sort()
+    t = ts[len(ts) // 2]
+    return {"ms": t * 1e3, "tflops": 2.0 * n ** 3 / t / 1e12, "host_bytes_moved": 3 * n * n * 8,
+            "memory": "pageable numpy arrays",
+            "what": ("ozAIE_DGEMM(Mode.crt)" if crt else f"ozAIE_DGEMM, b=7 d={d}")
+                    + " with host A, B, C: the copies in and out are inside the timed call"}
 
 
 def reference_host_phases(np, n):
@@ -579,6 +609,10 @@ def main():
                      "traffic": traffic, "kernel": kname,
                      "avg_launch_ms": m["gemm_ms"]},
     }
+    try:
+        line["host_pointer"] = host_pointer(np, oz, Ah, Bh, n, crt, d)
+    except Exception as e:  # reported beside value, never required
+        line["host_pointer"] = {"error": repr(e)}
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(args, np, Ah, Bh, d, n, crt)
