@@ -4,6 +4,8 @@
 // slice blocks, the epilogue arithmetic (exact 128-bit T and its conversion to FP64), the
 // XCD-aware tile map and the fused slice-GEMM tile bodies.  Everything here is inline.
 #pragma once
+#include <type_traits>
+
 #include "oz_internal.h"
 
 namespace oz {
@@ -513,6 +515,139 @@ __device__ __forceinline__ void gemm_lds_tile(const GemmLaunch &g, int tm, int t
                     for (int ni = 0; ni < WN; ni++)
                         acc[i + j][mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i][mi], bf[j][ni],
                                                                                    acc[i + j][mi][ni], 0, 0, 0);
+    }
+    __syncthreads();
+
+    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * wave_scratch_words<D, WM, WN>();
+    wave_epilogue<D, WM, WN>(g, epi, 0, acc, scratch, lane, rbase, cbase, er_v, ec_v);
+}
+
+// Staged variant (d <= 4, workgroup tile 128 x 128): the slice blocks go HBM -> LDS
+// (global_load_lds_dwordx4, each wave bringing NBLK / 4 whole 1 KiB blocks) into one of
+// four stages, so three contraction blocks are in flight, and the fragments of block b + 1
+// are read from LDS while the triangle of block b runs:
+//   step b: wait for the wave's own loads of block b + 1 (loads complete in order, so the
+//           later blocks' loads stay in flight), barrier (block b + 1 is in LDS; every wave
+//           has read block b), issue block b + 4 into the stage block b used, read block
+//           b + 1, multiply block b (its fragments are in registers since step b - 1).
+// The waits are builtins: the compiler's own wait placement does not read the inline-asm
+// barrier and would otherwise wait on the new reads before the first MFMA.  The steady
+// loop has no branch; a checked tail pass issues no load past the pass's last block and
+// waits for every load, so the epilogue's LDS scratch follows the last barrier safely.
+// Any number of contraction blocks per pass.
+constexpr int kStagedStages = 4;
+
+template <int D, int WM, int WN>
+__host__ __device__ constexpr int staged_tile_bytes() { return kStagedStages * FusedCfg<D, WM, WN>::STAGE; }
+
+template <int D, int WM, int WN>
+__device__ __forceinline__ void gemm_staged_tile(const GemmLaunch &g, int tm, int tn, int epi, int8_t *smem) {
+    using F = FusedCfg<D, WM, WN>;
+    constexpr int BA = F::BA, BB = F::BB, NBLK = F::NBLK, S = kStagedStages;
+    static_assert(NBLK % 4 == 0, "every wave brings the same number of blocks");
+    static_assert(staged_tile_bytes<D, WM, WN>() >= 4 * 4 * wave_scratch_words<D, WM, WN>(), "epilogue scratch");
+    constexpr int PER = NBLK / 4;
+    constexpr int VM = PER * (S - 2);
+    static_assert(VM < 64, "vmcnt range");
+    constexpr int kWaitSteady = (VM & 15) | ((VM >> 4) << 14) | (7 << 4);   // vmcnt(VM) lgkmcnt(0)
+    constexpr int kWaitAll = 7 << 4;                                         // vmcnt(0) lgkmcnt(0)
+    constexpr int NMMA = D * (D + 1) / 2 * WM * WN;
+
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = w & 1, wn = w >> 1;
+    const int64_t rbA0 = (int64_t)tm * BA, rbB0 = (int64_t)tn * BB;
+    const int64_t planeA = g.RbA * g.Kb * kBlkBytes, planeB = g.RbB * g.Kb * kBlkBytes;
+    const int rbase = tm * 64 * WM + wm * 32 * WM, cbase = tn * 64 * WN + wn * 32 * WN;
+    int er_v, ec_v;
+    load_tile_exps(g, epi, lane, rbase, 32 * WM, cbase, 32 * WN, er_v, ec_v);
+
+    const int64_t kb0 = g.kb0, nk = g.kb1 - g.kb0;
+    const int8_t *src[PER];   // wave-uniform: block (w + 4 it) of the stage, at the pass's first block
+#pragma unroll
+    for (int it = 0; it < PER; it++) {
+        const int bc = w + 4 * it;
+        if (bc < D * BA) {
+            const int j = bc / BA, mb = bc % BA;
+            src[it] = g.SA + j * planeA + ((rbA0 + mb) * g.Kb + kb0) * kBlkBytes;
+        } else {
+            const int bb = bc - D * BA;
+            const int j = bb / BB, nb = bb % BB;
+            src[it] = g.SB + j * planeB + ((rbB0 + nb) * g.Kb + kb0) * kBlkBytes;
+        }
+    }
+    auto issue = [&](int stage, int64_t b, auto check) {   // block b of the pass
+        if (decltype(check)::value && b >= nk) return;
+#pragma unroll
+        for (int it = 0; it < PER; it++)
+            __builtin_amdgcn_global_load_lds((const void *)(src[it] + b * kBlkBytes + lane * 16),
+                                             (lds_void *)(smem + stage * F::STAGE + (w + 4 * it) * kBlkBytes), 16,
+                                             0, 0);
+    };
+    auto sync = [&](int64_t b, auto check) {   // block b and all before it in LDS, then barrier
+        if (!decltype(check)::value || b + S - 2 < nk)
+            __builtin_amdgcn_s_waitcnt(kWaitSteady);
+        else
+            __builtin_amdgcn_s_waitcnt(kWaitAll);
+        lds_barrier();
+    };
+    auto readf = [&](v4i (&af)[D][WM], v4i (&bf)[D][WN], int stage) {
+        const int8_t *st = smem + stage * F::STAGE;
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+#pragma unroll
+            for (int mi = 0; mi < WM; mi++)
+                af[j][mi] = *reinterpret_cast<const v4i *>(st + (j * BA + wm * WM + mi) * kBlkBytes + lane * 16);
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++)
+                bf[j][ni] = *reinterpret_cast<const v4i *>(st + (D * BA + j * BB + wn * WN + ni) * kBlkBytes + lane * 16);
+        }
+    };
+    v16i acc[D][WM][WN];
+#pragma unroll
+    for (int k = 0; k < D; k++)
+#pragma unroll
+        for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++) acc[k][mi][ni] = v16i{};
+    v4i af[2][D][WM], bf[2][D][WN];
+    auto step = [&](int s, int64_t b, auto check) {
+        const int f = s & 1;
+        __builtin_amdgcn_sched_barrier(0);
+        sync(b + 1, check);
+        issue(s, b + S, check);
+        readf(af[f ^ 1], bf[f ^ 1], (s + 1) % S);   // past the last block: never multiplied
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; i + j < D; j++)
+#pragma unroll
+                for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+                    for (int ni = 0; ni < WN; ni++)
+                        acc[i + j][mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[f][i][mi], bf[f][j][ni],
+                                                                                   acc[i + j][mi][ni], 0, 0, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, D * (WM + WN), 0);   // the LDS reads first,
+        __builtin_amdgcn_sched_group_barrier(0x008, NMMA, 0);            // then the MFMAs
+        __builtin_amdgcn_sched_barrier(0);
+    };
+
+    using Checked = std::integral_constant<bool, true>;
+    using Steady = std::integral_constant<bool, false>;
+#pragma unroll
+    for (int s = 0; s < S - 1; s++) issue(s, s, Checked{});
+    sync(0, Checked{});
+    issue(S - 1, S - 1, Checked{});
+    readf(af[0], bf[0], 0);
+    int64_t t = 0;
+    for (; t + 2 * S <= nk; t += S) {   // every step of the pass loads a block that exists
+#pragma unroll
+        for (int s = 0; s < S; s++) step(s, t + s, Steady{});
+    }
+    for (; t < nk; t += S) {
+#pragma unroll
+        for (int s = 0; s < S; s++)
+            if (t + s < nk) step(s, t + s, Checked{});
     }
     __syncthreads();
 

@@ -52,9 +52,12 @@ void k_grp_gemm(const GemmLaunch *__restrict__ probs, const int4 *__restrict__ i
     // consecutive items (tiles of one problem in bands of tile-rows) go to one XCD
     const int4 it = items[xcd_remap(blockIdx.x, gridDim.x)];
     const GemmLaunch g = probs[it.x];
-    constexpr int LDS = FAM == KERNEL_DIRECT ? direct_tile_bytes<D>() : lds_tile_bytes<D, WM, WN>();
+    constexpr int LDS = FAM == KERNEL_DIRECT   ? direct_tile_bytes<D>()
+                        : FAM == KERNEL_STAGED ? staged_tile_bytes<D, WM, WN>()
+                                               : lds_tile_bytes<D, WM, WN>();
     __shared__ __attribute__((aligned(1024))) int8_t smem[LDS];
-    if (FAM == KERNEL_DIRECT) gemm_direct_tile<D, WM, WN>(g, it.y, it.z, EPI_C, smem);
+    if constexpr (FAM == KERNEL_DIRECT) gemm_direct_tile<D, WM, WN>(g, it.y, it.z, EPI_C, smem);
+    else if constexpr (FAM == KERNEL_STAGED) gemm_staged_tile<D, WM, WN>(g, it.y, it.z, EPI_C, smem);
     else gemm_lds_tile<D, WM, WN>(g, it.y, it.z, EPI_C, smem);
 }
 
@@ -76,8 +79,8 @@ static hipError_t launch_grp_d(const GemmLaunch *probs, const int4 *items, int n
         case GCFG_D12:
             if constexpr (D <= 8) hipLaunchKernelGGL((k_grp_gemm<D, 1, 2, KERNEL_DIRECT>), grid, blk, 0, st, probs, items);
             break;
-        case GCFG_D22:
-            if constexpr (D <= 4) hipLaunchKernelGGL((k_grp_gemm<D, 2, 2, KERNEL_DIRECT>), grid, blk, 0, st, probs, items);
+        case GCFG_D22:   // 128 x 128 tiles: the staged tile body (four LDS stages)
+            if constexpr (D <= 4) hipLaunchKernelGGL((k_grp_gemm<D, 2, 2, KERNEL_STAGED>), grid, blk, 0, st, probs, items);
             break;
         case GCFG_L11: hipLaunchKernelGGL((k_grp_gemm<D, 1, 1, KERNEL_LDS>), grid, blk, 0, st, probs, items); break;
         case GCFG_L12: hipLaunchKernelGGL((k_grp_gemm<D, 1, 2, KERNEL_LDS>), grid, blk, 0, st, probs, items); break;
