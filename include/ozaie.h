@@ -55,7 +55,7 @@ extern "C" {
 #define OZAIE_OPT_KCHUNK 3      /* cap on contraction indices per pass (0 = automatic)      */
 #define OZAIE_OPT_KERNEL 4      /* 0 auto, 1 bin-group kernel, 2 fused with LDS copies,
                                    3 fused with direct register loads, 4 fused pipelined,
-                                   5 fused staged (4 LDS stages; d <= 4, else as 2)       */
+                                   5 fused staged (4-8 LDS stages; d <= 4, else as 2)     */
 #define OZAIE_OPT_PHASE_TIMING 5 /* 1: plans record HIP events around the split and GEMM    */
 #define OZAIE_OPT_CRT_GROUP_MODULI 6 /* moduli per workgroup of the grouped CRT modulus GEMM
                                    (0 = automatic per tile shape); read at group creation */

@@ -524,26 +524,33 @@ __device__ __forceinline__ void gemm_lds_tile(const GemmLaunch &g, int tm, int t
 
 // Staged variant (d <= 4, workgroup tile 128 x 128): the slice blocks go HBM -> LDS
 // (global_load_lds_dwordx4, each wave bringing NBLK / 4 whole 1 KiB blocks) into one of
-// four stages, so three contraction blocks are in flight, and the fragments of block b + 1
+// S stages, so S - 1 contraction blocks are in flight, and the fragments of block b + 1
 // are read from LDS while the triangle of block b runs:
 //   step b: wait for the wave's own loads of block b + 1 (loads complete in order, so the
 //           later blocks' loads stay in flight), barrier (block b + 1 is in LDS; every wave
-//           has read block b), issue block b + 4 into the stage block b used, read block
+//           has read block b), issue block b + S into the stage block b used, read block
 //           b + 1, multiply block b (its fragments are in registers since step b - 1).
 // The waits are builtins: the compiler's own wait placement does not read the inline-asm
 // barrier and would otherwise wait on the new reads before the first MFMA.  The steady
 // loop has no branch; a checked tail pass issues no load past the pass's last block and
 // waits for every load, so the epilogue's LDS scratch follows the last barrier safely.
 // Any number of contraction blocks per pass.
-constexpr int kStagedStages = 4;
+// Stages S: as many as fit in 160 KiB of LDS, even, at most 8 (d = 1, 2: 8; d = 3: 6; d = 4:
+// 4).  Measured against four stages: d = 1 8-22% faster, d = 2 and 3 unchanged.
+template <int D, int WM, int WN>
+__host__ __device__ constexpr int staged_stages() {
+    const int s = (160 * 1024 / FusedCfg<D, WM, WN>::STAGE) & ~1;
+    return s > 8 ? 8 : s;
+}
 
 template <int D, int WM, int WN>
-__host__ __device__ constexpr int staged_tile_bytes() { return kStagedStages * FusedCfg<D, WM, WN>::STAGE; }
+__host__ __device__ constexpr int staged_tile_bytes() { return staged_stages<D, WM, WN>() * FusedCfg<D, WM, WN>::STAGE; }
 
 template <int D, int WM, int WN>
 __device__ __forceinline__ void gemm_staged_tile(const GemmLaunch &g, int tm, int tn, int epi, int8_t *smem) {
     using F = FusedCfg<D, WM, WN>;
-    constexpr int BA = F::BA, BB = F::BB, NBLK = F::NBLK, S = kStagedStages;
+    constexpr int BA = F::BA, BB = F::BB, NBLK = F::NBLK, S = staged_stages<D, WM, WN>();
+    static_assert(S >= 4, "at least three blocks in flight");
     static_assert(NBLK % 4 == 0, "every wave brings the same number of blocks");
     static_assert(staged_tile_bytes<D, WM, WN>() >= 4 * 4 * wave_scratch_words<D, WM, WN>(), "epilogue scratch");
     constexpr int PER = NBLK / 4;

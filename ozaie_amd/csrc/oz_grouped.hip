@@ -79,7 +79,7 @@ static hipError_t launch_grp_d(const GemmLaunch *probs, const int4 *items, int n
         case GCFG_D12:
             if constexpr (D <= 8) hipLaunchKernelGGL((k_grp_gemm<D, 1, 2, KERNEL_DIRECT>), grid, blk, 0, st, probs, items);
             break;
-        case GCFG_D22:   // 128 x 128 tiles: the staged tile body (four LDS stages)
+        case GCFG_D22:   // 128 x 128 tiles: the staged tile body (six LDS stages at d = 3)
             if constexpr (D <= 4) hipLaunchKernelGGL((k_grp_gemm<D, 2, 2, KERNEL_STAGED>), grid, blk, 0, st, probs, items);
             break;
         case GCFG_L11: hipLaunchKernelGGL((k_grp_gemm<D, 1, 1, KERNEL_LDS>), grid, blk, 0, st, probs, items); break;
