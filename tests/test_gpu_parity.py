@@ -73,7 +73,7 @@ def test_bins_bit_exact(gpu, b, d, kernel):
     assert np.array_equal(got.astype(np.int64), ref)
 
 
-C_CASES = [(7, 8), (7, 3), (7, 7), (7, 9), (3, 29), (3, 26), (4, 18), (7, 12), (5, 19)]
+C_CASES = [(7, 8), (7, 3), (7, 7), (7, 9), (3, 29), (3, 26), (4, 18), (7, 12), (5, 19), (7, 1), (7, 2), (6, 4)]
 
 
 @pytest.mark.parametrize("b,d", C_CASES)
