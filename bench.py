@@ -484,7 +484,10 @@ def grouped(args, torch, np, oz, slices=(3, 7, 8, 9), count=512):
                       f" same {same} status {status}")
             grp.close()
         del ref
-        best = min(("multi", "persistent"), key=lambda m: entry[m]["ms"])
+        # a mode counts only if its run completed (no barrier overrun) and gave the same C
+        valid = [m for m in ("multi", "persistent")
+                 if entry[m]["barrier_error"] == 0 and entry[m]["identical_to_multi"]]
+        best = min(valid, key=lambda m: entry[m]["ms"])
         entry.update({"best_mode": best, "ms": entry[best]["ms"], "tflops": entry[best]["tflops"],
                       "speedup": entry[best]["speedup"],
                       "speedup_vs_native_graph": entry[best]["speedup_vs_native_graph"]})

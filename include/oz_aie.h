@@ -12,8 +12,10 @@
 
 namespace oz_aie {
 
-// include/oz_aie.h:82 of the reference, plus this library's b = 7 presets.
-enum Mode { accurate = 0, fast = 1, fast3 = 2, accurate7 = 3 };
+// include/oz_aie.h:82 of the reference (accurate, fast), plus this library's b = 7 presets
+// and its CRT mode (16 int8 residue GEMMs, exact integer product of the scaled operands;
+// no reference counterpart).
+enum Mode { accurate = 0, fast = 1, fast3 = 2, accurate7 = 3, crt = 4 };
 
 // Replaces include/oz_aie.h:182 of the reference.  Same status codes: 0 OK, 1 rejected
 // dimensions (only with OZAIE_OPT_STRICT, otherwise any M, N, K >= 1 is accepted); the

@@ -155,9 +155,10 @@ int32_t ozaie_group_query(ozaie_group group, int32_t *out4);
 /* Tile configuration chosen for each problem (-1 in the per-problem fallback).        */
 int32_t ozaie_group_configs(ozaie_group group, int32_t *cfg_out);
 /* Waits for the group's last run and reports (then clears) the persistent kernel's error
- * flag: 1 if a grid-barrier wait exceeded its bound (results of that run are invalid;
- * this happens only when the grid could not be resident at once).  0 otherwise, and
- * always 0 for the multi-launch modes.                                                 */
+ * report: 1 if a grid-barrier wait exceeded its bound in any run since the previous
+ * status call (the C of such a run is invalid; this happens only when the grid could not
+ * be resident at once).  Each launch starts with a clear barrier, so a failed run does
+ * not fail the runs after it.  0 otherwise, and always 0 for the multi-launch modes.   */
 int32_t ozaie_group_status(ozaie_group group, int32_t *error_out);
 int32_t ozaie_group_destroy(ozaie_group group);
 /* One-shot grouped product without a graph; blocking on `stream`.                     */

@@ -42,6 +42,10 @@ ROUND_TRUNCATE = 0
 ROUND_NEAREST = 1
 
 
+# kernel ids of ozaie_plan_info / ozaie_plan_query (out[4])
+KERNEL_NAMES = ("fused", "group", "crt")
+
+
 class Mode(enum.IntEnum):
     """oz_aie::Mode (include/oz_aie.h:82) plus the int8-rate presets of this library."""
     accurate = 0   # reference table: b = 3, d = 29 .. 25 by log2 K (b = 4, d = 18 at 2^14+)
@@ -215,7 +219,10 @@ def ozAIE_setup_slices(M, N, K, bits, slices):
 
 def ozAIE_DGEMM(M, N, K, A, B, C):
     """C = A @ B with the configuration of the last setup; A, B, C are contiguous float64
-    numpy arrays (host) or torch tensors (host or device).  Returns the status code."""
+    numpy arrays (host) or torch tensors (host or device).  Returns the status code.
+    Blocking, like the reference: work already queued on torch's current stream (for
+    example the producer of a device operand) completes before the library starts."""
+    on_device = False
     for x in (A, B, C):
         if isinstance(x, np.ndarray):
             if x.dtype != np.float64 or not x.flags["C_CONTIGUOUS"]:
@@ -223,6 +230,10 @@ def ozAIE_DGEMM(M, N, K, A, B, C):
         else:
             if str(x.dtype) != "torch.float64" or not x.is_contiguous():
                 raise ValueError("torch operands must be contiguous float64")
+            on_device |= bool(x.is_cuda)
+    if on_device:
+        import torch
+        torch.cuda.current_stream().synchronize()
     return lib().ozAIE_DGEMM(int(M), int(N), int(K), _ptr(A), _ptr(B), _ptr(C))
 
 
@@ -238,7 +249,7 @@ def plan_info():
     out = np.zeros(5, dtype=np.int32)
     _check(lib().ozaie_plan_info(_ptr(out)), "ozaie_plan_info")
     return dict(bits=int(out[0]), slices=int(out[1]), kchunk=int(out[2]), passes=int(out[3]),
-                kernel=("fused", "group")[int(out[4])])
+                kernel=KERNEL_NAMES[int(out[4])])
 
 
 def release():
@@ -270,7 +281,7 @@ class Plan:
         out = np.zeros(5, dtype=np.int32)
         _check(lib().ozaie_plan_query(self._h, _ptr(out)), "ozaie_plan_query")
         return dict(bits=int(out[0]), slices=int(out[1]), kchunk=int(out[2]), passes=int(out[3]),
-                    kernel=("fused", "group", "crt")[int(out[4])])
+                    kernel=KERNEL_NAMES[int(out[4])])
 
     def __call__(self, A, B, C=None, transA=False, transB=False, stream=None):
         """C = op(A) @ op(B) on device float64 tensors (asynchronous on `stream`)."""
@@ -414,7 +425,9 @@ class Group:
         self._h = h
         self.bits, self.slices = int(bits), int(slices)
 
-    def __call__(self, As, Bs, Cs, stream=None):
+    def __call__(self, As, Bs, Cs, stream=None, check=False):
+        """Enqueue every product on `stream`.  check=True waits for the run and raises if
+        the persistent kernel reported a barrier overrun (its C would be invalid)."""
         cnt = len(self.shapes)
         if not (len(As) == len(Bs) == len(Cs) == cnt):
             raise ValueError("need one A, B and C per problem")
@@ -429,6 +442,8 @@ class Group:
         pc = np.array([c.data_ptr() for c in Cs], dtype=np.uint64)
         _check(lib().ozaie_group_run(self._h, _ptr(pa), _ptr(pb), _ptr(pc), _stream_handle(stream)),
                "ozaie_group_run")
+        if check and self.status():
+            raise OzaieError(ERR_INCOMPLETE, "persistent grouped run (grid barrier overran its bound)")
         return Cs
 
     def query(self):

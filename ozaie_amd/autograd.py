@@ -24,7 +24,10 @@ def _row_major(x):
 
 
 def _plan(M, N, K, bits, slices, rounding, crt=False):
-    key = (M, N, K, bits, slices, rounding, crt, torch.cuda.current_device())
+    # A plan owns its intermediate buffers (exponents, slices or residues, partial sums), so
+    # products enqueued on different streams must not share one: the key holds the stream.
+    key = (M, N, K, bits, slices, rounding, crt, torch.cuda.current_device(),
+           torch.cuda.current_stream().cuda_stream)
     p = _plans.get(key)
     if p is None:
         p = Plan(M, N, K, bits, slices, rounding, crt=crt)

@@ -67,6 +67,34 @@ def build(verbose=False, jobs=2):
     return LIB
 
 
+EXAMPLES_BIN = os.path.join(HERE, "..", "examples", "bin")
+REFERENCE_EXAMPLE = "/root/reference/example/oztest.cpp"
+
+
+def build_examples(verbose=False):
+    """Compile the consumers of the reference-compatible C++ API against include/oz_aie.h
+    and libozaie.so: examples/oz_example.cpp, and the reference's own example/oztest.cpp
+    read in place where the reference is mounted (compiled unchanged; the binary travels
+    with the tree, the source does not).  Outputs go to examples/bin/ with an rpath to
+    ozaie_amd/."""
+    os.makedirs(EXAMPLES_BIN, exist_ok=True)
+    srcs = {"oz_example": os.path.join(HERE, "..", "examples", "oz_example.cpp")}
+    if os.path.exists(REFERENCE_EXAMPLE):
+        srcs["oztest"] = REFERENCE_EXAMPLE
+    built = []
+    for name, src in srcs.items():
+        out = os.path.join(EXAMPLES_BIN, name)
+        cmd = ["g++", "-std=c++17", "-O1", "-I" + os.path.join(HERE, "..", "include"), src, "-o", out + ".tmp",
+               "-L" + HERE, "-lozaie", "-Wl,-rpath,$ORIGIN/../../ozaie_amd"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+        os.replace(out + ".tmp", out)
+        built.append(out)
+    return built
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_examples(verbose=True)
     sys.exit(0)

@@ -48,6 +48,24 @@ int set_hip_error(hipError_t e, const char *where) {
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
+// Makes `dev` current for the guard's lifetime and restores the caller's device on every
+// return path.
+struct DeviceGuard {
+    int prev = -1;
+    hipError_t enter(int dev) {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur == dev) return hipSuccess;
+        e = hipSetDevice(dev);
+        if (e == hipSuccess) prev = cur;
+        return e;
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
 // Reference mode table, include/oz_aie.h:60-80 (log2 K -> (b, d)).
 const int kAccurate[8][2] = {{3, 29}, {3, 28}, {3, 27}, {3, 27}, {3, 26}, {3, 25}, {3, 25}, {4, 18}};
 
@@ -292,7 +310,7 @@ int plan_alloc(ozaie_plan p, bool staging) {
         HIPCHK(hipMalloc(&p->dA, sizeof(double) * (size_t)p->M * p->K));
         HIPCHK(hipMalloc(&p->dB, sizeof(double) * (size_t)p->K * p->N));
         HIPCHK(hipMalloc(&p->dC, sizeof(double) * (size_t)p->M * p->N));
-        HIPCHK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&p->stream, hipStreamDefault));
     }
     if (g_opt_timing) {
         for (auto &e : p->ev) HIPCHK(hipEventCreate(&e));
@@ -393,16 +411,14 @@ int plan_execute(ozaie_plan p, int transA, int transB, const double *A, int64_t 
     if (!A || !B || !C) { g_last_error = "null pointer"; return OZAIE_ERR_ARG; }
     const int64_t minlda = transA ? p->M : p->K, minldb = transB ? p->K : p->N;
     if (lda < minlda || ldb < minldb || ldc < p->N) { g_last_error = "leading dimension too small"; return OZAIE_ERR_ARG; }
-    int cur = 0;
-    HIPCHK(hipGetDevice(&cur));
-    if (cur != p->device) HIPCHK(hipSetDevice(p->device));
+    DeviceGuard dg;
+    HIPCHK(dg.enter(p->device));
     if (p->timed) HIPCHK(hipEventRecord(p->ev[0], st));
     int rc = run_split(p, view_A(p, transA, A, lda), view_B(p, transB, B, ldb), st);
     if (!rc && p->timed) HIPCHK(hipEventRecord(p->ev[1], st));
     if (!rc) rc = run_passes(p, C, ldc, EPI_C, nullptr, st);
     if (!rc && p->timed && !p->crt) HIPCHK(hipEventRecord(p->ev[2], st));
     if (!rc && p->timed) HIPCHK(hipEventRecord(p->ev[3], st));
-    if (cur != p->device) (void)hipSetDevice(cur);
     return rc;
 }
 
@@ -431,8 +447,9 @@ int32_t ozAIE_setup(int M, int N, int K, int mode) {
         std::fprintf(stderr, "ozAIE_setup: dimensions must be >= 1\n");
         return OZAIE_ERR_DIMS;
     }
+    // int(std::log2(K)) as the reference computes it: K need not be a power of two
     int log2k = (int)std::log2((double)K);
-    if (g_opt_strict && (log2k < 7 || log2k > 11 || M != K || N != K || (K & (K - 1)))) {
+    if (g_opt_strict && (log2k < 7 || log2k > 11 || M != K || N != K)) {
         std::fprintf(stderr, "Only square matrices are currently supported\n");
         std::fprintf(stderr, "Single side dimension : power of 2 between 128 and 2048 (inclusive)\n");
         return OZAIE_ERR_DIMS;
@@ -469,9 +486,10 @@ int32_t ozAIE_DGEMM(int M, int N, int K, const double *A, const double *B, doubl
     if (!p) { g_last_error = "ozAIE_DGEMM before ozAIE_setup"; return OZAIE_ERR_NOSETUP; }
     if (M != p->M || N != p->N || K != p->K) { g_last_error = "dimensions differ from setup"; return OZAIE_ERR_MISMATCH; }
     if (!A || !B || !C) { g_last_error = "null pointer"; return OZAIE_ERR_ARG; }
-    int cur = 0;
-    HIPCHK(hipGetDevice(&cur));
-    if (cur != p->device) HIPCHK(hipSetDevice(p->device));
+    DeviceGuard dg;
+    HIPCHK(dg.enter(p->device));
+    // The plan's stream is a blocking stream: it starts after the work already queued on the
+    // legacy default stream (fills or producers of device-resident A, B, C).
     hipStream_t st = p->stream;
     const bool devA = is_device_ptr(A), devB = is_device_ptr(B), devC = is_device_ptr(C);
     const double *a = A, *bb = B;
@@ -483,7 +501,6 @@ int32_t ozAIE_DGEMM(int M, int N, int K, const double *A, const double *B, doubl
     if (!devC) HIPCHK(hipMemcpyAsync(C, p->dC, sizeof(double) * (size_t)M * N, hipMemcpyDeviceToHost, st));
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) { set_hip_error(e, "ozAIE_DGEMM"); return OZAIE_ERR_INCOMPLETE; }
-    if (cur != p->device) (void)hipSetDevice(cur);
     return OZAIE_OK;
 }
 

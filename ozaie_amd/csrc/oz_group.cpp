@@ -830,12 +830,13 @@ int32_t ozaie_group_status(ozaie_group g, int32_t *error_out) {
     if (hipGetDevice(&cur) != hipSuccess) return OZAIE_ERR_HIP;
     if (cur != g->device) (void)hipSetDevice(g->device);
     int rc = OZAIE_OK;
+    // control words: arrive, done, error (this launch), reported (sticky until read here)
     int32_t ctl[4] = {0, 0, 0, 0};
     hipError_t e = hipStreamSynchronize(g->last_stream);
     if (e == hipSuccess) e = hipMemcpy(ctl, g->PL.ctl, sizeof(ctl), hipMemcpyDeviceToHost);
-    if (e == hipSuccess && ctl[2]) e = hipMemset(static_cast<char *>(g->PL.ctl) + 2 * sizeof(int32_t), 0, sizeof(int32_t));
+    if (e == hipSuccess && ctl[3]) e = hipMemset(static_cast<char *>(g->PL.ctl) + 3 * sizeof(int32_t), 0, sizeof(int32_t));
     if (e != hipSuccess) rc = set_error(e, "ozaie_group_status");
-    *error_out = ctl[2];
+    *error_out = ctl[3];
     if (cur != g->device) (void)hipSetDevice(cur);
     return rc;
 }

@@ -262,10 +262,11 @@ __device__ __forceinline__ void wave_rowblock(const GroupOperand &o, int64_t rb,
 }
 
 struct PersistCtl {
-    int arrive;   // workgroups through the phase barrier
-    int done;     // workgroups finished (the last one resets both counters)
-    int error;    // sticky: a barrier wait exceeded its bound
-    int pad[61];
+    int arrive;     // workgroups through the phase barrier
+    int done;       // workgroups finished (the last one resets the launch's words)
+    int error;      // this launch: a barrier wait exceeded its bound (every waiter leaves)
+    int reported;   // sticky across launches until ozaie_group_status reads and clears it
+    int pad[60];
 };
 
 constexpr int kSpinLimit = 1 << 20;   // barrier polls (each a device-coherent load + sleep)
@@ -441,11 +442,15 @@ void k_grp_persistent(const GroupOperand *__restrict__ ops, const int4 *__restri
         }
     }
 
-    // ---- exit: the last workgroup resets the counters for the next launch
+    // ---- exit: the last workgroup hands this launch's error to the sticky report word and
+    // resets the launch's words, so a failed launch does not fail the next one
     __syncthreads();
     if (threadIdx.x == 0) {
         const int old = __hip_atomic_fetch_add(&ctl->done, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (old == G - 1) {
+            if (__hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                __hip_atomic_store(&ctl->reported, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctl->error, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&ctl->arrive, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&ctl->done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }

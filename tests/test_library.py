@@ -57,9 +57,14 @@ def test_strict_mode_reproduces_reference_validation():
         assert oz.ozAIE_setup(256, 512, 128, oz.Mode.accurate) == oz.ERR_DIMS
         assert oz.ozAIE_setup(64, 64, 64, oz.Mode.accurate) == oz.ERR_DIMS
         assert oz.ozAIE_setup(4096, 4096, 4096, oz.Mode.fast) == oz.ERR_DIMS
-        assert oz.ozAIE_setup(384, 384, 384, oz.Mode.fast) == oz.ERR_DIMS
+        assert oz.ozAIE_setup(127, 127, 127, oz.Mode.fast) == oz.ERR_DIMS
+        # int(std::log2(K)) in [7, 11] with M == N == K: the reference accepts K that is not
+        # a power of two (include/oz_aie.h:184-191), so strict mode does too
+        assert oz.ozAIE_setup(384, 384, 384, oz.Mode.fast) in (oz.OK, oz.ERR_NODEVICE)
+        assert oz.ozAIE_setup(4095, 4095, 4095, oz.Mode.accurate) in (oz.OK, oz.ERR_NODEVICE)
     finally:
         oz.set_option(oz.OPT_STRICT, 0)
+        oz.release()
 
 
 def test_dgemm_before_setup():
