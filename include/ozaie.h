@@ -118,6 +118,22 @@ int32_t ozaie_dgemm(ozaie_plan plan, int transA, int transB, const double *A, in
  * of that call.                                                                        */
 int32_t ozaie_plan_phase_times(ozaie_plan plan, float *out3);
 
+/* ---- allocation-free product.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The same product as ozaie_plan_create(M, N, K, bits, slices,
+ * flags) followed by ozaie_dgemm(...) on `stream`, bit-identical to it, but every
+ * intermediate (exponents, slices or residues, 128-bit partial sums, residue products)
+ * lives in a static device workspace of OZAIE_NOALLOC_BYTES per GPU, held until the
+ * process ends; no call allocates or frees memory.  A product whose intermediates exceed
+ * the workspace is rejected with OZAIE_ERR_NOMEM (ozaie_last_error gives the sizes).
+ * The caller passes A, B, C (device pointers).  All calls on one GPU share its workspace:
+ * calls that overlap on one GPU (other streams, other threads) race; calls on one stream
+ * are ordered.  Asynchronous on `stream`.                                               */
+#define OZAIE_NOALLOC_BYTES 268435456LL   /* 256 MiB */
+int32_t ozaie_dgemm_noalloc(int64_t M, int64_t N, int64_t K, int bits, int slices, int flags, int transA,
+                            int transB, const double *A, int64_t lda, const double *B, int64_t ldb, double *C,
+                            int64_t ldc, void *stream);
+
 /* ---- grouped products (many independent C_i = A_i B_i of uneven shape)
  * A_i (M_i x K_i), B_i (K_i x N_i), C_i (M_i x N_i): row-major, contiguous, device pointers.
  * All problems share b and d.  Slice counts 3, 7, 8, 9 run as one exponent launch, one

@@ -64,7 +64,10 @@ EXPORTS = [
     "ozaie_exact_entries", "ozaie_group_create", "ozaie_group_run", "ozaie_group_query",
     "ozaie_group_configs", "ozaie_group_status", "ozaie_group_destroy", "ozaie_dgemm_grouped", "ozaie_autotune",
     "ozaie_tune_get", "ozaie_tune_set", "ozaie_autotune_crt", "ozaie_tune_get_crt", "ozaie_tune_set_crt",
+    "ozaie_dgemm_noalloc",
 ]
+
+NOALLOC_BYTES = 256 << 20   # OZAIE_NOALLOC_BYTES: the static workspace of ozaie_dgemm_noalloc
 
 
 class OzaieError(RuntimeError):
@@ -161,6 +164,8 @@ def _declare(L):
     L.ozaie_tune_get_crt.restype = i32
     L.ozaie_tune_set_crt.argtypes = [c_int, P]
     L.ozaie_tune_set_crt.restype = i32
+    L.ozaie_dgemm_noalloc.argtypes = [i64, i64, i64, c_int, c_int, c_int, c_int, c_int, P, i64, P, i64, P, i64, P]
+    L.ozaie_dgemm_noalloc.restype = i32
     L.ozaie_last_error.argtypes = []
     L.ozaie_last_error.restype = ctypes.c_char_p
     L.ozaie_version.argtypes = []
@@ -320,6 +325,24 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def dgemm_noalloc(A, B, C, bits=7, slices=8, rounding=ROUND_TRUNCATE, crt=False, transA=False, transB=False,
+                  stream=None):
+    """C = op(A) @ op(B) through ozaie_dgemm_noalloc: no device allocation, intermediates
+    in the library's static per-GPU workspace (NOALLOC_BYTES).  Float64 device tensors,
+    row-major with unit stride along the last axis; C is written in place.  Asynchronous
+    on `stream`; calls overlapping on one GPU race (they share the workspace)."""
+    M, N = C.shape
+    K = A.shape[0] if transA else A.shape[1]
+    for x in (A, B, C):
+        if str(x.dtype) != "torch.float64" or not x.is_cuda or x.stride(1) != 1:
+            raise ValueError("operands must be float64 device tensors with unit stride along their last axis")
+    flags = (1 if rounding == ROUND_NEAREST else 0) | (4 if crt else 0)
+    _check(lib().ozaie_dgemm_noalloc(int(M), int(N), int(K), int(bits), int(slices), flags, int(transA),
+                                     int(transB), _ptr(A), A.stride(0), _ptr(B), B.stride(0), _ptr(C),
+                                     C.stride(0), _stream_handle(stream)), "ozaie_dgemm_noalloc")
+    return C
 
 
 def matmul(A, B, bits=7, slices=8, rounding=ROUND_TRUNCATE):

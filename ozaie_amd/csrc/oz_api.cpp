@@ -432,6 +432,40 @@ bool is_device_ptr(const void *ptr) {
 
 ozaie_plan g_plan = nullptr;
 
+// ---- workspace of the allocation-free entry point (ozaie_dgemm_noalloc)
+// One static device array per GPU the library runs on, held until the process ends.  In
+// host code `scratch` only names the symbol: its device address (per device) comes from
+// hipGetSymbolAddress on the first call on that device and is the only form passed to
+// kernels or copies.
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) char scratch[OZAIE_NOALLOC_BYTES];
+char *g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+int noalloc_workspace(int dev, char **out) {
+    if (dev < 0 || dev >= kNoAllocMaxDevices) { g_last_error = "device index beyond the workspace table"; return OZAIE_ERR_ARG; }
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) { g_last_error = "hipGetSymbolAddress returned no address"; return OZAIE_ERR_HIP; }
+        g_scratch_ptr[dev] = static_cast<char *>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return OZAIE_OK;
+}
+
+// Bytes of the plan's intermediates in the workspace (each buffer 256-byte aligned).
+int64_t noalloc_bytes(const ozaie_plan p) {
+    int64_t off = 0;
+    auto take = [&](int64_t bytes) { off += round_up(bytes, 256); };
+    take((int64_t)p->d * p->Mp * p->Kp);
+    take((int64_t)p->d * p->Np * p->Kp);
+    take(4 * p->Mp);
+    take(4 * p->Np);
+    if (!p->crt && (int64_t)p->nchunks * p->ngroups > 1) take(16 * p->M * p->N);
+    if (p->crt) take((int64_t)p->d * p->Mp * p->Np);
+    return off;
+}
+
 int setup_global(int64_t M, int64_t N, int64_t K, int b, int d) {
     if (g_plan) { plan_free(g_plan); g_plan = nullptr; }
     return plan_new(&g_plan, M, N, K, b, d, g_opt_rounding, true);
@@ -583,6 +617,43 @@ int32_t ozaie_dgemm(ozaie_plan plan, int transA, int transB, const double *A, in
                     const double *B, int64_t ldb, double *C, int64_t ldc, void *stream) {
     if (!plan) return OZAIE_ERR_ARG;
     return plan_execute(plan, transA, transB, A, lda, B, ldb, C, ldc, (hipStream_t)stream);
+}
+
+int32_t ozaie_dgemm_noalloc(int64_t M, int64_t N, int64_t K, int bits, int slices, int flags, int transA,
+                            int transB, const double *A, int64_t lda, const double *B, int64_t ldb, double *C,
+                            int64_t ldc, void *stream) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc = ensure_device();
+    if (rc) return rc;
+    // a plan on the stack: the same configuration as ozaie_plan_create, no device buffers
+    ozaie_plan_s p;
+    const int rounding = (flags & OZAIE_PLAN_NEAREST) ? OZAIE_ROUND_NEAREST : OZAIE_ROUND_TRUNCATE;
+    rc = (flags & OZAIE_PLAN_CRT) ? plan_configure_crt(&p, M, N, K, bits, slices, rounding)
+                                  : plan_configure(&p, M, N, K, bits, slices, rounding);
+    if (rc) return rc;
+    const int64_t need = noalloc_bytes(&p);
+    if (need > (int64_t)OZAIE_NOALLOC_BYTES) {
+        g_last_error = "ozaie_dgemm_noalloc: this product needs " + std::to_string(need) +
+                       " bytes of intermediates, the static workspace holds " +
+                       std::to_string((long long)OZAIE_NOALLOC_BYTES);
+        return OZAIE_ERR_NOMEM;
+    }
+    HIPCHK(hipGetDevice(&p.device));
+    char *ws = nullptr;
+    rc = noalloc_workspace(p.device, &ws);
+    if (rc) return rc;
+    int64_t off = 0;
+    auto take = [&](int64_t bytes) { char *q = ws + off; off += round_up(bytes, 256); return q; };
+    p.SA = reinterpret_cast<int8_t *>(take((int64_t)p.d * p.Mp * p.Kp));
+    p.SB = reinterpret_cast<int8_t *>(take((int64_t)p.d * p.Np * p.Kp));
+    p.Er = reinterpret_cast<int32_t *>(take(4 * p.Mp));
+    p.Ec = reinterpret_cast<int32_t *>(take(4 * p.Np));
+    if (!p.crt && (int64_t)p.nchunks * p.ngroups > 1) p.T = reinterpret_cast<__int128 *>(take(16 * p.M * p.N));
+    if (p.crt) p.R = reinterpret_cast<uint32_t *>(take((int64_t)p.d * p.Mp * p.Np));
+    // Every intermediate is fully written before it is read (exponent arrays that are
+    // combined with atomicMax are cleared by a fill kernel first), so the previous call's
+    // contents never leak into this one.
+    return plan_execute(&p, transA, transB, A, lda, B, ldb, C, ldc, (hipStream_t)stream);
 }
 
 int32_t ozaie_plan_phase_times(ozaie_plan plan, float *out3) {
