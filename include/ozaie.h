@@ -66,8 +66,8 @@ extern "C" {
 
 /* Replaces oz_aie::ozAIE_setup (include/oz_aie.h:182-300 of the reference).
  * Picks (b, d) for `mode` from K and prepares the device buffers.  Any M, N, K >= 1 is
- * accepted unless OZAIE_OPT_STRICT is set (then only square power-of-two K in
- * [128, 2048], like the reference).  Returns OZAIE_OK or an error code.               */
+ * accepted unless OZAIE_OPT_STRICT is set (then only M == N == K with int(log2 K) in
+ * [7, 11], as the reference checks).  Returns OZAIE_OK or an error code.              */
 int32_t ozAIE_setup(int M, int N, int K, int mode);
 
 /* Explicit slice configuration: b bits per slice (1..7), d slices (1..32), b*d <= 96. */
