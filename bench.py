@@ -3,13 +3,15 @@
 native FP64 DGEMM on the same GPU, at matched maximum relative error.
 
 Workload (one "step" = one C = A B over device-resident FP64 inputs):
-  square DGEMM M = N = K = 4096 (--size), entries (U(0,1) - 0.5) exp(phi N(0,1)), phi = 1,
-  seeded synthetic data; b = 7 split-scheme presets with 3 (fast), 7 (accurate), 8 and 9
-  slices, and the CRT mode with 16 moduli (exact integer product of the scaled operands).
-Errors are max / median |C - C*| / |C*| on a sampled 256 x 256 block of C against the
+  square DGEMM M = N = K = 8192 (--size), entries (U(0,1) - 0.5) exp(phi N(0,1)), phi = 1,
+  seeded synthetic data; b = 7 split-scheme presets (the reference rule) with 3 (fast),
+  7 (accurate), 8 (the reference's fast mode) and 9 slices, and the CRT mode with 16
+  moduli (exact integer product of the scaled operands; no reference counterpart).
+Errors are max / median |C - C*| / |C*| on a sampled 128 x 128 block of C against the
 correctly rounded exact product (libozaie's exact-entry meter).  `value` is the
 effective TFLOP/s (2 M N K / time) of the fastest preset whose maximum error is no
-worse than native DGEMM's on the same entries ("matched").
+worse than native DGEMM's on the same entries ("matched"); config.reference_rule_* give
+the fastest matched preset of the reference rule itself.
 
 The JSON line also carries:
   roofline      the GEMM kernel of the matched preset: int8 ops per launch (d(d+1)/2 * 2 M N K
@@ -18,12 +20,15 @@ The JSON line also carries:
                 against the 5.0 POP/s dense int8 peak;
                 traffic = HBM bytes per launch from the rocprofv3 FETCH/WRITE summaries in
                 profiles/ when present
-  cpu_baseline  the oracle's CPU path of the matched preset (the port of the reference
-                rule, or of the CRT mode) on a row sample of the same workload, on this
-                host's cores
+  cpu_baseline  the oracle's OpenMP restatement of the reference rule (b = 7, the matched
+                split preset) on a row sample of the same workload, on this host's cores;
+                the CRT-mode port, the host BLAS DGEMM and the reference's single-threaded
+                host phases beside it
   host_pointer  the matched preset through the reference-style entry point with host arrays
                 (PCIe copies of A, B and C inside the timed call); not `value`
-  target_8192   the same measurements at 8192^3
+  square_4096   the same measurements at 4096^3 (256 x 256 error block)
+  accurate_mode the reference's Mode.accurate (b = 3) at 256 x 512 x 128, 2048^3, 4096^3
+  long_k, grouped_512, autograd_2048: the other benchmark configurations
 """
 import argparse
 import json
@@ -44,16 +49,17 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--size", type=int, default=8192)
     ap.add_argument("--phi", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=12445)
     ap.add_argument("--slices", default="3,7,8,9", help="b=7 slice counts to measure")
-    ap.add_argument("--crt", default="16", help="CRT-mode modulus counts to measure ('' for none)")
+    ap.add_argument("--crt", default="16,17",
+                    help="CRT-mode modulus counts to measure ('' for none; 17: 192-bit rebuild, 3 more bits)")
     ap.add_argument("--crt-long", default="16,17",
                     help="CRT-mode modulus counts for the long-K section (17: 192-bit rebuild, more bits)")
-    ap.add_argument("--block", type=int, default=256, help="side of the sampled error block")
+    ap.add_argument("--block", type=int, default=128, help="side of the sampled error block")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-target", action="store_true", help="skip the 8192^3 section")
+    ap.add_argument("--no-target", action="store_true", help="skip the 4096^3 section")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the long-K, grouped and autograd sections")
     ap.add_argument("--quiet", action="store_true")
@@ -210,18 +216,15 @@ def load_traffic(d, n):
         return None
 
 
-def cpu_baseline(args, np, Ah, Bh, d, n, crt=False):
-    import oracle
-    run = (lambda A_: oracle.crt_dgemm(A_, Bh, d, 1)) if crt else (lambda A_: oracle.ozaki_dgemm(A_, Bh, 7, d))
-    # bounded sample: the first rows of C (the whole of B is split, as the GPU does); probes
-    # of 64 and 128 rows separate the fixed cost (B's split) from the per-row cost and size
-    # the sample to about OZAIE_CPU_SECONDS (default 12) of CPU work
-    target = float(os.environ.get("OZAIE_CPU_SECONDS", "12"))
-    ms = max(8, min(n, int(os.environ.get("OZAIE_CPU_ROWS", "64"))))
+def _timed_rows(np, Ah, n, run, target, rows0):
+    """Rows of C for about `target` seconds of `run` (the whole of B is prepared, as on the
+    GPU): probes of rows0 and 2 rows0 rows separate the fixed cost (B's split) from the
+    per-row cost.  Returns (rows, seconds)."""
+    ms = max(8, min(n, rows0))
     t0 = time.time()
     run(np.ascontiguousarray(Ah[:ms]))
     t = time.time() - t0
-    if "OZAIE_CPU_ROWS" not in os.environ and t < target and 2 * ms <= n:
+    if t < target and 2 * ms <= n:
         t0 = time.time()
         run(np.ascontiguousarray(Ah[:2 * ms]))
         t2 = time.time() - t0
@@ -232,16 +235,31 @@ def cpu_baseline(args, np, Ah, Bh, d, n, crt=False):
         t0 = time.time()
         run(A_s)
         t = time.time() - t0
-    A_s = np.ascontiguousarray(Ah[:ms])
+    return ms, t
+
+
+def cpu_baseline(args, np, Ah, Bh, n, split_d, crt_mods):
+    """The host Ozaki path on this machine's cores: the oracle's OpenMP restatement of the
+    reference rule (b = 7, `split_d` slices: split, int8 slice products, exact bin sum,
+    truncating conversion) timed on a row sample of the workload (`value`); the CRT-mode
+    port on its own sample beside it; the host BLAS DGEMM; and the reference's single-
+    threaded host work (paired-int64 conversion, naive DGEMM).  Flat keys only."""
+    import oracle
+    target = float(os.environ.get("OZAIE_CPU_SECONDS", "12"))
+    ms, t = _timed_rows(np, Ah, n, lambda A_: oracle.ozaki_dgemm(A_, Bh, 7, split_d), target, 64)
     rec = {"value": 2.0 * ms * n * n / t / 1e12, "unit": "TFLOP/s (effective FP64)",
            "cores": oracle.num_threads(), "kind": "port",
-           "sample": (f"{ms} of {n} rows of C, CRT mode with {d} moduli, full residue split of B; {t:.2f} s"
-                      if crt else f"{ms} of {n} rows of C, b=7 d={d}, full split of B; {t:.2f} s")}
-    # host BLAS DGEMM on the same row sample, for reference
+           "sample": f"reference rule b=7 d={split_d}: {ms} of {n} rows of C, full split of B; {t:.2f} s",
+           "scheme": "split", "bits": 7, "slices": split_d}
+    if crt_mods:
+        mc, tc = _timed_rows(np, Ah, n, lambda A_: oracle.crt_dgemm(A_, Bh, crt_mods, 1), target / 2, 16)
+        rec["crt_port_value"] = 2.0 * mc * n * n / tc / 1e12
+        rec["crt_port_sample"] = f"CRT mode, {crt_mods} moduli: {mc} of {n} rows of C, residues of all of B; {tc:.2f} s"
+    A_s = np.ascontiguousarray(Ah[:ms])
     t0 = time.time()
     A_s @ Bh
-    t = time.time() - t0
-    rec["host_dgemm_tflops"] = 2.0 * ms * n * n / t / 1e12
+    th = time.time() - t0
+    rec["host_dgemm_tflops"] = 2.0 * ms * n * n / th / 1e12
     try:
         cfg = np.show_config(mode="dicts")
         rec["host_blas"] = cfg["Build Dependencies"]["blas"]["name"]
@@ -249,9 +267,13 @@ def cpu_baseline(args, np, Ah, Bh, d, n, crt=False):
         rec["host_blas"] = "numpy default"
     rec["cpu_model"] = cpu_model()
     try:
-        rec["reference_host_phases"] = reference_host_phases(np, n)
+        ph = reference_host_phases(np, n)
+        rec["reference_host_conversion_s"] = ph["conversion"]["seconds"]
+        rec["reference_host_conversion_elements"] = ph["conversion"]["elements"]
+        rec["reference_host_conversion_threads"] = 1
+        rec["reference_naive_dgemm_256x512x128_gflops"] = ph["naive_dgemm_256x512x128"]["gflops"]
     except Exception as e:
-        rec["reference_host_phases"] = {"error": repr(e)}
+        rec["reference_host_phases_error"] = repr(e)
     return rec
 
 
@@ -557,6 +579,51 @@ def autograd_section(args, torch, np, oz, n=2048, slices=9):
     return rec
 
 
+def accurate_section(args, torch, np, oz):
+    """The reference's own default mode, Mode.accurate (b = 3, d = 29 .. 25 by log2 K,
+    include/oz_aie.h:60-70), on the repository example's shape 256 x 512 x 128 (phi = 0.5)
+    and on squares 2048^3 and 4096^3 (phi = 1): time, effective TFLOP/s, speed against
+    native DGEMM, error on a sampled block against the exact product."""
+    out = []
+    for (M, N, K, phi) in ((256, 512, 128, 0.5), (2048, 2048, 2048, 1.0), (4096, 4096, 4096, 1.0)):
+        g = torch.Generator(device="cuda:0")
+        g.manual_seed(args.seed)
+        A = (torch.rand(M, K, dtype=torch.float64, device="cuda:0", generator=g) - 0.5) * torch.exp(
+            phi * torch.randn(M, K, dtype=torch.float64, device="cuda:0", generator=g))
+        B = (torch.rand(K, N, dtype=torch.float64, device="cuda:0", generator=g) - 0.5) * torch.exp(
+            phi * torch.randn(K, N, dtype=torch.float64, device="cuda:0", generator=g))
+        C = torch.empty(M, N, dtype=torch.float64, device="cuda:0")
+        flop = 2.0 * M * N * K
+        t_nat = time_steps(torch, lambda: torch.matmul(A, B, out=C), 10, 5)
+        blk = min(64, M, N)
+        rows = np.repeat(np.arange(blk), blk)
+        cols = np.tile(np.arange(blk), blk)
+        exact = oz.exact_entries(A[:blk].cpu().numpy(), B[:, :blk].cpu().numpy(), rows, cols)
+        nat_err = rel_errors(np, C[:blk, :blk].cpu().numpy().ravel(), exact)
+        b, d = oz.mode_slices(oz.Mode.accurate, K)
+        oz.set_option(oz.OPT_PHASE_TIMING, 1)
+        p = oz.Plan(M, N, K, b, d)
+        oz.set_option(oz.OPT_PHASE_TIMING, 0)
+        t = time_steps(torch, lambda: p(A, B, C), 10, 5)
+        gemm = []
+        for _ in range(5):
+            p(A, B, C)
+            gemm.append(p.phase_times()[1])
+        err = rel_errors(np, C[:blk, :blk].cpu().numpy().ravel(), exact)
+        info = p.info()
+        p.close()
+        rec = {"M": M, "N": N, "K": K, "phi": phi, "bits": b, "slices": d, "ms": t, "tflops": flop / t / 1e9,
+               "native_ms": t_nat, "speedup": t_nat / t, "max_rel_err": err[0], "median_rel_err": err[1],
+               "native_max_rel_err": nat_err[0], "gemm_ms": float(np.mean(gemm)),
+               "slice_products": d * (d + 1) // 2, "passes": info["passes"], "kernel": info["kernel"]}
+        out.append(rec)
+        log(args, f"  accurate {M}x{N}x{K} b={b} d={d}: {t:.3f} ms x{t_nat / t:.3f} err {err} "
+                  f"(native {nat_err[0]:.2e}) passes {info['passes']}")
+        del A, B, C
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -586,6 +653,20 @@ def main():
     kname = (f"CRT modulus GEMM (k_crt_gemm_lds<4, 4>), {d} moduli" if crt else f"fused slice GEMM, d={d}")
     cfg = ({"scheme": "crt", "moduli": d, "bits": m["bits"], "rounding": "nearest"} if crt
            else {"scheme": "split", "bits": 7, "slices": d})
+    # the reference rule (b = 7 split scheme): its fastest preset at matched error, and the
+    # reference's own fast preset (d = 8), flat so that every record keeps them
+    split = [p for p in main_res["presets"] if p.get("scheme") == "split"]
+    rr_ok = [p for p in split if p["max_rel_err"] <= main_res["native"]["max_rel_err"]]
+    rr = max(rr_ok, key=lambda p: p["tflops"]) if rr_ok else min(split, key=lambda p: p["max_rel_err"])
+    ref_fast = next((p for p in split if p["slices"] == 8), None)
+    rr_cfg = {"reference_rule_slices": rr["slices"], "reference_rule_bits": 7,
+              "reference_rule_matched": bool(rr_ok), "reference_rule_ms": rr["ms"],
+              "reference_rule_tflops": rr["tflops"], "reference_rule_speedup": rr["speedup"],
+              "reference_rule_max_rel_err": rr["max_rel_err"],
+              "reference_rule_median_rel_err": rr["median_rel_err"]}
+    if ref_fast:
+        rr_cfg.update({"reference_fast_d8_speedup": ref_fast["speedup"],
+                       "reference_fast_d8_max_rel_err": ref_fast["max_rel_err"]})
     line = {
         "metric": "effective FP64 TFLOP/s vs native DGEMM on MI355X at matched max rel err vs MPFR",
         "value": m["tflops"],
@@ -601,12 +682,16 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"square DGEMM M=N=K={n}, entries (U-0.5)*exp({args.phi}*N(0,1)), "
                                f"seed {args.seed}, device-resident inputs",
-                   **cfg, "matched": m["matched"],
+                   **cfg, "matched": m["matched"], "speedup_vs_native": m["speedup"],
+                   "native_ms": main_res["native"]["ms"], "native_max_rel_err": main_res["native"]["max_rel_err"],
+                   "max_rel_err": m["max_rel_err"], "median_rel_err": m["median_rel_err"],
                    "error_reference": "correctly rounded exact product on a sampled "
-                                      f"{args.block}x{args.block} block"},
+                                      f"{args.block}x{args.block} block",
+                   **rr_cfg},
         "native_dgemm": main_res["native"],
         "speedup_vs_native": m["speedup"],
         "presets": main_res["presets"],
+        "reference_rule": dict(rr, matched=bool(rr_ok)),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS,
                      "unit": "TFLOP/s", "ops": "int8 MAC = 2 ops", "frac": achieved / INT8_PEAK_TOPS,
                      "traffic": traffic, "kernel": kname,
@@ -618,15 +703,18 @@ def main():
         line["host_pointer"] = {"error": repr(e)}
     if not args.no_cpu:
         try:
-            line["cpu_baseline"] = cpu_baseline(args, np, Ah, Bh, d, n, crt)
+            crt_port = d if crt else (crt_list(args)[0] if crt_list(args) else 0)
+            line["cpu_baseline"] = cpu_baseline(args, np, Ah, Bh, n, rr["slices"], crt_port)
         except Exception as e:  # the baseline is reported, never required
             line["cpu_baseline"] = {"error": repr(e)}
-    if not args.no_target and n != 8192:
-        tgt, _ = measure(args, torch, np, oz, 8192, max(3, args.steps // 2), max(2, args.warmup // 2), 64, slices)
-        line["target_8192"] = {"native_dgemm": tgt["native"], "presets": tgt["presets"],
-                               "matched": tgt["matched"],
-                               "speedup_vs_native": tgt["matched"]["speedup"]}
+    del Ah, Bh
+    if not args.no_target and n != 4096:
+        sq, _ = measure(args, torch, np, oz, 4096, args.steps, args.warmup, 256, slices)
+        line["square_4096"] = {"workload": "square DGEMM 4096^3, phi=1 (256x256 error block)",
+                               "native_dgemm": sq["native"], "presets": sq["presets"],
+                               "matched": sq["matched"], "speedup_vs_native": sq["matched"]["speedup"]}
     if not args.no_extra:
+        line["accurate_mode"] = accurate_section(args, torch, np, oz)
         line["long_k"] = long_k(args, torch, np, oz)
         line["grouped_512"] = grouped(args, torch, np, oz)
         line["autograd_2048"] = autograd_section(args, torch, np, oz)
