@@ -84,7 +84,8 @@ int32_t ozaie_get_option(int option);
 
 /* Number of (b, d) and passes of the current global plan:
  * out[0]=b out[1]=d out[2]=k-chunk out[3]=passes out[4]=kernel (0 fused, 1 group,
- * 2 CRT mode: then b = bits per operand and d = number of moduli)                       */
+ * 2 CRT mode: then b = bits per operand and d = number of moduli, 3 paired digits: b <= 3
+ * with d >= 8, adjacent digits paired into one int8 digit, the same C)                  */
 int32_t ozaie_plan_info(int32_t *out5);
 
 /* Release every device buffer and plan (reference: cleanup_xrt, include/oz_aie.h:153). */

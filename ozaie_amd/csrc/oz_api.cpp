@@ -187,6 +187,8 @@ struct ozaie_plan_s {
     int nchunks = 0, ngroups = 1, kernel = 0, family = 0;
     int device = 0;
     bool crt = false;     // CRT mode: b = bits per operand, d = number of moduli
+    bool pairs = false;   // paired-digit path (pair_cfg): kernel 3
+    int32_t *X = nullptr; // its anti-diagonal sums, Mp x Np int32
     CrtConsts cc{};
     uint32_t *R = nullptr;  // CRT residue products [d][Mp/4][Np]
     int8_t *SA = nullptr, *SB = nullptr;
@@ -214,6 +216,7 @@ void plan_free(ozaie_plan p) {
     if (p->Ec) (void)hipFree(p->Ec);
     if (p->T) (void)hipFree(p->T);
     if (p->R) (void)hipFree(p->R);
+    if (p->X) (void)hipFree(p->X);
     if (p->dA) (void)hipFree(p->dA);
     if (p->dB) (void)hipFree(p->dB);
     if (p->dC) (void)hipFree(p->dC);
@@ -238,7 +241,12 @@ int64_t oz::max_kchunk_blocks(int b, int d) { return std::max<int64_t>(2, max_kc
 
 namespace {
 
-int plan_configure(ozaie_plan p, int64_t M, int64_t N, int64_t K, int b, int d, int rounding) {
+// Paired-digit path (pair_cfg) for b <= 3 from this many slices on: (P(P+1)/2 + Q) int8
+// products instead of d(d+1)/2 (d = 29: 120 instead of 435), one fused pass.
+constexpr int kPairMinSlices = 8;
+
+int plan_configure(ozaie_plan p, int64_t M, int64_t N, int64_t K, int b, int d, int rounding,
+                   bool allow_pairs = true) {
     if (M < 1 || N < 1 || K < 1) { g_last_error = "dimensions must be >= 1"; return OZAIE_ERR_DIMS; }
     if (M > (1ll << 31) - 256 || N > (1ll << 31) - 256 || K > (1ll << 40)) {
         g_last_error = "dimensions too large"; return OZAIE_ERR_DIMS;
@@ -264,6 +272,24 @@ int plan_configure(ozaie_plan p, int64_t M, int64_t N, int64_t K, int b, int d, 
     p->kernel = (fused_supported(d) && g_opt_kernel != 1) ? 0 : 1;
     p->family = g_opt_kernel >= 2 ? g_opt_kernel : KERNEL_AUTO;
     p->ngroups = p->kernel == 0 ? 1 : (d + kGroupBins - 1) / kGroupBins;
+    if (allow_pairs && g_opt_kernel != 1 && b <= 3 && d >= kPairMinSlices) {
+        const PairCfg pc = pair_cfg(b, d);
+        const int64_t m1 = (1 << b) - 1, m2 = (1 << (2 * b)) - 1;
+        // the anti-diagonal sums over all of K stay int32; for even d the epilogue forms 2^b T
+        const bool x_ok = (double)pc.Q * m1 * m1 * (double)K < 2147483647.0;
+        const bool t_ok = pc.odd || 2.0 + std::log2((double)K) + 2.0 * b + (double)b * d <= 126.0;
+        if (x_ok && t_ok && fused_supported(pc.P)) {
+            p->pairs = true;
+            p->kernel = 3;
+            p->ngroups = 1;
+            p->Mp = round_up(M, kPairRowPad);
+            p->Np = round_up(N, kPairRowPad);
+            int64_t kcp = (int64_t)INT32_MAX / ((int64_t)pc.P * m2 * m2) / (2 * kBlk) * (2 * kBlk);
+            if (g_opt_kchunk > 0) kcp = std::min<int64_t>(kcp, std::max<int64_t>(2 * kBlk, g_opt_kchunk / (2 * kBlk) * (2 * kBlk)));
+            p->kcb = std::max<int64_t>(2, kcp / kBlk / 2 * 2);
+            p->nchunks = (int)((p->Kb + p->kcb - 1) / p->kcb);
+        }
+    }
     if (p->Mp / kBlk > 65535 || p->Np / kBlk > 65535) {
         g_last_error = "too many rows for the split grid"; return OZAIE_ERR_DIMS;
     }
@@ -306,6 +332,7 @@ int plan_alloc(ozaie_plan p, bool staging) {
     if (!p->crt && (int64_t)p->nchunks * p->ngroups > 1)
         HIPCHK(hipMalloc(&p->T, sizeof(__int128) * (size_t)p->M * p->N));
     if (p->crt) HIPCHK(hipMalloc(&p->R, (size_t)p->d * p->Mp * p->Np));
+    if (p->pairs) HIPCHK(hipMalloc(&p->X, sizeof(int32_t) * (size_t)p->Mp * p->Np));
     if (staging) {
         HIPCHK(hipMalloc(&p->dA, sizeof(double) * (size_t)p->M * p->K));
         HIPCHK(hipMalloc(&p->dB, sizeof(double) * (size_t)p->K * p->N));
@@ -320,12 +347,12 @@ int plan_alloc(ozaie_plan p, bool staging) {
 }
 
 int plan_new(ozaie_plan *out, int64_t M, int64_t N, int64_t K, int b, int d, int rounding,
-             bool staging, bool crt = false) {
+             bool staging, bool crt = false, bool allow_pairs = true) {
     *out = nullptr;
     int rc = ensure_device();
     if (rc) return rc;
     ozaie_plan p = new ozaie_plan_s();
-    rc = crt ? plan_configure_crt(p, M, N, K, b, d, rounding) : plan_configure(p, M, N, K, b, d, rounding);
+    rc = crt ? plan_configure_crt(p, M, N, K, b, d, rounding) : plan_configure(p, M, N, K, b, d, rounding, allow_pairs);
     if (!rc) rc = plan_alloc(p, staging);
     if (rc) { plan_free(p); return rc; }
     *out = p;
@@ -367,11 +394,15 @@ int run_split(ozaie_plan p, const OperandView &va, const OperandView &vb, hipStr
     HIPCHK(launch_maxexp(va, p->Er, p->Mp, st));
     if (p->crt)
         HIPCHK(launch_crt_split(va, p->Er, p->cc, p->SA, p->Mp, p->Kp, st));
+    else if (p->pairs)
+        HIPCHK(launch_split_pairs(va, p->Er, p->b, p->d, p->SA, p->Mp, p->Kp, false, st));
     else
         HIPCHK(launch_split(va, p->Er, p->b, p->d, p->SA, p->Mp, p->Kp, st));
     HIPCHK(launch_maxexp(vb, p->Ec, p->Np, st));
     if (p->crt)
         HIPCHK(launch_crt_split(vb, p->Ec, p->cc, p->SB, p->Np, p->Kp, st));
+    else if (p->pairs)
+        HIPCHK(launch_split_pairs(vb, p->Ec, p->b, p->d, p->SB, p->Np, p->Kp, true, st));
     else
         HIPCHK(launch_split(vb, p->Ec, p->b, p->d, p->SB, p->Np, p->Kp, st));
     return OZAIE_OK;
@@ -390,6 +421,20 @@ int run_passes(ozaie_plan p, double *C, int64_t ldc, int epi, int32_t *bins, hip
     }
     GemmLaunch g = base_launch(p);
     g.C = C; g.ldc = ldc; g.bins = bins;
+    if (p->pairs) {
+        if (epi != EPI_C) { g_last_error = "the paired-digit path has no bin output"; return OZAIE_ERR_ARG; }
+        const PairCfg pc = pair_cfg(p->b, p->d);
+        const int64_t planeA = p->Mp / kBlk * p->Kb * kBlkBytes, planeB = p->Np / kBlk * p->Kb * kBlkBytes;
+        HIPCHK(launch_gemm_antidiag(p->SA + pc.P * planeA, p->SB + pc.P * planeB, p->Mp, p->Np, pc.Q * p->Kb, p->X,
+                                    p->Np, st));
+        g.b = 2 * p->b;
+        g.d = pc.P;
+        g.cb = p->b;
+        g.cd = pc.odd ? p->d : p->d + 1;
+        g.xadd = p->X;
+        g.ldx = p->Np;
+        g.xsign = pc.odd ? 1 : -1;
+    }
     const int npass = p->nchunks * p->ngroups;
     int pass = 0;
     for (int c = 0; c < p->nchunks; c++) {
@@ -399,8 +444,8 @@ int run_passes(ozaie_plan p, double *C, int64_t ldc, int epi, int32_t *bins, hip
             g.k0 = gi * kGroupBins;
             g.first = pass == 0;
             g.last = pass == npass - 1;
-            if (p->kernel == 0) HIPCHK(launch_gemm_fused(g, epi, st));
-            else HIPCHK(launch_gemm_group(g, epi, st));
+            if (p->kernel == 1) HIPCHK(launch_gemm_group(g, epi, st));
+            else HIPCHK(launch_gemm_fused(g, epi, st));   // 0 fused, 3 paired digits (fused over P pair planes)
         }
     }
     return OZAIE_OK;
@@ -463,6 +508,7 @@ int64_t noalloc_bytes(const ozaie_plan p) {
     take(4 * p->Np);
     if (!p->crt && (int64_t)p->nchunks * p->ngroups > 1) take(16 * p->M * p->N);
     if (p->crt) take((int64_t)p->d * p->Mp * p->Np);
+    if (p->pairs) take(4 * p->Mp * p->Np);
     return off;
 }
 
@@ -650,6 +696,7 @@ int32_t ozaie_dgemm_noalloc(int64_t M, int64_t N, int64_t K, int bits, int slice
     p.Ec = reinterpret_cast<int32_t *>(take(4 * p.Np));
     if (!p.crt && (int64_t)p.nchunks * p.ngroups > 1) p.T = reinterpret_cast<__int128 *>(take(16 * p.M * p.N));
     if (p.crt) p.R = reinterpret_cast<uint32_t *>(take((int64_t)p.d * p.Mp * p.Np));
+    if (p.pairs) p.X = reinterpret_cast<int32_t *>(take(4 * p.Mp * p.Np));
     // Every intermediate is fully written before it is read (exponent arrays that are
     // combined with atomicMax are cleared by a fill kernel first), so the previous call's
     // contents never leak into this one.
@@ -760,7 +807,7 @@ static int debug_bins_impl(int64_t M, int64_t N, int64_t K, const double *A, con
     ozaie_plan p = nullptr;
     const int save_kernel = g_opt_kernel;
     g_opt_kernel = kernel;
-    int rc = plan_new(&p, M, N, K, bits, slices, OZAIE_ROUND_TRUNCATE, false);
+    int rc = plan_new(&p, M, N, K, bits, slices, OZAIE_ROUND_TRUNCATE, false, false, false);
     g_opt_kernel = save_kernel;
     if (rc) return rc;
     if (p->nchunks != 1) { plan_free(p); g_last_error = "K needs more than one pass"; return OZAIE_ERR_DIMS; }

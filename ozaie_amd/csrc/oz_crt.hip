@@ -97,135 +97,6 @@ hipError_t launch_crt_split(const OperandView &op, const int32_t *E, const CrtCo
     return hipGetLastError();
 }
 
-constexpr int kCrtStages = 4;   // LDS stages of the modulus GEMM's contraction loop
-
-// Contraction loop of one (64 WM) x (64 WN) tile, the workgroup's 2 WM A and 2 WN B blocks
-// of each contraction block shared through LDS.  The global loads write LDS themselves
-// (global_load_lds_dwordx4: lane l's 16 bytes land at the wave-uniform LDS address + 16 l,
-// which is a block's own layout), so there is no register staging and no LDS store; each
-// wave brings PER whole 1 KiB blocks of a contraction block.  S stages keep blocks
-// b + 1 .. b + S - 1 in flight while block b is multiplied, and the fragments of block b + 1
-// are read from LDS while the MFMAs of block b run:
-//   step b: wait for the wave's own loads of block b + 1 (loads complete in order, so the
-//           S - 2 later blocks stay in flight), barrier (block b + 1 is in LDS; every wave
-//           has read block b), issue block b + S into the stage block b used, read block
-//           b + 1, multiply block b (its fragments are in registers since step b - 1).
-// No load goes past block nk - 1: where fewer than S - 2 blocks follow, the wait is for
-// every load, so after the last step nothing is in flight, and the only LDS read after
-// the last barrier is the last step's read of a stage into fragments that are never
-// multiplied: prime() for another plane may follow at once.  The conditions are uniform
-// over the workgroup, so every wave meets the same barriers.
-// setup: A0 / B0 = the tile's first row block of one modulus plane of A / B (rows of
-// rowstride bytes), nk >= 1 contraction blocks; prime: issue the first S - 1 blocks;
-// run: the products; advance: move to another plane, for the next prime.
-template <int WM, int WN, int S>
-struct CrtTileLoop {
-    static constexpr int BA = 2 * WM, BB = 2 * WN, NB = BA + BB;   // blocks per contraction block
-    static constexpr int PER = NB / 4;                            // blocks each wave brings in
-    static_assert(NB % 4 == 0, "blocks must split over four waves");
-    static_assert(S >= 3 && S % 2 == 0, "two fragment sets alternate over an even number of stages");
-    static constexpr int VM = PER * (S - 2);   // loads of the wave still allowed in flight
-    static_assert(VM < 64, "vmcnt range");
-    // s_waitcnt vmcnt(VM) lgkmcnt(0) and vmcnt(0) lgkmcnt(0) as builtins: the compiler's own
-    // wait placement sees them (it does not read the inline asm of lds_barrier), so it knows
-    // the fragments of the previous step have arrived and adds no wait on the LDS reads
-    // issued after the barrier.
-    static constexpr int kWaitSteady = (VM & 15) | ((VM >> 4) << 14) | (7 << 4);
-    static constexpr int kWaitAll = 7 << 4;
-    const int8_t *src[PER];   // wave-uniform: first byte of each block row the wave brings
-    int64_t nk;
-    int lane, w, wm, wn;
-
-    __device__ __forceinline__ void setup(const int8_t *A0, const int8_t *B0, int64_t rowstride, int64_t nblk) {
-        nk = nblk;
-        lane = threadIdx.x & 63;
-        w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-        wm = w & 1;
-        wn = w >> 1;
-#pragma unroll
-        for (int i = 0; i < PER; i++) {
-            const int bi = w + 4 * i;   // block bi of the contraction block: A rows, then B columns
-            src[i] = bi < BA ? A0 + (int64_t)bi * rowstride : B0 + (int64_t)(bi - BA) * rowstride;
-        }
-    }
-    __device__ __forceinline__ void advance(int64_t offA, int64_t offB) {
-#pragma unroll
-        for (int i = 0; i < PER; i++) src[i] += (w + 4 * i < BA) ? offA : offB;
-    }
-    template <bool CHECK = true>
-    __device__ __forceinline__ void issue(int8_t *smem, int slot, int64_t kb) {
-        if (CHECK && kb >= nk) return;
-#pragma unroll
-        for (int i = 0; i < PER; i++)
-            __builtin_amdgcn_global_load_lds(
-                (const void *)(src[i] + kb * kBlkBytes + lane * 16),
-                (__attribute__((address_space(3))) void *)(smem + (slot * NB + w + 4 * i) * kBlkBytes), 16, 0, 0);
-    }
-    __device__ __forceinline__ void prime(int8_t *smem) {
-#pragma unroll
-        for (int s = 0; s < S - 1; s++) issue(smem, s, s);
-    }
-    // wait until block kb and all before it are in LDS, then barrier
-    template <bool CHECK = true>
-    __device__ __forceinline__ void sync(int64_t kb) {
-        if (!CHECK || kb + S - 2 < nk)
-            __builtin_amdgcn_s_waitcnt(kWaitSteady);
-        else
-            __builtin_amdgcn_s_waitcnt(kWaitAll);
-        lds_barrier();
-    }
-    __device__ __forceinline__ void read(const int8_t *st, v4i (&af)[WM], v4i (&bf)[WN]) {
-#pragma unroll
-        for (int mi = 0; mi < WM; mi++)
-            af[mi] = *reinterpret_cast<const v4i *>(st + (wm * WM + mi) * kBlkBytes + lane * 16);
-#pragma unroll
-        for (int ni = 0; ni < WN; ni++)
-            bf[ni] = *reinterpret_cast<const v4i *>(st + (BA + wn * WN + ni) * kBlkBytes + lane * 16);
-    }
-    // Step b (stage s = b mod S, fragment set s & 1).  CHECK = false: blocks b + S - 1 and
-    // b + S exist, so the step has no branch; the LDS reads and the MFMAs then share one
-    // basic block and the compiler adds no wait between them.  The read of block b + 1 is
-    // made even past the last block: it reads a stage (possibly while the next plane's
-    // first blocks land in it) into fragments that are never multiplied.
-    template <bool CHECK>
-    __device__ __forceinline__ void step(int s, int64_t b, int8_t *smem, v4i (&af)[2][WM], v4i (&bf)[2][WN],
-                                         v16i (&acc)[WM][WN]) {
-        const int f = s & 1;
-        __builtin_amdgcn_sched_barrier(0);
-        sync<CHECK>(b + 1);
-        issue<CHECK>(smem, s, b + S);
-        read(smem + ((s + 1) % S) * NB * kBlkBytes, af[f ^ 1], bf[f ^ 1]);
-#pragma unroll
-        for (int mi = 0; mi < WM; mi++)
-#pragma unroll
-            for (int ni = 0; ni < WN; ni++)
-                acc[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[f][mi], bf[f][ni], acc[mi][ni], 0, 0, 0);
-        __builtin_amdgcn_sched_group_barrier(0x100, WM + WN, 0);   // the LDS reads first,
-        __builtin_amdgcn_sched_group_barrier(0x008, WM * WN, 0);   // then the MFMAs
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    __device__ __forceinline__ void run(int8_t *smem, v16i (&acc)[WM][WN]) {
-#pragma unroll
-        for (int mi = 0; mi < WM; mi++)
-#pragma unroll
-            for (int ni = 0; ni < WN; ni++) acc[mi][ni] = v16i{};
-        v4i af[2][WM], bf[2][WN];
-        sync(0);
-        issue(smem, S - 1, S - 1);
-        read(smem, af[0], bf[0]);
-        int64_t t = 0;
-        for (; t + 2 * S <= nk; t += S) {   // every step of the pass loads a block that exists
-#pragma unroll
-            for (int s = 0; s < S; s++) step<false>(s, t + s, smem, af, bf, acc);
-        }
-        for (; t < nk; t += S) {
-#pragma unroll
-            for (int s = 0; s < S; s++)
-                if (t + s < nk) step<true>(s, t + s, smem, af, bf, acc);
-        }
-    }
-};
-
 // Residues of the wave's (32 WM) x (32 WN) sums, four rows of one column per word of
 // Rj[row / 4][col] (row stride ldr words), one signed byte each: r = x - p q with q the
 // integer nearest x / p, congruent to x, |r| <= p / 2.  q comes from fma(x, 1/p, 1.5 2^52),

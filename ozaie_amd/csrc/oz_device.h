@@ -137,6 +137,57 @@ __device__ __forceinline__ void emit_digits16(uint64_t (&mx)[16], const int (&bi
     }
 }
 
+// Paired-digit split of 16 elements (pair_cfg): the digit string Q = floor(M_x 2^(bd - 53 -
+// bias)) (below 2^96), taken as shifted left by b when d is odd so that it holds D =
+// ceil(d / 2) whole pairs; pair plane j < P is the 2b-bit field at 2b(D-1-j), single-digit plane I < Q
+// the b-bit field at 2b(D-1-I) + b (d odd: the pair's high digit s_2I) or at 2b(D-1-I)
+// (d even: its low digit s_2I+1).  dstS: the lane's chunk in pair plane 0 (planes planeS
+// apart); dstX: its chunk in single-digit slot 0 (slots planeX apart, written in reverse
+// order when reverse_q).  Every field is below 64, so the carry-free byte negation of
+// emit_digits16 applies.
+__device__ __forceinline__ void emit_pairs16(const uint64_t (&mx)[16], const int (&bias)[16], uint32_t negm,
+                                             int b, int d, int8_t *__restrict__ dstS, int64_t planeS,
+                                             int8_t *__restrict__ dstX, int64_t planeX, bool reverse_q) {
+    const PairCfg pc = pair_cfg(b, d);
+    const int D = (d + 1) / 2;
+    uint32_t nbm[4];
+#pragma unroll
+    for (int gi = 0; gi < 4; gi++) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int h = 0; h < 4; h++) m |= ((negm >> (4 * gi + h)) & 1u) ? (0xFFu << (8 * h)) : 0u;
+        nbm[gi] = m;
+    }
+    // The field at bit pos of Q' = M_x 2^s (s = bd - 53 - bias + b [d odd]) straight from the
+    // 53-bit mantissa with one 64-bit shift by s - pos (left if positive): the digit string
+    // itself may be wider than 64 bits, the shifted mantissa never needs to be.
+    int s[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) s[i] = b * d - 53 - bias[i] + (pc.odd ? b : 0);
+    auto field = [&](int i, int pos, uint32_t mask) -> uint32_t {
+        const int t = s[i] - pos;
+        const uint64_t v = t >= 0 ? (t >= 64 ? 0ull : mx[i] << t) : (t <= -64 ? 0ull : mx[i] >> (-t));
+        return (uint32_t)v & mask;
+    };
+    auto plane_out = [&](int8_t *dst, int pos, uint32_t mask) {
+        uint32_t w[4];
+#pragma unroll
+        for (int gi = 0; gi < 4; gi++) {
+            uint32_t p = 0;
+#pragma unroll
+            for (int h = 0; h < 4; h++) p |= field(4 * gi + h, pos, mask) << (8 * h);
+            const uint32_t pn = (0x80808080u - p) ^ 0x80808080u;
+            w[gi] = (pn & nbm[gi]) | (p & ~nbm[gi]);
+        }
+        *reinterpret_cast<uint4 *>(dst) = make_uint4(w[0], w[1], w[2], w[3]);
+    };
+    for (int j = 0; j < pc.P; j++) plane_out(dstS + (int64_t)j * planeS, 2 * b * (D - 1 - j), (1u << (2 * b)) - 1u);
+    for (int I = 0; I < pc.Q; I++) {
+        const int slot = reverse_q ? pc.Q - 1 - I : I;
+        plane_out(dstX + (int64_t)slot * planeX, 2 * b * (D - 1 - I) + (pc.odd ? b : 0), (1u << b) - 1u);
+    }
+}
+
 // The three steps of a tile of 32 rows x TK contraction indices (TK = 128 or 256),
 // separable so that a caller can overlap the global reads of the next tile with the digit
 // extraction of the current one.
@@ -203,6 +254,41 @@ __device__ __forceinline__ void split_emit(const double *tile, int64_t R, const 
         int8_t *dst = S + (rb * Kb + kb) * (int64_t)kBlkBytes + (q & 1) * 512 + row * 16;
         emit_digits16(mx, bias, negm, b, d, dst, plane);
     }
+}
+
+// split_emit for the paired-digit layout: pair planes at S (standard layout, Rb x Kb blocks
+// per plane), single-digit slots at S + P planes ([rb][slot][kb] blocks).
+template <bool KC>
+__device__ __forceinline__ void split_emit_pairs(const double *tile, int64_t R, const int32_t *__restrict__ Erow,
+                                                 int b, int d, int8_t *__restrict__ S, int64_t Rb, int64_t Kb,
+                                                 int64_t rb, int64_t kt, bool reverse_q) {
+    constexpr int TK = 128;
+    constexpr int LDW = KC ? (TK + 1) : (32 + 1);
+    const PairCfg pc = pair_cfg(b, d);
+    const int t = threadIdx.x;
+    const int64_t r0 = rb * 32;
+    const int row = t & 31;
+    const int emax = (r0 + row < R) ? Erow[row] : 0;
+    const int64_t plane = Rb * Kb * (int64_t)kBlkBytes;
+    const int q = t >> 5;   // k in [16q, 16q + 16) of the tile
+    const int64_t kb = kt * (TK / 32) + (q >> 1);
+    if (kb >= Kb) return;   // no barrier below
+    uint64_t mx[16];
+    int bias[16];
+    uint32_t negm = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const double v = KC ? tile[row * LDW + 16 * q + i] : tile[(16 * q + i) * LDW + row];
+        const uint64_t u = (uint64_t)__double_as_longlong(v);
+        const int e = (int)((u >> 52) & 0x7FF);
+        mx[i] = (u & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
+        bias[i] = emax - e;
+        negm |= (uint32_t)(u >> 63) << i;
+    }
+    const int64_t intra = (q & 1) * 512 + row * 16;
+    int8_t *dstS = S + (rb * Kb + kb) * (int64_t)kBlkBytes + intra;
+    int8_t *dstX = S + pc.P * plane + ((rb * pc.Q) * Kb + kb) * (int64_t)kBlkBytes + intra;
+    emit_pairs16(mx, bias, negm, b, d, dstS, plane, dstX, Kb * (int64_t)kBlkBytes, reverse_q);
 }
 
 template <bool KC>
@@ -279,10 +365,13 @@ __device__ __forceinline__ void epilogue_elem(const GemmLaunch &g, int epi, int 
     }
     i128 *tp = g.T + (int64_t)row * g.ldT + col;
     if (!g.first) T += *tp;
-    if (g.last)
-        g.C[(int64_t)row * g.ldc + col] = t_to_double(T, es, g.b, g.d, g.rounding);
-    else
+    if (g.last) {
+        if (g.xadd) T += (i128)(g.xsign * (int64_t)g.xadd[(int64_t)row * g.ldx + col]);
+        const int cb = g.cd ? g.cb : g.b, cd = g.cd ? g.cd : g.d;
+        g.C[(int64_t)row * g.ldc + col] = t_to_double(T, es, cb, cd, g.rounding);
+    } else {
         *tp = T;
+    }
 }
 
 // Exact T = sum_k v_k 2^(B(D-1-k)) for a compile-time digit width B with all D bins of one
@@ -419,7 +508,7 @@ __device__ __forceinline__ void wave_epilogue(const GemmLaunch &g, int epi, int 
                                               int ec_v) {
     scratch[NB * 256 + lane] = er_v;
     scratch[NB * 256 + 64 + lane] = ec_v;
-    if (epi == EPI_C && k0 == 0 && NB == g.d && g.b == 7)
+    if (epi == EPI_C && k0 == 0 && NB == g.d && g.b == 7 && !g.xadd && !g.cd)
         wave_epilogue_t<NB, WM, WN, 7>(g, epi, k0, acc, scratch, lane, rbase, cbase);
     else
         wave_epilogue_t<NB, WM, WN, 0>(g, epi, k0, acc, scratch, lane, rbase, cbase);
@@ -661,6 +750,137 @@ __device__ __forceinline__ void gemm_staged_tile(const GemmLaunch &g, int tm, in
     int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * wave_scratch_words<D, WM, WN>();
     wave_epilogue<D, WM, WN>(g, epi, 0, acc, scratch, lane, rbase, cbase, er_v, ec_v);
 }
+
+constexpr int kCrtStages = 4;   // LDS stages of the modulus GEMM's contraction loop
+
+// ------------------------------------------------------------------ single-product LDS loop
+// (the CRT modulus GEMM and the anti-diagonal GEMM of the paired-digit path)
+// Contraction loop of one (64 WM) x (64 WN) tile, the workgroup's 2 WM A and 2 WN B blocks
+// of each contraction block shared through LDS.  The global loads write LDS themselves
+// (global_load_lds_dwordx4: lane l's 16 bytes land at the wave-uniform LDS address + 16 l,
+// which is a block's own layout), so there is no register staging and no LDS store; each
+// wave brings PER whole 1 KiB blocks of a contraction block.  S stages keep blocks
+// b + 1 .. b + S - 1 in flight while block b is multiplied, and the fragments of block b + 1
+// are read from LDS while the MFMAs of block b run:
+//   step b: wait for the wave's own loads of block b + 1 (loads complete in order, so the
+//           S - 2 later blocks stay in flight), barrier (block b + 1 is in LDS; every wave
+//           has read block b), issue block b + S into the stage block b used, read block
+//           b + 1, multiply block b (its fragments are in registers since step b - 1).
+// No load goes past block nk - 1: where fewer than S - 2 blocks follow, the wait is for
+// every load, so after the last step nothing is in flight, and the only LDS read after
+// the last barrier is the last step's read of a stage into fragments that are never
+// multiplied: prime() for another plane may follow at once.  The conditions are uniform
+// over the workgroup, so every wave meets the same barriers.
+// setup: A0 / B0 = the tile's first row block of one modulus plane of A / B (rows of
+// rowstride bytes), nk >= 1 contraction blocks; prime: issue the first S - 1 blocks;
+// run: the products; advance: move to another plane, for the next prime.
+template <int WM, int WN, int S>
+struct CrtTileLoop {
+    static constexpr int BA = 2 * WM, BB = 2 * WN, NB = BA + BB;   // blocks per contraction block
+    static constexpr int PER = NB / 4;                            // blocks each wave brings in
+    static_assert(NB % 4 == 0, "blocks must split over four waves");
+    static_assert(S >= 3 && S % 2 == 0, "two fragment sets alternate over an even number of stages");
+    static constexpr int VM = PER * (S - 2);   // loads of the wave still allowed in flight
+    static_assert(VM < 64, "vmcnt range");
+    // s_waitcnt vmcnt(VM) lgkmcnt(0) and vmcnt(0) lgkmcnt(0) as builtins: the compiler's own
+    // wait placement sees them (it does not read the inline asm of lds_barrier), so it knows
+    // the fragments of the previous step have arrived and adds no wait on the LDS reads
+    // issued after the barrier.
+    static constexpr int kWaitSteady = (VM & 15) | ((VM >> 4) << 14) | (7 << 4);
+    static constexpr int kWaitAll = 7 << 4;
+    const int8_t *src[PER];   // wave-uniform: first byte of each block row the wave brings
+    int64_t nk;
+    int lane, w, wm, wn;
+
+    __device__ __forceinline__ void setup(const int8_t *A0, const int8_t *B0, int64_t rowstride, int64_t nblk) {
+        nk = nblk;
+        lane = threadIdx.x & 63;
+        w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        wm = w & 1;
+        wn = w >> 1;
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int bi = w + 4 * i;   // block bi of the contraction block: A rows, then B columns
+            src[i] = bi < BA ? A0 + (int64_t)bi * rowstride : B0 + (int64_t)(bi - BA) * rowstride;
+        }
+    }
+    __device__ __forceinline__ void advance(int64_t offA, int64_t offB) {
+#pragma unroll
+        for (int i = 0; i < PER; i++) src[i] += (w + 4 * i < BA) ? offA : offB;
+    }
+    template <bool CHECK = true>
+    __device__ __forceinline__ void issue(int8_t *smem, int slot, int64_t kb) {
+        if (CHECK && kb >= nk) return;
+#pragma unroll
+        for (int i = 0; i < PER; i++)
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(src[i] + kb * kBlkBytes + lane * 16),
+                (__attribute__((address_space(3))) void *)(smem + (slot * NB + w + 4 * i) * kBlkBytes), 16, 0, 0);
+    }
+    __device__ __forceinline__ void prime(int8_t *smem) {
+#pragma unroll
+        for (int s = 0; s < S - 1; s++) issue(smem, s, s);
+    }
+    // wait until block kb and all before it are in LDS, then barrier
+    template <bool CHECK = true>
+    __device__ __forceinline__ void sync(int64_t kb) {
+        if (!CHECK || kb + S - 2 < nk)
+            __builtin_amdgcn_s_waitcnt(kWaitSteady);
+        else
+            __builtin_amdgcn_s_waitcnt(kWaitAll);
+        lds_barrier();
+    }
+    __device__ __forceinline__ void read(const int8_t *st, v4i (&af)[WM], v4i (&bf)[WN]) {
+#pragma unroll
+        for (int mi = 0; mi < WM; mi++)
+            af[mi] = *reinterpret_cast<const v4i *>(st + (wm * WM + mi) * kBlkBytes + lane * 16);
+#pragma unroll
+        for (int ni = 0; ni < WN; ni++)
+            bf[ni] = *reinterpret_cast<const v4i *>(st + (BA + wn * WN + ni) * kBlkBytes + lane * 16);
+    }
+    // Step b (stage s = b mod S, fragment set s & 1).  CHECK = false: blocks b + S - 1 and
+    // b + S exist, so the step has no branch; the LDS reads and the MFMAs then share one
+    // basic block and the compiler adds no wait between them.  The read of block b + 1 is
+    // made even past the last block: it reads a stage (possibly while the next plane's
+    // first blocks land in it) into fragments that are never multiplied.
+    template <bool CHECK>
+    __device__ __forceinline__ void step(int s, int64_t b, int8_t *smem, v4i (&af)[2][WM], v4i (&bf)[2][WN],
+                                         v16i (&acc)[WM][WN]) {
+        const int f = s & 1;
+        __builtin_amdgcn_sched_barrier(0);
+        sync<CHECK>(b + 1);
+        issue<CHECK>(smem, s, b + S);
+        read(smem + ((s + 1) % S) * NB * kBlkBytes, af[f ^ 1], bf[f ^ 1]);
+#pragma unroll
+        for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++)
+                acc[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[f][mi], bf[f][ni], acc[mi][ni], 0, 0, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, WM + WN, 0);   // the LDS reads first,
+        __builtin_amdgcn_sched_group_barrier(0x008, WM * WN, 0);   // then the MFMAs
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    __device__ __forceinline__ void run(int8_t *smem, v16i (&acc)[WM][WN]) {
+#pragma unroll
+        for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++) acc[mi][ni] = v16i{};
+        v4i af[2][WM], bf[2][WN];
+        sync(0);
+        issue(smem, S - 1, S - 1);
+        read(smem, af[0], bf[0]);
+        int64_t t = 0;
+        for (; t + 2 * S <= nk; t += S) {   // every step of the pass loads a block that exists
+#pragma unroll
+            for (int s = 0; s < S; s++) step<false>(s, t + s, smem, af, bf, acc);
+        }
+        for (; t < nk; t += S) {
+#pragma unroll
+            for (int s = 0; s < S; s++)
+                if (t + s < nk) step<true>(s, t + s, smem, af, bf, acc);
+        }
+    }
+};
 
 // Direct variant: no LDS in the main loop.  Each wave streams its own A and B
 // fragments HBM/L2 -> VGPR (one 1 KiB block per wave instruction, 16 bytes per lane),

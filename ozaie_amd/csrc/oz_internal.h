@@ -53,6 +53,14 @@ struct GemmLaunch {
     int64_t *paired;        // reference wire format: [M][N][2] int64 (optional)
     int K_true;             // K of the product (paired-format group sizes)
     int family;             // fused-kernel family (KERNEL_*), KERNEL_AUTO = per-d default
+    // Paired-digit path (see launch_split_pairs): the bins are weighted with b = 2 b0 and
+    // d = P above, the conversion uses (cb, cd); 0 means the same as (b, d).  At the last
+    // pass the int32 anti-diagonal term xadd[row * ldx + col] is added (xsign = +1) or
+    // subtracted (-1) before the conversion.
+    int cb, cd;
+    const int32_t *xadd;
+    int64_t ldx;
+    int xsign;
 };
 
 // Fused-kernel families (same arithmetic, different data movement).
@@ -72,6 +80,39 @@ enum GroupCfg { GCFG_D11 = 0, GCFG_D12 = 1, GCFG_D22 = 2, GCFG_L11 = 3, GCFG_L12
 bool group_cfg_supported(int d, int cfg);
 int group_cfg_tile_m(int cfg);   // rows of C per workgroup
 int group_cfg_tile_n(int cfg);   // columns of C per workgroup
+
+// Paired-digit path for the small-digit presets (b <= 3, the reference's accurate mode):
+// adjacent digits s_2I, s_2I+1 of the b d-bit digit string are paired into one int8 digit
+// S_I = s_2I 2^b + s_2I+1 (|S_I| < 2^(2b) <= 64), so the triangle of d(d+1)/2 slice
+// products becomes a triangle of P(P+1)/2 products of pairs plus one anti-diagonal of
+// Q = H + 1 products of single digits that the truncation of the triangle cuts through:
+//   d odd  (H = (d-1)/2, P = H):      T      = sum_{k<P} SB_k 2^(2b(P-1-k)) + X,
+//                                      X  = sum_{I+J=H} s_2I t_2J   (high digits)
+//   d even (H = (d-2)/2, P = H + 1):  T 2^b = sum_{k<P} SB_k 2^(2b(P-1-k)) - Z,
+//                                      Z  = sum_{I+J=H} s_2I+1 t_2J+1 (low digits)
+// with SB_k = sum_{I+J=k} S_I T_J.  T is the same exact integer as the plain triangle's,
+// so C is bit-identical.  Plane layout of one operand (d planes in all): the P pair
+// planes S[j][rb][kb][1024] as usual, then the Q single-digit planes in the layout
+// X[rb][I][kb][1024] (for B stored in reverse order, I -> Q - 1 - I), so that the
+// anti-diagonal is one int8 GEMM over Q * Kb contraction blocks.
+struct PairCfg {
+    int b, d, odd, H, P, Q;
+};
+__host__ __device__ inline PairCfg pair_cfg(int b, int d) {
+    PairCfg c;
+    c.b = b; c.d = d; c.odd = d & 1;
+    c.H = c.odd ? (d - 1) / 2 : (d - 2) / 2;
+    c.P = c.odd ? c.H : c.H + 1;
+    c.Q = c.H + 1;
+    return c;
+}
+hipError_t launch_split_pairs(const OperandView &op, const int32_t *E, int b, int d, int8_t *S, int64_t Rp,
+                              int64_t Kp, bool reverse_q, hipStream_t st);
+// X[r][c] (int32, row stride ldx) = sum over Q*Kb contraction blocks of the single-digit
+// planes (tile 256 x 256: Mp and Np multiples of 256).
+hipError_t launch_gemm_antidiag(const int8_t *SxA, const int8_t *SxB, int64_t Mp, int64_t Np, int64_t nblk,
+                                int32_t *X, int64_t ldx, hipStream_t st);
+constexpr int kPairRowPad = 256;
 
 // Grouped launchers: items are int4 {operand, a, b, c} for the split phase and
 // int4 {problem, tile row, tile column, 0} for the GEMM phase.

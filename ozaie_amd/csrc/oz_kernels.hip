@@ -92,6 +92,82 @@ hipError_t launch_split(const OperandView &op, const int32_t *E, int b, int d, i
     return hipGetLastError();
 }
 
+// Paired-digit split (b <= 3 presets): the same grid and staging as k_split, pair planes
+// and single-digit slots written by emit_pairs16.
+template <bool KC>
+__global__ __launch_bounds__(256) void k_split_pairs(const double *__restrict__ X, int64_t sr, int64_t sk,
+                                                     int64_t R, int64_t K, const int32_t *__restrict__ E, int b,
+                                                     int d, int8_t *__restrict__ S, int64_t Rb, int64_t Kb,
+                                                     int reverse_q) {
+    __shared__ double tile[kSplitLds / 8];
+    double v[16];
+    split_gather<KC>(X, sr, sk, R, K, blockIdx.y, blockIdx.x, v);
+    split_stage<KC>(v, tile);
+    __syncthreads();
+    split_emit_pairs<KC>(tile, R, E + (int64_t)blockIdx.y * 32, b, d, S, Rb, Kb, blockIdx.y, blockIdx.x,
+                         reverse_q != 0);
+}
+
+hipError_t launch_split_pairs(const OperandView &op, const int32_t *E, int b, int d, int8_t *S, int64_t Rp,
+                              int64_t Kp, bool reverse_q, hipStream_t st) {
+    const int64_t Rb = Rp / kBlk, Kb = Kp / kBlk;
+    const int64_t nkt = (Kb + 3) / 4;
+    if (Rb > 65535 || nkt > 0x7fffffff || b < 1 || b > 3 || d < 2 || b * d > 96) return hipErrorInvalidValue;
+    if (op.sk == 1)
+        hipLaunchKernelGGL(k_split_pairs<true>, dim3((unsigned)nkt, (unsigned)Rb), dim3(256), 0, st, op.X, op.sr,
+                           op.sk, op.R, op.K, E, b, d, S, Rb, Kb, (int)reverse_q);
+    else
+        hipLaunchKernelGGL(k_split_pairs<false>, dim3((unsigned)nkt, (unsigned)Rb), dim3(256), 0, st, op.X, op.sr,
+                           op.sk, op.R, op.K, E, b, d, S, Rb, Kb, (int)reverse_q);
+    return hipGetLastError();
+}
+
+// The anti-diagonal of the paired-digit path: one int8 GEMM over nblk contraction blocks
+// (Q single-digit slots x Kb blocks, contiguous per 32-row block), workgroup tile 256 x 256
+// through the CRT modulus GEMM's LDS loop, int32 result stored whole (the sums stay below
+// Q K (2^b - 1)^2 < 2^31, which the planner checks).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_gemm_antidiag(const int8_t *__restrict__ SxA, const int8_t *__restrict__ SxB, int64_t Mp, int64_t Np,
+                     int64_t nblk, int32_t *__restrict__ X, int64_t ldx) {
+    constexpr int WM = 4, WN = 4;
+    __shared__ __attribute__((aligned(1024))) int8_t smem[kCrtStages * (2 * WM + 2 * WN) * kBlkBytes];
+    const int ntm = (int)(Mp / (64 * WM)), ntn = (int)(Np / (64 * WN));
+    int tm, tn;
+    tile_map(blockIdx.x, ntm, ntn, tm, tn);
+    const int64_t rowstride = nblk * kBlkBytes;
+    v16i acc[WM][WN];
+    CrtTileLoop<WM, WN, kCrtStages> L;
+    L.setup(SxA + (int64_t)tm * 2 * WM * rowstride, SxB + (int64_t)tn * 2 * WN * rowstride, rowstride, nblk);
+    L.prime(smem);
+    L.run(smem, acc);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = w & 1, wn = w >> 1;
+    const int64_t rbase = (int64_t)tm * 64 * WM + wm * 32 * WM, cbase = (int64_t)tn * 64 * WN + wn * 32 * WN;
+#pragma unroll
+    for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+        for (int ni = 0; ni < WN; ni++)
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int64_t row = rbase + mi * 32 + q * 8 + (lane >> 5) * 4 + e;
+                    const int64_t col = cbase + ni * 32 + (lane & 31);
+                    X[row * ldx + col] = acc[mi][ni][4 * q + e];
+                }
+}
+
+hipError_t launch_gemm_antidiag(const int8_t *SxA, const int8_t *SxB, int64_t Mp, int64_t Np, int64_t nblk,
+                                int32_t *X, int64_t ldx, hipStream_t st) {
+    if (Mp % 256 || Np % 256 || nblk < 1 || ldx < Np) return hipErrorInvalidValue;
+    const int64_t tiles = (Mp / 256) * (Np / 256);
+    if (tiles <= 0) return hipSuccess;
+    if (tiles > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gemm_antidiag, dim3((unsigned)tiles), dim3(256), 0, st, SxA, SxB, Mp, Np, nblk, X, ldx);
+    return hipGetLastError();
+}
+
 template <int D, int WM, int WN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_gemm_fused(GemmLaunch g, int epi) {
