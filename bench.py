@@ -17,7 +17,9 @@ The JSON line also carries:
   roofline      the GEMM kernel of the matched preset: int8 ops per launch (d(d+1)/2 * 2 M N K
                 for the split scheme, moduli * 2 M N K for the CRT mode) / its average
                 duration (HIP events recorded by the library on the stream it launches on)
-                against the 5.0 POP/s dense int8 peak;
+                against the 5.0 POP/s dense int8 peak, and against the ceiling the matrix
+                cores of this card sustain with register-only operands (ozaie_probe_mfma:
+                zero, constant and toggling random operands);
                 traffic = HBM bytes per launch from the rocprofv3 FETCH/WRITE summaries in
                 profiles/ when present
   cpu_baseline  the oracle's OpenMP restatement of the reference rule (b = 7, the matched
@@ -65,6 +67,8 @@ def parse():
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--grouped-only", action="store_true",
                     help="only the 512-problem grouped section (prints its record as JSON)")
+    ap.add_argument("--probe-only", action="store_true",
+                    help="only the measured int8 MFMA ceiling (ozaie_probe_mfma), as one JSON line")
     ap.add_argument("--profile-run", action="store_true",
                     help="only run each preset's GEMM twice and one 1 GiB device copy (for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -641,6 +645,13 @@ def main():
     if args.profile_run:
         profile_run(args, torch, oz, n, slices)
         return
+    if args.probe_only:
+        rec = {}
+        for pat in ("zeros", "constant", "random"):
+            pops, ms = oz.probe_mfma(pat)
+            rec[pat] = {"tops": pops * 1e3, "ms": ms}
+        print(json.dumps({"int8_mfma_ceiling": rec, "datasheet_tops": INT8_PEAK_TOPS}), flush=True)
+        return
     if args.grouped_only:
         print(json.dumps(grouped(args, torch, np, oz, slices=tuple(slices))), flush=True)
         return 0
@@ -650,6 +661,14 @@ def main():
     d = m["moduli"] if crt else m["slices"]
     achieved = preset_ops(m, n) / m["gemm_ms"] / 1e9
     traffic = load_traffic("crt" if crt else d, n)
+    # measured int8 MFMA ceiling of this card (register-only operands), beside the datasheet
+    ceiling = {}
+    for pat in ("zeros", "constant", "random"):
+        try:
+            ceiling[pat] = oz.probe_mfma(pat)[0] * 1e3   # TOP/s
+        except Exception as e:  # reported beside value, never required
+            log(args, f"  mfma probe {pat}: {e!r}")
+    log(args, f"  int8 MFMA ceiling (TOP/s): {ceiling}")
     kname = (f"CRT modulus GEMM (k_crt_gemm_lds<4, 4>), {d} moduli" if crt else f"fused slice GEMM, d={d}")
     cfg = ({"scheme": "crt", "moduli": d, "bits": m["bits"], "rounding": "nearest"} if crt
            else {"scheme": "split", "bits": 7, "slices": d})
@@ -695,7 +714,11 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS,
                      "unit": "TFLOP/s", "ops": "int8 MAC = 2 ops", "frac": achieved / INT8_PEAK_TOPS,
                      "traffic": traffic, "kernel": kname,
-                     "avg_launch_ms": m["gemm_ms"]},
+                     "avg_launch_ms": m["gemm_ms"],
+                     "measured_ceiling_random": ceiling.get("random"),
+                     "measured_ceiling_constant": ceiling.get("constant"),
+                     "measured_ceiling_zeros": ceiling.get("zeros"),
+                     "frac_of_measured_ceiling": (achieved / ceiling["random"]) if ceiling.get("random") else None},
     }
     try:
         line["host_pointer"] = host_pointer(np, oz, Ah, Bh, n, crt, d)

@@ -230,6 +230,17 @@ int32_t ozaie_paired_output(int64_t M, int64_t N, int64_t K, const double *A,
 int32_t ozaie_exact_entries(const double *A, int64_t lda, const double *B, int64_t ldb, int64_t K,
                             const int64_t *rows, const int64_t *cols, int64_t count, double *out);
 
+/* ---- measured int8 MFMA ceiling (diagnostic; no reference counterpart)
+ * Sustained rate, in dense int8 POP/s (one MAC = 2 ops), of v_mfma_i32_32x32x32_i8 run from
+ * registers only on every CU (one wave per SIMD, 16 independent accumulator tiles per
+ * wave), for `iters` rounds of 128 MFMAs after one warm-up launch.  pattern 0: all-zero
+ * operands; 1: one random fragment pair for every MFMA (constant operands); 2: eight random
+ * fragment pairs in turn (operands toggle like slice or residue data).  The rate the
+ * matrix cores sustain depends on the operand data through power and clock; this is the
+ * measured ceiling the GEMM kernels are compared with beside the datasheet peak.
+ * ms_out (optional): milliseconds of the timed launch.                                  */
+int32_t ozaie_probe_mfma(int pattern, int iters, double *pops_out, double *ms_out);
+
 /* Last HIP error string (for diagnostics). */
 const char *ozaie_last_error(void);
 

@@ -64,7 +64,7 @@ EXPORTS = [
     "ozaie_exact_entries", "ozaie_group_create", "ozaie_group_run", "ozaie_group_query",
     "ozaie_group_configs", "ozaie_group_status", "ozaie_group_destroy", "ozaie_dgemm_grouped", "ozaie_autotune",
     "ozaie_tune_get", "ozaie_tune_set", "ozaie_autotune_crt", "ozaie_tune_get_crt", "ozaie_tune_set_crt",
-    "ozaie_dgemm_noalloc",
+    "ozaie_dgemm_noalloc", "ozaie_probe_mfma",
 ]
 
 NOALLOC_BYTES = 256 << 20   # OZAIE_NOALLOC_BYTES: the static workspace of ozaie_dgemm_noalloc
@@ -166,6 +166,8 @@ def _declare(L):
     L.ozaie_tune_set_crt.restype = i32
     L.ozaie_dgemm_noalloc.argtypes = [i64, i64, i64, c_int, c_int, c_int, c_int, c_int, P, i64, P, i64, P, i64, P]
     L.ozaie_dgemm_noalloc.restype = i32
+    L.ozaie_probe_mfma.argtypes = [c_int, c_int, P, P]
+    L.ozaie_probe_mfma.restype = i32
     L.ozaie_last_error.argtypes = []
     L.ozaie_last_error.restype = ctypes.c_char_p
     L.ozaie_version.argtypes = []
@@ -524,6 +526,20 @@ def set_tune_table(slices, table, crt=False):
         _check(lib().ozaie_tune_set_crt(int(slices), _ptr(t)), "ozaie_tune_set_crt")
     else:
         _check(lib().ozaie_tune_set(int(slices), _ptr(t)), "ozaie_tune_set")
+
+
+PROBE_PATTERNS = {"zeros": 0, "constant": 1, "random": 2}
+
+
+def probe_mfma(pattern="random", iters=20000):
+    """Measured int8 MFMA ceiling (POP/s, ms of the timed launch) with register-only
+    operands: "zeros", "constant" (one random fragment pair) or "random" (operands that
+    toggle every MFMA, like slice data)."""
+    pops = ctypes.c_double()
+    ms = ctypes.c_double()
+    _check(lib().ozaie_probe_mfma(PROBE_PATTERNS[pattern], int(iters), ctypes.byref(pops), ctypes.byref(ms)),
+           "ozaie_probe_mfma")
+    return pops.value, ms.value
 
 
 def version():

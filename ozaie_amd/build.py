@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libozaie.so")
 ARCH = os.environ.get("OZAIE_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["oz_kernels.hip", "oz_grouped.hip", "oz_crt.hip", "oz_api.cpp", "oz_group.cpp"]
+SOURCES = ["oz_kernels.hip", "oz_grouped.hip", "oz_crt.hip", "oz_probe.hip", "oz_api.cpp", "oz_group.cpp"]
 HOST_SOURCES = ["oz_verify.cpp"]   # plain host C++ (g++, OpenMP)
 
 

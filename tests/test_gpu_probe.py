@@ -1,0 +1,22 @@
+"""The measured int8 MFMA ceiling (ozaie_probe_mfma) runs and returns rates below the
+dense datasheet peak of 5.0 POP/s; data that toggles draws more power than zeros, so its
+sustained rate is not above the all-zero rate by more than noise."""
+import pytest
+
+import ozaie_amd as oz
+
+
+@pytest.mark.gpu
+def test_probe_rates(gpu):
+    rates = {p: oz.probe_mfma(p, iters=4000)[0] for p in ("zeros", "constant", "random")}
+    print(rates, flush=True)
+    for p, r in rates.items():
+        assert 0.3 < r < 5.2, (p, r)
+    assert rates["random"] <= rates["zeros"] * 1.05
+
+
+def test_probe_rejects_bad_arguments():
+    import ctypes
+    v = ctypes.c_double()
+    assert oz.lib().ozaie_probe_mfma(3, 10, ctypes.byref(v), None) == oz.ERR_ARG
+    assert oz.lib().ozaie_probe_mfma(0, 0, ctypes.byref(v), None) == oz.ERR_ARG
