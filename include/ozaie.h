@@ -137,18 +137,23 @@ int32_t ozaie_dgemm_noalloc(int64_t M, int64_t N, int64_t K, int bits, int slice
 
 /* ---- grouped products (many independent C_i = A_i B_i of uneven shape)
  * A_i (M_i x K_i), B_i (K_i x N_i), C_i (M_i x N_i): row-major, contiguous, device pointers.
- * All problems share b and d.  Slice counts 3, 7, 8, 9 run as one exponent launch, one
- * split launch and one fused-GEMM launch per tile configuration, each tile
- * configuration chosen per size class by the autotuner table (or a heuristic when the
- * class is untuned); other slice counts run one plan per problem.  Every K_i must fit
- * one contraction pass.  The launches are captured once in a hipGraph (flags bit 1
- * disables the graph) and replayed by ozaie_group_run; descriptors are re-uploaded only
- * when the pointers change.  flags bit 0: nearest-even rounding.
- * flags bit 2 (OZAIE_GROUP_PERSISTENT): slice counts 3, 7, 8, 9 run as ONE persistent
- * launch (one workgroup per CU: exponent max + split of every operand, a grid barrier,
- * then every GEMM tile, balanced per XCD on the host) with one tile shape for the group.
- * flags bits 4..7: persistent tile configuration + 1 (0: per-d default; configuration
- * ids as for ozaie_autotune, direct-load tiles only).
+ * All problems share b.  The slice count is `slices` for every problem, or with slices =
+ * 0 each problem's size class's entry of the slice-count table (ozaie_tune_set_slices;
+ * 9 where untuned).  Slice counts 3, 7, 8, 9 run as one exponent launch, one split launch
+ * and one fused-GEMM launch per (slice count, tile configuration), each problem's tile
+ * configuration chosen per size class by the autotuner table of its slice count (or a
+ * heuristic when the class is untuned); a group with any other slice count runs one plan
+ * per problem.  Every K_i must fit one contraction pass.  The launches are captured once
+ * in a hipGraph (flags bit 1 disables the graph) and replayed by ozaie_group_run;
+ * descriptors are re-uploaded only when the pointers change.  flags bit 0: nearest-even
+ * rounding.
+ * flags bit 2 (OZAIE_GROUP_PERSISTENT): ONE persistent launch (one workgroup per CU:
+ * exponent max + split of every operand, a grid barrier, then every GEMM tile, balanced
+ * per XCD on the host), each problem with its own slice count and with its size class's
+ * tile configuration mapped to the persistent kernel's tiles (direct register loads:
+ * 64x64 for every slice count, 64x128 for 3 and 7, 128x128 for 3).  flags bits 4..7:
+ * forced persistent tile configuration + 1 (0: per size class; ids as for
+ * ozaie_autotune), applied to the problems whose slice count has it.
  * flags bit 3 (OZAIE_GROUP_CRT): the CRT mode for every problem (no reference
  * counterpart): `slices` = moduli (2..20), `bits` = 0 for the largest exact bit count of
  * each problem's K (crt_beta(moduli, K_i)) or a fixed count no larger than any problem's;
@@ -190,6 +195,12 @@ int32_t ozaie_dgemm_grouped(int count, const int64_t *M, const int64_t *N, const
  * 4: 64x128 LDS copies).  reps: timed runs per candidate.                             */
 int32_t ozaie_autotune(int bits, int slices, int reps, int32_t *table9);
 int32_t ozaie_tune_get(int slices, int32_t *table9);   /* -1 entries: untuned */
+/* Slice count per size class for groups created with slices = 0 (3, 7, 8 or 9; -1 =
+ * untuned, 9).  ozaie_amd.tune.tune_slices calibrates it: per class the cheapest count
+ * whose maximum error on a calibration sample is no worse than native FP64 DGEMM's
+ * (the analogue of the reference's per-log2 K choice of (b, d), include/oz_aie.h:60-80). */
+int32_t ozaie_tune_get_slices(int32_t *table9);
+int32_t ozaie_tune_set_slices(const int32_t *table9);
 int32_t ozaie_tune_set(int slices, const int32_t *table9);
 /* The same for CRT groups with `moduli` moduli (no reference counterpart); tile shape
  * ids 0: 256x256, 1: 128x128, 2: 64x64 (modulus GEMM through LDS).                     */

@@ -64,7 +64,7 @@ EXPORTS = [
     "ozaie_exact_entries", "ozaie_group_create", "ozaie_group_run", "ozaie_group_query",
     "ozaie_group_configs", "ozaie_group_status", "ozaie_group_destroy", "ozaie_dgemm_grouped", "ozaie_autotune",
     "ozaie_tune_get", "ozaie_tune_set", "ozaie_autotune_crt", "ozaie_tune_get_crt", "ozaie_tune_set_crt",
-    "ozaie_dgemm_noalloc", "ozaie_probe_mfma",
+    "ozaie_dgemm_noalloc", "ozaie_probe_mfma", "ozaie_tune_get_slices", "ozaie_tune_set_slices",
 ]
 
 NOALLOC_BYTES = 256 << 20   # OZAIE_NOALLOC_BYTES: the static workspace of ozaie_dgemm_noalloc
@@ -166,6 +166,10 @@ def _declare(L):
     L.ozaie_tune_set_crt.restype = i32
     L.ozaie_dgemm_noalloc.argtypes = [i64, i64, i64, c_int, c_int, c_int, c_int, c_int, P, i64, P, i64, P, i64, P]
     L.ozaie_dgemm_noalloc.restype = i32
+    L.ozaie_tune_get_slices.argtypes = [P]
+    L.ozaie_tune_get_slices.restype = i32
+    L.ozaie_tune_set_slices.argtypes = [P]
+    L.ozaie_tune_set_slices.restype = i32
     L.ozaie_probe_mfma.argtypes = [c_int, c_int, P, P]
     L.ozaie_probe_mfma.restype = i32
     L.ozaie_last_error.argtypes = []
@@ -420,8 +424,9 @@ def exact_entries(A, B, rows, cols):
 
 class Group:
     """Many independent products C_i = A_i @ B_i (float64 device tensors, row-major) with
-    b bits and d slices each, replayed through one captured hipGraph.  persistent=True
-    runs the whole group as one persistent kernel launch (slice counts 3, 7, 8, 9).
+    b bits and d slices each (slices=0: each problem's size class's count from the
+    slice-count table), replayed through one captured hipGraph.  persistent=True runs the
+    whole group as one persistent kernel launch (slice counts 3, 7, 8, 9).
     crt=True runs every problem in the CRT mode with `slices` moduli and `bits` = 0 (the
     largest exact bit count per problem) or a fixed bit count; crt_cfg forces a tile shape
     (0: 256x256, 1: 128x128, 2: 64x64)."""
@@ -540,6 +545,25 @@ def probe_mfma(pattern="random", iters=20000):
     _check(lib().ozaie_probe_mfma(PROBE_PATTERNS[pattern], int(iters), ctypes.byref(pops), ctypes.byref(ms)),
            "ozaie_probe_mfma")
     return pops.value, ms.value
+
+
+def slice_table():
+    """Slice count per size class for groups created with slices=0 (-1: untuned, 9)."""
+    out = np.zeros(9, dtype=np.int32)
+    _check(lib().ozaie_tune_get_slices(_ptr(out)), "ozaie_tune_get_slices")
+    return out
+
+
+def set_slice_table(table):
+    t = np.ascontiguousarray(table, dtype=np.int32)
+    _check(lib().ozaie_tune_set_slices(_ptr(t)), "ozaie_tune_set_slices")
+
+
+def size_class(M, N, K):
+    """The autotuner's size class of a product: 3 * bucket(max(M, N)) + bucket(K), buckets
+    <= 256, <= 1024, larger."""
+    bucket = lambda x: 0 if x <= 256 else (1 if x <= 1024 else 2)
+    return 3 * bucket(max(M, N)) + bucket(K)
 
 
 def version():

@@ -56,6 +56,27 @@ int pick_cfg(int d, int64_t M, int64_t N, int64_t K, int force) {
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
+// Slice count per size class for groups created with slices = 0 (-1 = untuned: 9, the
+// count whose maximum error matches native FP64 DGEMM at phi = 1 on every class measured).
+// Set by ozaie_tune_set_slices, usually from the calibration in ozaie_amd.tune.
+int g_tune_slices[9];
+bool g_tune_slices_init = false;
+constexpr int kDefaultGroupSlices = 9;
+
+int pick_slices(int64_t M, int64_t N, int64_t K) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (!g_tune_slices_init) {
+        for (int &v : g_tune_slices) v = -1;
+        g_tune_slices_init = true;
+    }
+    const int t = g_tune_slices[size_class(M, N, K)];
+    return t > 0 ? t : kDefaultGroupSlices;
+}
+
+// Index of a grouped-kernel slice count (3, 7, 8, 9) in the per-(d, tile) launch tables.
+constexpr int kGroupSliceCounts[4] = {3, 7, 8, 9};
+int dslot(int d) { return d == 3 ? 0 : (d == 7 ? 1 : (d == 8 ? 2 : (d == 9 ? 3 : -1))); }
+
 // CRT mode: g_tune_crt[moduli][size class] = CrtCfg, -1 = untuned.
 int g_tune_crt[kCrtMaxMod + 1][9];
 bool g_tune_crt_init = false;
@@ -95,6 +116,7 @@ int pick_crt_cfg(int n, int64_t M, int64_t N, int64_t K, int force) {
 struct ozaie_group_s {
     int count = 0, b = 0, d = 0, rounding = 0, flags = 0, device = 0;
     std::vector<int64_t> M, N, K;
+    std::vector<int> dp;                   // slice count per problem (split scheme)
     std::vector<int> cfg;
     bool fallback = false;                 // slice count without grouped kernels
     std::vector<ozaie_plan> plans;         // fallback: one plan per problem
@@ -106,8 +128,8 @@ struct ozaie_group_s {
     GemmLaunch *d_probs = nullptr;
     int4 *d_items_max = nullptr, *d_items_split = nullptr;
     int n_max = 0, n_split = 0;
-    int4 *d_items_gemm[GCFG_COUNT] = {};
-    int n_gemm[GCFG_COUNT] = {};
+    int4 *d_items_gemm[4][GCFG_COUNT] = {};   // per slice-count slot (dslot) and tile configuration
+    int n_gemm[4][GCFG_COUNT] = {};
     int64_t tiles = 0;
     // host mirrors of the descriptors (pinned) and the pointers they hold
     GroupOperand *h_ops = nullptr;
@@ -119,9 +141,8 @@ struct ozaie_group_s {
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    // persistent single-launch mode (flags bit 2)
+    // persistent single-launch mode (flags bit 2): tile configuration per problem in cfg
     bool persist = false;
-    int pcfg = 0;
     char *pbuf = nullptr;                  // item lists + control block
     PersistLaunch PL{};
     hipStream_t last_stream = nullptr;
@@ -152,8 +173,9 @@ void group_free(ozaie_group g) {
     if (g->d_probs) (void)hipFree(g->d_probs);
     if (g->d_items_max) (void)hipFree(g->d_items_max);
     if (g->d_items_split) (void)hipFree(g->d_items_split);
-    for (auto &p : g->d_items_gemm)
-        if (p) (void)hipFree(p);
+    for (auto &row : g->d_items_gemm)
+        for (auto &p : row)
+            if (p) (void)hipFree(p);
     if (g->h_ops) (void)hipHostFree(g->h_ops);
     if (g->h_probs) (void)hipHostFree(g->h_probs);
     if (g->d_cprobs) (void)hipFree(g->d_cprobs);
@@ -178,9 +200,17 @@ int upload(T **dptr, const std::vector<T> &v) {
     return OZAIE_OK;
 }
 
-// Default tile of the persistent kernel per slice count (registers allow 64 x 128 tiles
-// with prefetch for d = 3 and 7 only; 128 x 128 at d = 3).
-int default_persist_cfg(int d) { return d == 3 ? GCFG_D22 : (d == 7 ? GCFG_D12 : GCFG_D11); }
+// Tile of the persistent kernel for a problem whose size class the multi-launch table
+// assigns `cfg`: the same shape where the persistent kernel has it (its tiles load
+// fragments straight into registers, so an LDS-copy configuration maps to the direct one
+// of the same shape), else the widest it has for d (registers allow 64 x 128 tiles with
+// prefetch for d = 3 and 7 only, 128 x 128 at d = 3).
+int persist_cfg_for(int d, int cfg) {
+    if (persist_cfg_supported(d, cfg)) return cfg;
+    if (cfg == GCFG_L11) return GCFG_D11;
+    if (cfg == GCFG_L12 || cfg == GCFG_D12) return persist_cfg_supported(d, GCFG_D12) ? GCFG_D12 : GCFG_D11;
+    return d == 3 ? GCFG_D22 : (d == 7 ? GCFG_D12 : GCFG_D11);
+}
 
 // Longest-processing-time assignment of items (in the given order) to the workgroups in
 // `wgs`: each item goes to the least loaded one.  Appends to per-workgroup lists.
@@ -231,21 +261,25 @@ int build_persist(ozaie_group g, const std::vector<int64_t> &Kp) {
     for (int w = 0; w < 4 * G; w++) slots[w] = w;
     std::vector<std::vector<int>> listsA(4 * G), listsB(G);
     lpt_assign(costA, orderA, slots, listsA);
-    // phase B
-    const int tm_rows = group_cfg_tile_m(g->pcfg), tn_cols = group_cfg_tile_n(g->pcfg);
-    const int64_t area = (int64_t)(tm_rows / 64) * (tn_cols / 64);
+    // phase B: each problem in its own tile configuration, tiles costed by contraction
+    // blocks (plus the epilogue) times tile area times slice products
     std::vector<int> porder(n);
     for (int p = 0; p < n; p++) porder[p] = p;
-    std::stable_sort(porder.begin(), porder.end(), [&](int a, int b) { return Kp[a] > Kp[b]; });
+    std::stable_sort(porder.begin(), porder.end(), [&](int a, int b) {
+        return Kp[a] * g->dp[a] * (g->dp[a] + 1) > Kp[b] * g->dp[b] * (g->dp[b] + 1);
+    });
     std::vector<int4> itB;
     std::vector<int64_t> costB;
     for (int p : porder) {
+        const int c = g->cfg[p], dd = g->dp[p];
+        const int tm_rows = group_cfg_tile_m(c), tn_cols = group_cfg_tile_n(c);
+        const int64_t area = (int64_t)(tm_rows / 64) * (tn_cols / 64);
         const int64_t ntm = (g->M[p] + tm_rows - 1) / tm_rows, ntn = (g->N[p] + tn_cols - 1) / tn_cols;
         for (int64_t t = 0; t < ntm * ntn; t++) {
             int tm, tn;
             band_map((int)t, (int)ntm, (int)ntn, tm, tn);
-            itB.push_back(int4{p, tm, tn, 0});
-            costB.push_back((Kp[p] / kBlk + 4) * area);   // contraction blocks + epilogue
+            itB.push_back(int4{p, tm, tn, persist_code(dd, c)});
+            costB.push_back((Kp[p] / kBlk + 4) * area * (dd * (dd + 1) / 2));
         }
     }
     g->tiles = (int64_t)itB.size();
@@ -275,6 +309,8 @@ int build_persist(ozaie_group g, const std::vector<int64_t> &Kp) {
     }
     for (int w = 0; w < G; w++) {
         offB[w] = (int)flatB.size();
+        // items of one (d, tile) code consecutive: one tile body per run in the kernel
+        std::stable_sort(listsB[w].begin(), listsB[w].end(), [&](int a, int b) { return itB[a].w < itB[b].w; });
         for (int i : listsB[w]) flatB.push_back(itB[i]);
     }
     offA[4 * G] = (int)flatA.size();
@@ -460,11 +496,12 @@ int group_build_crt(ozaie_group g, int force_cfg) {
 int group_build(ozaie_group g, int force_cfg) {
     if (g->crt) return group_build_crt(g, force_cfg);
     const int n = g->count;
-    const int64_t kmax = max_kchunk_blocks(g->b, g->d) * kBlk;
     std::vector<int64_t> Mp(n), Np(n), Kp(n);
     int64_t nE = 0, slice_bytes = 0;
+    const int pforce = ((g->flags >> 4) & 15) - 1;   // forced persistent tile configuration
     for (int p = 0; p < n; p++) {
-        if (g->K[p] > kmax) {
+        const int dd = g->dp[p];
+        if (g->K[p] > max_kchunk_blocks(g->b, dd) * kBlk) {
             set_error_msg("grouped problems must fit one contraction pass (K too long for these slices)");
             return OZAIE_ERR_DIMS;
         }
@@ -472,8 +509,9 @@ int group_build(ozaie_group g, int force_cfg) {
         Np[p] = round_up(g->N[p], kRowPad);
         Kp[p] = round_up(g->K[p], 2 * kBlk);
         nE += Mp[p] + Np[p];
-        slice_bytes += (int64_t)g->d * (Mp[p] + Np[p]) * Kp[p];
-        g->cfg[p] = g->persist ? g->pcfg : pick_cfg(g->d, g->M[p], g->N[p], g->K[p], force_cfg);
+        slice_bytes += (int64_t)dd * (Mp[p] + Np[p]) * Kp[p];
+        const int c = pick_cfg(dd, g->M[p], g->N[p], g->K[p], force_cfg);
+        g->cfg[p] = !g->persist ? c : (pforce >= 0 && persist_cfg_supported(dd, pforce) ? pforce : persist_cfg_for(dd, c));
     }
     const int64_t e_bytes = round_up(nE * 4, 1024);
     GCHK(hipMalloc(&g->ws, (size_t)(e_bytes + slice_bytes)));
@@ -484,16 +522,18 @@ int group_build(ozaie_group g, int force_cfg) {
     std::memset(g->h_ops, 0, sizeof(GroupOperand) * 2 * n);
     std::memset(g->h_probs, 0, sizeof(GemmLaunch) * n);
     std::vector<int4> imax, isplit;
-    std::vector<int4> igemm[GCFG_COUNT];
+    std::vector<int4> igemm[4][GCFG_COUNT];
     int64_t eoff = 0, soff = e_bytes;
     for (int p = 0; p < n; p++) {
+        const int dd = g->dp[p];
         GroupOperand &a = g->h_ops[2 * p], &bo = g->h_ops[2 * p + 1];
         a.R = g->M[p]; a.K = g->K[p]; a.Rp = Mp[p]; a.sr = g->K[p]; a.sk = 1;
         bo.R = g->N[p]; bo.K = g->K[p]; bo.Rp = Np[p]; bo.sr = 1; bo.sk = g->N[p];
+        a.d = bo.d = dd;
         a.E = g->E_all + eoff; eoff += Mp[p];
         bo.E = g->E_all + eoff; eoff += Np[p];
-        a.S = reinterpret_cast<int8_t *>(g->ws + soff); soff += (int64_t)g->d * Mp[p] * Kp[p];
-        bo.S = reinterpret_cast<int8_t *>(g->ws + soff); soff += (int64_t)g->d * Np[p] * Kp[p];
+        a.S = reinterpret_cast<int8_t *>(g->ws + soff); soff += (int64_t)dd * Mp[p] * Kp[p];
+        bo.S = reinterpret_cast<int8_t *>(g->ws + soff); soff += (int64_t)dd * Np[p] * Kp[p];
         a.Rb = Mp[p] / kBlk; bo.Rb = Np[p] / kBlk;
         a.Kb = bo.Kb = Kp[p] / kBlk;
         // exponent max: rows of A (one wave per row), columns of B (slabs of 256 x 256)
@@ -512,7 +552,7 @@ int group_build(ozaie_group g, int force_cfg) {
         GemmLaunch &gl = g->h_probs[p];
         gl.SA = a.S; gl.SB = bo.S; gl.Er = a.E; gl.Ec = bo.E;
         gl.RbA = a.Rb; gl.RbB = bo.Rb; gl.Kb = a.Kb; gl.kb0 = 0; gl.kb1 = a.Kb;
-        gl.M = (int)g->M[p]; gl.N = (int)g->N[p]; gl.b = g->b; gl.d = g->d; gl.k0 = 0;
+        gl.M = (int)g->M[p]; gl.N = (int)g->N[p]; gl.b = g->b; gl.d = dd; gl.k0 = 0;
         gl.rounding = g->rounding; gl.first = 1; gl.last = 1; gl.ldc = g->N[p];
         gl.K_true = (int)g->K[p];
         const int c = g->cfg[p];
@@ -521,7 +561,7 @@ int group_build(ozaie_group g, int force_cfg) {
         for (int64_t t = 0; t < ntm * ntn; t++) {
             int tm, tn;
             band_map((int)t, (int)ntm, (int)ntn, tm, tn);
-            igemm[c].push_back(int4{p, tm, tn, 0});
+            igemm[dslot(dd)][c].push_back(int4{p, tm, tn, 0});
         }
         g->tiles += ntm * ntn;
     }
@@ -544,10 +584,11 @@ int group_build(ozaie_group g, int force_cfg) {
     if ((rc = upload(&g->d_items_split, isplit))) return rc;
     g->n_max = (int)imax.size();
     g->n_split = (int)isplit.size();
-    for (int c = 0; c < GCFG_COUNT; c++) {
-        if ((rc = upload(&g->d_items_gemm[c], igemm[c]))) return rc;
-        g->n_gemm[c] = (int)igemm[c].size();
-    }
+    for (int ds = 0; ds < 4; ds++)
+        for (int c = 0; c < GCFG_COUNT; c++) {
+            if ((rc = upload(&g->d_items_gemm[ds][c], igemm[ds][c]))) return rc;
+            g->n_gemm[ds][c] = (int)igemm[ds][c].size();
+        }
     return OZAIE_OK;
 }
 
@@ -562,14 +603,15 @@ int group_enqueue(ozaie_group g, hipStream_t st) {
         return OZAIE_OK;
     }
     if (g->persist) {
-        GCHK(launch_group_persistent(g->PL, g->d, g->pcfg, st));
+        GCHK(launch_group_persistent(g->PL, st));
         return OZAIE_OK;
     }
     GCHK(launch_fill_i32(g->E_all, g->n_E, 0, st));
     GCHK(launch_group_maxexp(g->d_ops, g->d_items_max, g->n_max, st));
-    GCHK(launch_group_split(g->d_ops, g->d_items_split, g->n_split, g->b, g->d, st));
-    for (int c = 0; c < GCFG_COUNT; c++)
-        GCHK(launch_group_gemm(g->d_probs, g->d_items_gemm[c], g->n_gemm[c], g->d, c, st));
+    GCHK(launch_group_split(g->d_ops, g->d_items_split, g->n_split, g->b, st));
+    for (int ds = 0; ds < 4; ds++)
+        for (int c = 0; c < GCFG_COUNT; c++)
+            GCHK(launch_group_gemm(g->d_probs, g->d_items_gemm[ds][c], g->n_gemm[ds][c], kGroupSliceCounts[ds], c, st));
     return OZAIE_OK;
 }
 
@@ -605,7 +647,7 @@ int group_new(ozaie_group *out, int count, const int64_t *M, const int64_t *N, c
             return OZAIE_ERR_ARG;
         }
         if (want >= 0) force_cfg = want;
-    } else if (bits < 1 || bits > kMaxBits || slices < 1 || slices > kMaxSlices || bits * slices > 96) {
+    } else if (bits < 1 || bits > kMaxBits || slices < 0 || slices > kMaxSlices || bits * slices > 96) {
         return OZAIE_ERR_ARG;
     }
     for (int p = 0; p < count; p++)
@@ -624,21 +666,23 @@ int group_new(ozaie_group *out, int count, const int64_t *M, const int64_t *N, c
     g->N.assign(N, N + count);
     g->K.assign(K, K + count);
     g->cfg.assign(count, 0);
+    // slice count per problem: the group's, or (slices = 0) its size class's from the table
+    g->dp.assign(count, slices);
+    if (!crt && slices == 0)
+        for (int p = 0; p < count; p++) g->dp[p] = pick_slices(M[p], N[p], K[p]);
     int rc = OZAIE_OK;
     hipError_t e = hipGetDevice(&g->device);
     if (e != hipSuccess) rc = set_error(e, "hipGetDevice");
     g->crt = crt;
-    g->fallback = !crt && !group_cfg_supported(slices, GCFG_D11);
-    if (!crt && !g->fallback && (flags & 4)) {
-        // persistent single launch: forced configuration if it has a persistent kernel
-        const int want = ((flags >> 4) & 15) - 1 >= 0 ? ((flags >> 4) & 15) - 1 : force_cfg;
-        g->pcfg = (want >= 0 && persist_cfg_supported(slices, want)) ? want : default_persist_cfg(slices);
-        g->persist = persist_cfg_supported(slices, g->pcfg);
-    }
+    g->fallback = false;
+    if (!crt)
+        for (int p = 0; p < count; p++) g->fallback |= !group_cfg_supported(g->dp[p], GCFG_D11);
+    // persistent single launch: every problem has a persistent tile body (any of 3, 7, 8, 9)
+    g->persist = !crt && !g->fallback && (flags & OZAIE_GROUP_PERSISTENT);
     if (!rc && g->fallback) {
         for (int p = 0; p < count && !rc; p++) {
             ozaie_plan pl = nullptr;
-            rc = ozaie_plan_create(&pl, M[p], N[p], K[p], bits, slices, flags & 1);
+            rc = ozaie_plan_create(&pl, M[p], N[p], K[p], bits, g->dp[p], flags & 1);
             if (!rc) g->plans.push_back(pl);
         }
         if (!rc) {
@@ -807,7 +851,8 @@ int32_t ozaie_group_query(ozaie_group g, int32_t *out4) {
     }
     else {
         launches = 3;
-        for (int c = 0; c < GCFG_COUNT; c++) launches += g->n_gemm[c] > 0;
+        for (int ds = 0; ds < 4; ds++)
+            for (int c = 0; c < GCFG_COUNT; c++) launches += g->n_gemm[ds][c] > 0;
     }
     out4[0] = launches;
     out4[1] = g->exec != nullptr;
@@ -847,6 +892,24 @@ int32_t ozaie_group_destroy(ozaie_group g) {
         (void)hipDeviceSynchronize();
     }
     group_free(g);
+    return OZAIE_OK;
+}
+
+int32_t ozaie_tune_get_slices(int32_t *table9) {
+    if (!table9) return OZAIE_ERR_ARG;
+    (void)pick_slices(1, 1, 1);   // initialises the table
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (int c = 0; c < 9; c++) table9[c] = g_tune_slices[c];
+    return OZAIE_OK;
+}
+
+int32_t ozaie_tune_set_slices(const int32_t *table9) {
+    if (!table9) return OZAIE_ERR_ARG;
+    for (int c = 0; c < 9; c++)
+        if (table9[c] != -1 && !group_cfg_supported(table9[c], GCFG_D11)) return OZAIE_ERR_ARG;
+    (void)pick_slices(1, 1, 1);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (int c = 0; c < 9; c++) g_tune_slices[c] = table9[c];
     return OZAIE_OK;
 }
 

@@ -23,15 +23,15 @@ __global__ __launch_bounds__(256) void k_grp_maxexp(const GroupOperand *__restri
 }
 
 __global__ __launch_bounds__(256) void k_grp_split(const GroupOperand *__restrict__ ops,
-                                                   const int4 *__restrict__ items, int b, int d) {
+                                                   const int4 *__restrict__ items, int b) {
     __shared__ double tile[kSplitLds / 8];
     const int4 it = items[blockIdx.x];
     const GroupOperand &o = ops[it.x];
     const int32_t *Erow = o.E + (int64_t)it.y * 32;
     if (o.sk == 1)
-        split_tile<true>(o.X, o.sr, o.sk, o.R, o.K, Erow, b, d, o.S, o.Rb, o.Kb, it.y, it.z, tile);
+        split_tile<true>(o.X, o.sr, o.sk, o.R, o.K, Erow, b, o.d, o.S, o.Rb, o.Kb, it.y, it.z, tile);
     else
-        split_tile<false>(o.X, o.sr, o.sk, o.R, o.K, Erow, b, d, o.S, o.Rb, o.Kb, it.y, it.z, tile);
+        split_tile<false>(o.X, o.sr, o.sk, o.R, o.K, Erow, b, o.d, o.S, o.Rb, o.Kb, it.y, it.z, tile);
 }
 
 hipError_t launch_group_maxexp(const GroupOperand *ops, const int4 *items, int n, hipStream_t st) {
@@ -40,9 +40,9 @@ hipError_t launch_group_maxexp(const GroupOperand *ops, const int4 *items, int n
     return hipGetLastError();
 }
 
-hipError_t launch_group_split(const GroupOperand *ops, const int4 *items, int n, int b, int d, hipStream_t st) {
+hipError_t launch_group_split(const GroupOperand *ops, const int4 *items, int n, int b, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_grp_split, dim3((unsigned)n), dim3(256), 0, st, ops, items, b, d);
+    hipLaunchKernelGGL(k_grp_split, dim3((unsigned)n), dim3(256), 0, st, ops, items, b);
     return hipGetLastError();
 }
 
@@ -330,24 +330,102 @@ __device__ __forceinline__ PTile ptile_setup(const GemmLaunch *__restrict__ prob
     return t;
 }
 
-template <int D>
-__host__ __device__ constexpr int persist_lds_bytes() {
-    constexpr int a = 4 * 32 * kWaveTileLd * 8;   // phase A: one tile per wave
-    return a > direct_tile_bytes<D>() ? a : direct_tile_bytes<D>();
+// LDS of the persistent kernel: phase A's per-wave tiles (larger than any phase B epilogue
+// scratch of the tile bodies it runs, d <= 9).
+constexpr int kPersistLds = 4 * 32 * kWaveTileLd * 8;
+static_assert(kPersistLds >= direct_tile_bytes<9>(), "phase B epilogue scratch");
+
+// Phase B tiles [i0, i1) of one workgroup's list, all of one (D, WM, WN): fragments
+// double-buffered in registers (direct loads), the first block of the next tile of the
+// run loaded during the current tile's last block and epilogue.
+template <int D, int WM, int WN>
+__device__ __forceinline__ void persist_tiles(const GemmLaunch *__restrict__ probs, const int4 *__restrict__ itemsB,
+                                              int i0, int i1, int b, int rounding, int8_t *smem) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = w & 1, wn = w >> 1;
+    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * wave_scratch_words<D, WM, WN>();
+    const int loff = lane * 16;
+
+    v4i a0[D][WM], b0[D][WN], a1[D][WM], b1[D][WN];
+    auto load = [&](v4i (&af)[D][WM], v4i (&bf)[D][WN], const PTile &t, int kb) {
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+#pragma unroll
+            for (int mi = 0; mi < WM; mi++)
+                af[j][mi] = *reinterpret_cast<const v4i *>(t.pa + j * t.planeA + mi * t.rowstride +
+                                                          (int64_t)kb * kBlkBytes + loff);
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++)
+                bf[j][ni] = *reinterpret_cast<const v4i *>(t.pb + j * t.planeB + ni * t.rowstride +
+                                                          (int64_t)kb * kBlkBytes + loff);
+        }
+    };
+    v16i acc[D][WM][WN];
+    auto mma = [&](const v4i (&af)[D][WM], const v4i (&bf)[D][WN]) {
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; i + j < D; j++)
+#pragma unroll
+                for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+                    for (int ni = 0; ni < WN; ni++)
+                        acc[i + j][mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i][mi], bf[j][ni],
+                                                                                   acc[i + j][mi][ni], 0, 0, 0);
+    };
+
+    PTile cur{}, nxt{};
+    cur = ptile_setup<WM, WN>(probs, itemsB[i0], wm, wn);
+    load(a0, b0, cur, 0);
+    for (int i = i0; i < i1; i++) {
+        const bool has_next = i + 1 < i1;
+        if (has_next) nxt = ptile_setup<WM, WN>(probs, itemsB[i + 1], wm, wn);
+        GemmLaunch g{};
+        g.M = cur.M;
+        g.N = cur.N;
+        g.b = b;
+        g.d = D;
+        g.rounding = rounding;
+        g.first = 1;
+        g.last = 1;
+        g.C = cur.C;
+        g.ldc = cur.ldc;
+        g.Er = cur.Er;
+        g.Ec = cur.Ec;
+        int er_v, ec_v;
+        load_tile_exps(g, EPI_C, lane, cur.rbase, 32 * WM, cur.cbase, 32 * WN, er_v, ec_v);
+#pragma unroll
+        for (int k = 0; k < D; k++)
+#pragma unroll
+            for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+                for (int ni = 0; ni < WN; ni++) acc[k][mi][ni] = v16i{};
+        const int nk = cur.nk;   // even
+        for (int t = 0; t < nk; t += 2) {
+            load(a1, b1, cur, t + 1);
+            mma(a0, b0);
+            if (t + 2 < nk) load(a0, b0, cur, t + 2);
+            else if (has_next) load(a0, b0, nxt, 0);   // next tile's first block
+            mma(a1, b1);
+        }
+        wave_epilogue<D, WM, WN>(g, EPI_C, 0, acc, scratch, lane, cur.rbase, cur.cbase, er_v, ec_v);
+        cur = nxt;
+    }
 }
 
-template <int D, int WM, int WN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_grp_persistent(const GroupOperand *__restrict__ ops, const int4 *__restrict__ itemsA,
                       const int *__restrict__ offA, const GemmLaunch *__restrict__ probs,
                       const int4 *__restrict__ itemsB, const int *__restrict__ offB, PersistCtl *ctl,
                       int b, int rounding) {
-    __shared__ __attribute__((aligned(1024))) int8_t smem[persist_lds_bytes<D>()];
+    __shared__ __attribute__((aligned(1024))) int8_t smem[kPersistLds];
     __shared__ int flag_lds;
     const int G = gridDim.x;
 
-    // ---- phase A: exponent maximum + split, one 32-row block per item, each wave working
-    // through its own item list (no workgroup barriers)
+    // ---- phase A: exponent maximum + split (each operand with its problem's slice count),
+    // one 32-row block per item, each wave working through its own item list (no
+    // workgroup barriers)
     {
         const int lane = threadIdx.x & 63;
         const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -357,88 +435,35 @@ void k_grp_persistent(const GroupOperand *__restrict__ ops, const int4 *__restri
         for (int i = a0; i < a1; i++) {
             const int4 it = itemsA[i];
             const GroupOperand &o = ops[it.x];
-            if (o.sk == 1) wave_rowblock<true>(o, it.y, b, D, wt, lane);
-            else wave_rowblock<false>(o, it.y, b, D, wt, lane);
+            if (o.sk == 1) wave_rowblock<true>(o, it.y, b, o.d, wt, lane);
+            else wave_rowblock<false>(o, it.y, b, o.d, wt, lane);
         }
     }
 
     const bool ok = grid_barrier(ctl, G, &flag_lds);
 
-    // ---- phase B: fused slice-GEMM tiles
+    // ---- phase B: fused slice-GEMM tiles in runs of one (d, tile shape)
     if (ok) {
-        const int lane = threadIdx.x & 63;
-        const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-        const int wm = w & 1, wn = w >> 1;
-        int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * wave_scratch_words<D, WM, WN>();
-        const int i0 = offB[blockIdx.x], i1 = offB[blockIdx.x + 1];
-        const int loff = lane * 16;
-
-        v4i a0[D][WM], b0[D][WN], a1[D][WM], b1[D][WN];
-        auto load = [&](v4i (&af)[D][WM], v4i (&bf)[D][WN], const PTile &t, int kb) {
-#pragma unroll
-            for (int j = 0; j < D; j++) {
-#pragma unroll
-                for (int mi = 0; mi < WM; mi++)
-                    af[j][mi] = *reinterpret_cast<const v4i *>(t.pa + j * t.planeA + mi * t.rowstride +
-                                                              (int64_t)kb * kBlkBytes + loff);
-#pragma unroll
-                for (int ni = 0; ni < WN; ni++)
-                    bf[j][ni] = *reinterpret_cast<const v4i *>(t.pb + j * t.planeB + ni * t.rowstride +
-                                                              (int64_t)kb * kBlkBytes + loff);
+        __syncthreads();   // phase A's LDS tiles are dead before the epilogue scratch reuses them
+        const int i1 = offB[blockIdx.x + 1];
+        for (int i = offB[blockIdx.x]; i < i1;) {
+            const int code = itemsB[i].w;
+            int j = i + 1;
+            while (j < i1 && itemsB[j].w == code) j++;
+            switch (code) {
+                case 3 * 8 + GCFG_D11: persist_tiles<3, 1, 1>(probs, itemsB, i, j, b, rounding, smem); break;
+                case 3 * 8 + GCFG_D12: persist_tiles<3, 1, 2>(probs, itemsB, i, j, b, rounding, smem); break;
+                case 3 * 8 + GCFG_D22: persist_tiles<3, 2, 2>(probs, itemsB, i, j, b, rounding, smem); break;
+                case 7 * 8 + GCFG_D11: persist_tiles<7, 1, 1>(probs, itemsB, i, j, b, rounding, smem); break;
+                case 7 * 8 + GCFG_D12: persist_tiles<7, 1, 2>(probs, itemsB, i, j, b, rounding, smem); break;
+                case 8 * 8 + GCFG_D11: persist_tiles<8, 1, 1>(probs, itemsB, i, j, b, rounding, smem); break;
+                case 9 * 8 + GCFG_D11: persist_tiles<9, 1, 1>(probs, itemsB, i, j, b, rounding, smem); break;
+                default: break;   // the host builds only the codes above
             }
-        };
-        v16i acc[D][WM][WN];
-        auto mma = [&](const v4i (&af)[D][WM], const v4i (&bf)[D][WN]) {
-#pragma unroll
-            for (int i = 0; i < D; i++)
-#pragma unroll
-                for (int j = 0; i + j < D; j++)
-#pragma unroll
-                    for (int mi = 0; mi < WM; mi++)
-#pragma unroll
-                        for (int ni = 0; ni < WN; ni++)
-                            acc[i + j][mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i][mi], bf[j][ni],
-                                                                                       acc[i + j][mi][ni], 0, 0, 0);
-        };
-
-        PTile cur{}, nxt{};
-        if (i0 < i1) {
-            cur = ptile_setup<WM, WN>(probs, itemsB[i0], wm, wn);
-            load(a0, b0, cur, 0);
-        }
-        for (int i = i0; i < i1; i++) {
-            const bool has_next = i + 1 < i1;
-            if (has_next) nxt = ptile_setup<WM, WN>(probs, itemsB[i + 1], wm, wn);
-            GemmLaunch g{};
-            g.M = cur.M;
-            g.N = cur.N;
-            g.b = b;
-            g.d = D;
-            g.rounding = rounding;
-            g.first = 1;
-            g.last = 1;
-            g.C = cur.C;
-            g.ldc = cur.ldc;
-            g.Er = cur.Er;
-            g.Ec = cur.Ec;
-            int er_v, ec_v;
-            load_tile_exps(g, EPI_C, lane, cur.rbase, 32 * WM, cur.cbase, 32 * WN, er_v, ec_v);
-#pragma unroll
-            for (int k = 0; k < D; k++)
-#pragma unroll
-                for (int mi = 0; mi < WM; mi++)
-#pragma unroll
-                    for (int ni = 0; ni < WN; ni++) acc[k][mi][ni] = v16i{};
-            const int nk = cur.nk;   // even
-            for (int t = 0; t < nk; t += 2) {
-                load(a1, b1, cur, t + 1);
-                mma(a0, b0);
-                if (t + 2 < nk) load(a0, b0, cur, t + 2);
-                else if (has_next) load(a0, b0, nxt, 0);   // next tile's first block
-                mma(a1, b1);
-            }
-            wave_epilogue<D, WM, WN>(g, EPI_C, 0, acc, scratch, lane, cur.rbase, cur.cbase, er_v, ec_v);
-            cur = nxt;
+            // the next run lays out the waves' epilogue scratch differently: every wave has
+            // finished this run's epilogues before any starts the next run's
+            __syncthreads();
+            i = j;
         }
     }
 
@@ -467,27 +492,11 @@ bool persist_cfg_supported(int d, int cfg) {
 
 int persist_ctl_bytes() { return (int)sizeof(PersistCtl); }
 
-template <int D, int WM, int WN>
-static hipError_t launch_persist_t(const PersistLaunch &L, hipStream_t st) {
-    hipLaunchKernelGGL((k_grp_persistent<D, WM, WN>), dim3((unsigned)L.grid), dim3(256), 0, st, L.ops, L.itemsA,
-                       L.offA, L.probs, L.itemsB, L.offB, reinterpret_cast<PersistCtl *>(L.ctl), L.b,
-                       L.rounding);
-    return hipGetLastError();
-}
-
-hipError_t launch_group_persistent(const PersistLaunch &L, int d, int cfg, hipStream_t st) {
+hipError_t launch_group_persistent(const PersistLaunch &L, hipStream_t st) {
     if (L.grid <= 0) return hipSuccess;
-    if (!persist_cfg_supported(d, cfg)) return hipErrorInvalidValue;
-    switch (d * 8 + cfg) {
-        case 3 * 8 + GCFG_D11: return launch_persist_t<3, 1, 1>(L, st);
-        case 3 * 8 + GCFG_D12: return launch_persist_t<3, 1, 2>(L, st);
-        case 3 * 8 + GCFG_D22: return launch_persist_t<3, 2, 2>(L, st);
-        case 7 * 8 + GCFG_D11: return launch_persist_t<7, 1, 1>(L, st);
-        case 7 * 8 + GCFG_D12: return launch_persist_t<7, 1, 2>(L, st);
-        case 8 * 8 + GCFG_D11: return launch_persist_t<8, 1, 1>(L, st);
-        case 9 * 8 + GCFG_D11: return launch_persist_t<9, 1, 1>(L, st);
-        default: return hipErrorInvalidValue;
-    }
+    hipLaunchKernelGGL(k_grp_persistent, dim3((unsigned)L.grid), dim3(256), 0, st, L.ops, L.itemsA, L.offA, L.probs,
+                       L.itemsB, L.offB, reinterpret_cast<PersistCtl *>(L.ctl), L.b, L.rounding);
+    return hipGetLastError();
 }
 
 }  // namespace oz

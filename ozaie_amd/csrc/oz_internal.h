@@ -73,6 +73,7 @@ struct GroupOperand {
     int32_t *E;
     int8_t *S;
     int64_t Rb, Kb;
+    int d, pad;         // slice count of this operand's problem (split scheme)
 };
 
 // Tile configurations of the grouped fused GEMM (wave tile (32 WM) x (32 WN)).
@@ -117,12 +118,15 @@ constexpr int kPairRowPad = 256;
 // Grouped launchers: items are int4 {operand, a, b, c} for the split phase and
 // int4 {problem, tile row, tile column, 0} for the GEMM phase.
 hipError_t launch_group_maxexp(const GroupOperand *ops, const int4 *items, int n, hipStream_t st);
-hipError_t launch_group_split(const GroupOperand *ops, const int4 *items, int n, int b, int d, hipStream_t st);
+hipError_t launch_group_split(const GroupOperand *ops, const int4 *items, int n, int b, hipStream_t st);
 hipError_t launch_group_gemm(const GemmLaunch *probs, const int4 *items, int n, int d, int cfg, hipStream_t st);
 // Persistent grouped launch (one kernel: split phase, grid barrier, GEMM phase).  Phase A
 // lists are per wave (items [offA[4w+v], offA[4w+v+1]) for wave v of workgroup w), phase B
 // lists per workgroup (items [offB[w], offB[w+1])); phase A items are
-// int4 {operand, row block, 0, 0}, phase B items int4 {problem, tile row, tile col, 0}.
+// int4 {operand, row block, 0, 0}, phase B items int4 {problem, tile row, tile col,
+// persist_code(d, cfg)}, each workgroup's list ordered so that items of one code are
+// consecutive (one tile body per run of items, the next tile's first block prefetched
+// inside a run).
 // ctl is persist_ctl_bytes() of zero-initialised device memory, reset by the kernel itself
 // at the end of every launch; its third int is a sticky error flag (barrier wait bound).
 struct PersistLaunch {
@@ -136,8 +140,9 @@ struct PersistLaunch {
     int grid, b, rounding;
 };
 bool persist_cfg_supported(int d, int cfg);
+inline int persist_code(int d, int cfg) { return d * 8 + cfg; }
 int persist_ctl_bytes();
-hipError_t launch_group_persistent(const PersistLaunch &L, int d, int cfg, hipStream_t st);
+hipError_t launch_group_persistent(const PersistLaunch &L, hipStream_t st);
 // Tile order inside a problem: bands of 8 tile-rows, so tiles that run together share
 // rows of A (device tile mapping and the grouped planner).
 __host__ __device__ inline void band_map(int lin, int ntm, int ntn, int &tm, int &tn) {

@@ -163,3 +163,16 @@ def test_sweep_mode_table_and_resume_keys(tmp_path):
     assert loaded == [rec]
     assert sweep.record_key(loaded[0]) == sweep.record_key({"kind": "accuracy", "K": 128, "phi": 1.0,
                                                             "seed": 0, "mode": "fast"})
+
+
+def test_slice_table_validation():
+    """The per-size-class slice-count table of groups created with slices=0 accepts the
+    grouped kernels' counts (3, 7, 8, 9) and -1 (untuned), and nothing else."""
+    old = oz.slice_table()
+    try:
+        oz.set_slice_table([3, 7, 8, 9, -1, 9, 8, 7, 3])
+        assert list(oz.slice_table()) == [3, 7, 8, 9, -1, 9, 8, 7, 3]
+        assert oz.lib().ozaie_tune_set_slices(oz._ptr(__import__("numpy").array([5] * 9, dtype="int32"))) == oz.ERR_ARG
+        assert oz.size_class(100, 2000, 300) == 3 * 2 + 1
+    finally:
+        oz.set_slice_table(old)
