@@ -519,26 +519,63 @@ def grouped(args, torch, np, oz, slices=(3, 7, 8, 9), count=512):
                       "speedup_vs_native_graph": entry[best]["speedup_vs_native_graph"]})
         rec["presets"].append(entry)
         oz.set_tune_table(d, [-1] * 9)
+    def run_modes(entry, make):
+        """Time the multi-launch and the persistent form of one preset; both must give the
+        same C, and a mode counts only if its run completed (no barrier overrun)."""
+        ref = None
+        for mode in ("multi", "persistent"):
+            grp = make(mode == "persistent")
+            t = time_steps(torch, lambda: grp(As, Bs, Cs), 5, 3)
+            q = grp.query()
+            status = grp.status()
+            err = errs()
+            if ref is None:
+                ref = [c.clone() for c in Cs]
+                same = True
+            else:
+                same = all(torch.equal(a, b) for a, b in zip(Cs, ref))
+            entry[mode] = {"ms": t, "tflops": flop / t / 1e9, "speedup": t_nat / t,
+                           "speedup_vs_native_graph": (t_nat_graph / t) if t_nat_graph else None,
+                           "launches": q["launches"], "graph": q["graph"], "workgroups": q["workgroups"],
+                           "barrier_error": status, "identical_to_multi": same}
+            entry["max_rel_err"], entry["median_rel_err"] = err
+            log(args, f"  grouped {entry.get('label')} {mode}: {t:.2f} ms x{t_nat / t:.2f} err {err} "
+                      f"launches {q['launches']} same {same} status {status}")
+            grp.close()
+        valid = [m for m in ("multi", "persistent")
+                 if entry[m]["barrier_error"] == 0 and entry[m]["identical_to_multi"]]
+        best = min(valid, key=lambda m: entry[m]["ms"])
+        entry.update({"best_mode": best, "ms": entry[best]["ms"], "tflops": entry[best]["tflops"],
+                      "speedup": entry[best]["speedup"],
+                      "speedup_vs_native_graph": entry[best]["speedup_vs_native_graph"]})
+
+    # per size class slice counts: the calibration tuner picks, per class, the cheapest count
+    # whose error on calibration products matches native DGEMM's; tiles tuned per chosen count
+    try:
+        from ozaie_amd import tune
+        t0 = time.time()
+        stable, sdet = tune.tune_slices(phi=1.0, seeds=(0, 1), sample=32)
+        entry = {"label": "auto", "bits": 7, "slices": "per size class", "slice_table": [int(x) for x in stable],
+                 "calibration_s": time.time() - t0,
+                 "calibration": [{"shape": list(x["shape"]), "native_max_rel_err": x["native_max_rel_err"],
+                                  "slices": x["slices"]} for x in sdet]}
+        run_modes(entry, lambda pers: oz.Group(shapes, 7, 0, persistent=pers))
+        rec["presets"].append(entry)
+        oz.set_slice_table([-1] * 9)
+        for d_ in set(stable):
+            oz.set_tune_table(d_, [-1] * 9)
+    except Exception as e:  # reported, never required
+        log(args, f"  grouped auto: {e!r}")
     for mods in crt_list(args):
-        # CRT mode: per-phase launches, tile shape per size class from the CRT autotuner
+        # CRT mode: tile shape per size class from the CRT autotuner; per-phase launches and
+        # the single persistent launch
         t0 = time.time()
         table = [int(x) for x in oz.autotune(0, mods, reps=2, crt=True)]
         t_tune = time.time() - t0
-        grp = oz.Group(shapes, 0, mods, crt=True)
-        t = time_steps(torch, lambda: grp(As, Bs, Cs), 5, 3)
-        q = grp.query()
-        err = errs()
-        entry = {"scheme": "crt", "moduli": mods, "bits": "per problem: crt_beta(moduli, K)",
-                 "autotune_table": table, "autotune_s": t_tune, "best_mode": "multi",
-                 "multi": {"ms": t, "tflops": flop / t / 1e9, "speedup": t_nat / t,
-                           "speedup_vs_native_graph": (t_nat_graph / t) if t_nat_graph else None,
-                           "launches": q["launches"], "graph": q["graph"], "workgroups": q["workgroups"]},
-                 "ms": t, "tflops": flop / t / 1e9, "speedup": t_nat / t,
-                 "speedup_vs_native_graph": (t_nat_graph / t) if t_nat_graph else None,
-                 "max_rel_err": err[0], "median_rel_err": err[1]}
-        log(args, f"  grouped crt {mods}: {t:.2f} ms x{t_nat / t:.2f} err {err} launches {q['launches']}"
-                  f" table {table}")
-        grp.close()
+        entry = {"label": f"crt{mods}", "scheme": "crt", "moduli": mods,
+                 "bits": "per problem: crt_beta(moduli, K)", "autotune_table": table, "autotune_s": t_tune}
+        run_modes(entry, lambda pers: oz.Group(shapes, 0, mods, rounding=oz.ROUND_NEAREST, crt=True,
+                                               persistent=pers))
         rec["presets"].append(entry)
         oz.set_tune_table(mods, [-1] * 9, crt=True)
     log(args, f"  grouped native {t_nat:.2f} ms (graph {t_nat_graph}) err {nat_err}")

@@ -159,7 +159,11 @@ int32_t ozaie_dgemm_noalloc(int64_t M, int64_t N, int64_t K, int bits, int slice
  * each problem's K (crt_beta(moduli, K_i)) or a fixed count no larger than any problem's;
  * K_i <= 65536.  Four phase launches plus one modulus-GEMM launch per tile shape (ids as
  * for ozaie_autotune_crt; flags bits 4..7 = forced shape + 1), same graph capture; each
- * C_i is bit-identical to a CRT plan of that problem.  Not combinable with bit 2.
+ * C_i is bit-identical to a CRT plan of that problem.  With bit 2 as well, ONE persistent
+ * launch (two workgroups per CU): exponent maxima + residues of every operand, a grid
+ * barrier, every modulus-GEMM tile (each problem in its size class's tile shape, 256 x 256
+ * run as 128 x 128 tiles, all moduli per tile), a second grid barrier, the rebuild of
+ * every C_i.
  * Runs of one group must not overlap (they share the workspace and the control block). */
 #define OZAIE_GROUP_NEAREST 1
 #define OZAIE_GROUP_NO_GRAPH 2

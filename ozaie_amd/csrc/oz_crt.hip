@@ -10,32 +10,21 @@ namespace oz {
 // multiplied modulo n pairwise-coprime moduli p <= 256 with one exact int8 GEMM each,
 // and the Chinese remainder theorem rebuilds sum_k q_a q_b exactly (|C'| <= M / 4).
 
-// Residues of the staged tile: thread (row, q) owns 16 consecutive k of one row and
-// writes one 16-byte lane chunk per modulus plane.  With q = +-(hi 2^31 + lo), the value
-// x = +-(hi (2^31 mod p) + lo) is an exact double (|x| < 2^40) congruent to q, and
-// x - p n, n the integer nearest x / p, is its residue in [-(p-1)/2, (p-1)/2] for odd p:
-// x / p is never within 1 / (2p) of a half-integer, far above the error of the product
-// x fl(1/p), which fma(x, fl(1/p), 1.5 2^52) rounds to an integer; the residue plus
-// 1.5 2^52 holds it in two's complement in its low word.  For p = 256 (plane 0) the residue
-// is the low byte of q in two's complement.
-template <bool KC>
-__device__ __forceinline__ void crt_emit(const double *tile, int64_t R, const int32_t *__restrict__ Erow,
-                                         const CrtConsts &c, int beta, int8_t *__restrict__ S, int64_t Rb,
-                                         int64_t Kb, int64_t rb, int64_t kt) {
-    constexpr int TK = 128;
-    constexpr int LDW = KC ? (TK + 1) : (32 + 1);
-    const int t = threadIdx.x;
-    const int64_t r0 = rb * 32;
-    const int row = t & 31, q = t >> 5;
-    const int64_t kb = kt * 4 + (q >> 1);
-    if (kb >= Kb) return;
-    const int emax = (r0 + row < R) ? Erow[row] : 0;
+// Residues of 16 consecutive elements v[i] of one row (row maximum exponent field emax,
+// beta bits) into one 16-byte chunk per modulus plane at dst (planes `plane` bytes apart).
+// With q = +-(hi 2^31 + lo), the value x = +-(hi (2^31 mod p) + lo) is an exact double
+// (|x| < 2^40) congruent to q, and x - p n, n the integer nearest x / p, is its residue in
+// [-(p-1)/2, (p-1)/2] for odd p: x / p is never within 1 / (2p) of a half-integer, far
+// above the error of the product x fl(1/p), which fma(x, fl(1/p), 1.5 2^52) rounds to an
+// integer; the residue plus 1.5 2^52 holds it in two's complement in its low word.  For
+// p = 256 (plane 0) the residue is the low byte of q in two's complement.
+__device__ __forceinline__ void crt_emit16(const double (&v)[16], int emax, int beta, const CrtConsts &c,
+                                           int8_t *__restrict__ dst, int64_t plane) {
     double hi[16], lo[16];
     uint32_t low8[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        const double v = KC ? tile[row * LDW + 16 * q + i] : tile[(16 * q + i) * LDW + row];
-        const uint64_t u = (uint64_t)__double_as_longlong(v);
+        const uint64_t u = (uint64_t)__double_as_longlong(v[i]);
         const int e = (int)((u >> 52) & 0x7FF);
         const uint64_t mx = (u & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
         const int sh = 53 + emax - (e ? e : 1) - beta;   // subnormals: exponent 1 - 1023
@@ -47,8 +36,6 @@ __device__ __forceinline__ void crt_emit(const double *tile, int64_t R, const in
         const uint32_t qlo = neg ? (uint32_t)(0u - (uint32_t)Q) : (uint32_t)Q;   // low word of q
         low8[i >> 2] |= (qlo & 0xFFu) << (8 * (i & 3));
     }
-    const int64_t plane = Rb * Kb * (int64_t)kBlkBytes;
-    int8_t *dst = S + (rb * Kb + kb) * (int64_t)kBlkBytes + (q & 1) * 512 + row * 16;
     *reinterpret_cast<uint4 *>(dst) = make_uint4(low8[0], low8[1], low8[2], low8[3]);   // p = 256
     for (int j = 1; j < c.n; j++) {
         const double p = c.p[j], inv = c.inv[j], c31 = c.c31[j];
@@ -69,6 +56,27 @@ __device__ __forceinline__ void crt_emit(const double *tile, int64_t R, const in
         }
         *reinterpret_cast<uint4 *>(dst + (int64_t)j * plane) = make_uint4(w[0], w[1], w[2], w[3]);
     }
+}
+
+// Residues of the staged tile: thread (row, q) owns 16 consecutive k of one row and
+// writes one 16-byte lane chunk per modulus plane.
+template <bool KC>
+__device__ __forceinline__ void crt_emit(const double *tile, int64_t R, const int32_t *__restrict__ Erow,
+                                         const CrtConsts &c, int beta, int8_t *__restrict__ S, int64_t Rb,
+                                         int64_t Kb, int64_t rb, int64_t kt) {
+    constexpr int TK = 128;
+    constexpr int LDW = KC ? (TK + 1) : (32 + 1);
+    const int t = threadIdx.x;
+    const int64_t r0 = rb * 32;
+    const int row = t & 31, q = t >> 5;
+    const int64_t kb = kt * 4 + (q >> 1);
+    if (kb >= Kb) return;
+    const int emax = (r0 + row < R) ? Erow[row] : 0;
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) v[i] = KC ? tile[row * LDW + 16 * q + i] : tile[(16 * q + i) * LDW + row];
+    const int64_t plane = Rb * Kb * (int64_t)kBlkBytes;
+    crt_emit16(v, emax, beta, c, S + (rb * Kb + kb) * (int64_t)kBlkBytes + (q & 1) * 512 + row * 16, plane);
 }
 
 template <bool KC>
@@ -558,6 +566,130 @@ hipError_t launch_group_crt_rebuild(const CrtGroupProb *probs, const int4 *items
         hipLaunchKernelGGL(k_grp_crt_rebuild<true>, dim3((unsigned)n), dim3(256), 0, st, probs, items, rounding, c);
     else
         hipLaunchKernelGGL(k_grp_crt_rebuild<false>, dim3((unsigned)n), dim3(256), 0, st, probs, items, rounding, c);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ persistent CRT group
+constexpr int kCrtPersistLds = 4 * 32 * kWaveTileLd * 8;   // phase A's per-wave tiles
+static_assert(kCrtPersistLds >= 64 * 1024, "phase B stages");
+static_assert(2 * kCrtPersistLds <= 160 * 1024, "two workgroups per CU");
+constexpr int kCrtPersistPerCu = 2;   // workgroups per CU of the persistent CRT launch
+
+// Phase B items [i0, i1) of one workgroup, all of one tile shape: every modulus of each
+// tile in turn (the next modulus's first blocks in flight during each epilogue).  At one
+// wave per SIMD the smaller tiles have few MFMAs per contraction block (4 at 128 x 128, 1
+// at 64 x 64), so their loops keep more blocks in flight: the stages of 64 KiB of LDS.
+template <int WM, int WN>
+constexpr int crt_persist_stages() { return (64 * 1024) / ((2 * WM + 2 * WN) * kBlkBytes); }
+
+template <int WM, int WN>
+__device__ __forceinline__ void crt_persist_tiles(const CrtGroupProb *__restrict__ probs,
+                                                  const int4 *__restrict__ items, int i0, int i1,
+                                                  const CrtConsts &c, int8_t *smem) {
+    for (int i = i0; i < i1; i++) {
+        const int4 it = items[i];
+        const CrtGroupProb &pr = probs[it.x];
+        const int64_t rowstride = pr.Kb * kBlkBytes;
+        const int64_t planeA = pr.RbA * rowstride, planeB = pr.RbB * rowstride;
+        const int64_t planeR = pr.RbA * (kBlk / 4) * pr.Np;
+        CrtTileLoop<WM, WN, crt_persist_stages<WM, WN>()> L;
+        L.setup(pr.SA + (int64_t)it.y * 2 * WM * rowstride, pr.SB + (int64_t)it.z * 2 * WN * rowstride, rowstride,
+                pr.Kb);
+        L.prime(smem);
+        for (int j = 0; j < c.n; j++) {
+            v16i acc[WM][WN];
+            L.run(smem, acc);
+            if (j + 1 < c.n) {
+                L.advance(planeA, planeB);
+                L.prime(smem);
+            }
+            crt_tile_store<WM, WN, true>(acc, c.p[j], c.inv[j], pr.R + (int64_t)j * planeR, pr.Np, it.y, it.z, pr.M,
+                                         pr.N);
+        }
+    }
+}
+
+// Two workgroups per CU (two waves per SIMD, 256 registers each): the memory-bound
+// residue and rebuild phases and the small tiles' contraction loops need the second wave
+// to hide latency; the 256 x 256 tile (256 accumulator registers) does not fit and runs as
+// 128 x 128 tiles here (the host maps it).  2 x kCrtPersistLds of LDS fit one CU.
+template <bool WIDE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_grp_crt_persistent(CrtPersistLaunch L, CrtConsts c) {
+    __shared__ __attribute__((aligned(1024))) int8_t smem[kCrtPersistLds];
+    __shared__ int flag_lds;
+    const int G = gridDim.x;
+    PersistCtl *ctl = reinterpret_cast<PersistCtl *>(L.ctl);
+
+    // ---- phase A: exponent maxima + residues, one 32-row block per item, per-wave lists
+    {
+        const int lane = threadIdx.x & 63;
+        const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        double *wt = reinterpret_cast<double *>(smem) + w * (32 * kWaveTileLd);
+        const int slot = blockIdx.x * 4 + w;
+        for (int i = L.offA[slot]; i < L.offA[slot + 1]; i++) {
+            const int4 it = L.itemsA[i];
+            const GroupOperand &o = L.ops[it.x];
+            const int beta = L.probs[it.x >> 1].beta;
+            const int64_t plane = o.Rb * o.Kb * (int64_t)kBlkBytes;
+            auto emit = [&](const double *w_, int row, int c0, int emax, int8_t *dst) {
+                double v[16];
+#pragma unroll
+                for (int e = 0; e < 16; e++) v[e] = w_[row * kWaveTileLd + c0 + e];
+                crt_emit16(v, emax, beta, c, dst, plane);
+            };
+            if (o.sk == 1) wave_rowblock_t<true>(o, it.y, wt, lane, emit);
+            else wave_rowblock_t<false>(o, it.y, wt, lane, emit);
+        }
+    }
+    bool ok = grid_barrier(ctl, G, &flag_lds);
+
+    // ---- phase B: modulus-GEMM tiles in runs of one shape
+    if (ok) {
+        const int i1 = L.offB[blockIdx.x + 1];
+        for (int i = L.offB[blockIdx.x]; i < i1;) {
+            const int cfg = L.itemsB[i].w;
+            int j = i + 1;
+            while (j < i1 && L.itemsB[j].w == cfg) j++;
+            switch (cfg) {
+                case CCFG_22: crt_persist_tiles<2, 2>(L.probs, L.itemsB, i, j, c, smem); break;
+                case CCFG_11: crt_persist_tiles<1, 1>(L.probs, L.itemsB, i, j, c, smem); break;
+                default: break;   // the host builds only the shapes above
+            }
+            __syncthreads();   // the next shape lays out its LDS stages differently
+            i = j;
+        }
+        ok = grid_barrier(ctl, 2 * G, &flag_lds);
+    }
+
+    // ---- phase C: rebuild, 256 (row quad, four columns) groups per item
+    if (ok) {
+        for (int i = L.offC[blockIdx.x]; i < L.offC[blockIdx.x + 1]; i++) {
+            const int4 it = L.itemsC[i];
+            const CrtGroupProb &pr = L.probs[it.x];
+            const uint32_t nc4 = (uint32_t)(pr.N + 3) / 4;
+            const uint32_t idx = (uint32_t)it.y + threadIdx.x;
+            if (idx < (uint32_t)((pr.M + 3) / 4) * nc4) {
+                const uint32_t row4 = idx / nc4;
+                const int64_t col0 = 4 * (int64_t)(idx - row4 * nc4);
+                const int rest = pr.M - 4 * (int)row4;
+                crt_rebuild_cols<WIDE>(pr.R + (int64_t)row4 * pr.Np, pr.RbA * (kBlk / 4) * pr.Np, col0, pr.N,
+                                       rest < 4 ? rest : 4, c, pr.Er + 4 * (int64_t)row4, pr.Ec, pr.beta, L.rounding,
+                                       pr.C + 4 * (int64_t)row4 * pr.ldc, pr.ldc);
+            }
+        }
+    }
+    persist_exit(ctl, G);
+}
+
+int crt_persist_per_cu() { return kCrtPersistPerCu; }
+
+hipError_t launch_group_crt_persistent(const CrtPersistLaunch &L, const CrtConsts &c, hipStream_t st) {
+    if (L.grid <= 0) return hipSuccess;
+    if (c.n > kCrtNarrowMod)
+        hipLaunchKernelGGL(k_grp_crt_persistent<true>, dim3((unsigned)L.grid), dim3(256), 0, st, L, c);
+    else
+        hipLaunchKernelGGL(k_grp_crt_persistent<false>, dim3((unsigned)L.grid), dim3(256), 0, st, L, c);
     return hipGetLastError();
 }
 

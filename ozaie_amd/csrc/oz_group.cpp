@@ -156,6 +156,7 @@ struct ozaie_group_s {
     int4 *d_items_rebuild = nullptr;
     int n_rebuild = 0;
     int crt_mpw[CCFG_COUNT] = {};         // moduli per GEMM workgroup, per tile shape
+    CrtPersistLaunch CPL{};               // persistent CRT group (item lists in pbuf)
 };
 
 namespace {
@@ -374,6 +375,134 @@ std::vector<int4> xcd_deal(const std::vector<int4> &tiles, const std::vector<int
     return out;
 }
 
+// Per-workgroup (or per-wave) lists flattened into items + offsets, uploaded into one
+// buffer region; returns the device addresses through the out pointers.
+struct ListBlob {
+    std::vector<int4> items;
+    std::vector<int> off;
+};
+
+ListBlob flatten(const std::vector<std::vector<int>> &lists, const std::vector<int4> &it) {
+    ListBlob b;
+    b.off.resize(lists.size() + 1);
+    for (size_t w = 0; w < lists.size(); w++) {
+        b.off[w] = (int)b.items.size();
+        for (int i : lists[w]) b.items.push_back(it[i]);
+    }
+    b.off[lists.size()] = (int)b.items.size();
+    return b;
+}
+
+// Item lists of the persistent CRT group: phase A as for the split scheme (row blocks of
+// real rows, per wave, longest first), phase B the tiles of every problem (each with all
+// moduli) in band order, problems by decreasing contraction length, cut into 8 parts of
+// equal cost (one per XCD) and balanced over each XCD's workgroups, each workgroup's list
+// ordered by tile shape; phase C the rebuild items balanced over all workgroups.
+int build_persist_crt(ozaie_group g, const std::vector<int64_t> &Kp) {
+    const int n = g->count;
+    int dev = 0, ncu = 0;
+    GCHK(hipGetDevice(&dev));
+    GCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int G = std::max(1, ncu) * crt_persist_per_cu();
+    std::vector<int4> itA, itB, itC;
+    std::vector<int64_t> costA, costB, costC;
+    for (int p = 0; p < n; p++)
+        for (int op = 0; op < 2; op++) {
+            const GroupOperand &o = g->h_ops[2 * p + op];
+            for (int64_t rb = 0; rb < (o.R + kBlk - 1) / kBlk; rb++) {
+                itA.push_back(int4{2 * p + op, (int)rb, 0, 0});
+                costA.push_back(o.K + 256);
+            }
+        }
+    std::vector<int> porder(n);
+    for (int p = 0; p < n; p++) porder[p] = p;
+    std::stable_sort(porder.begin(), porder.end(), [&](int a, int b) { return Kp[a] > Kp[b]; });
+    for (int p : porder) {
+        // the 256 x 256 tile runs as 128 x 128 tiles in the persistent launch (the padding
+        // to 256 is a multiple of 128)
+        const int c = g->cfg[p] == CCFG_44 ? CCFG_22 : g->cfg[p];
+        const int64_t area = (int64_t)crt_group_cfg_wm(c) * crt_group_cfg_wn(c);
+        const CrtGroupProb &cp = g->h_cprobs[p];
+        const int ntm = (int)(cp.RbA * kBlk / (64 * crt_group_cfg_wm(c))), ntn = (int)(cp.Np / (64 * crt_group_cfg_wn(c)));
+        for (int t = 0; t < ntm * ntn; t++) {
+            int tm, tn;
+            band_map(t, ntm, ntn, tm, tn);
+            itB.push_back(int4{p, tm, tn, c});
+            costB.push_back((Kp[p] / kBlk + 6) * area * g->d);   // contraction blocks + epilogue, every modulus
+        }
+        const int64_t groups = (g->M[p] + 3) / 4 * ((g->N[p] + 3) / 4);
+        for (int64_t base = 0; base < groups; base += 256) {
+            itC.push_back(int4{p, (int)base, 0, 0});
+            costC.push_back(std::min<int64_t>(256, groups - base));
+        }
+    }
+    auto order_of = [](const std::vector<int64_t> &cost) {
+        std::vector<int> o(cost.size());
+        for (size_t i = 0; i < o.size(); i++) o[i] = (int)i;
+        std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+        return o;
+    };
+    std::vector<int> slots(4 * G), wgs(G);
+    for (int w = 0; w < 4 * G; w++) slots[w] = w;
+    for (int w = 0; w < G; w++) wgs[w] = w;
+    std::vector<std::vector<int>> listsA(4 * G), listsB(G), listsC(G);
+    lpt_assign(costA, order_of(costA), slots, listsA);
+    lpt_assign(costC, order_of(costC), wgs, listsC);
+    int64_t total = 0;
+    for (int64_t c : costB) total += c;
+    const int nx = (G % 8 == 0) ? 8 : 1;
+    size_t pos = 0;
+    int64_t acc = 0;
+    for (int x = 0; x < nx; x++) {
+        std::vector<int> part;
+        const int64_t target = (x == nx - 1) ? INT64_MAX : total * (x + 1) / nx;
+        while (pos < itB.size() && acc < target) {
+            part.push_back((int)pos);
+            acc += costB[pos];
+            pos++;
+        }
+        std::vector<int> xw;
+        for (int w = x; w < G; w += nx) xw.push_back(w);
+        lpt_assign(costB, part, xw, listsB);
+    }
+    for (auto &l : listsB)
+        std::stable_sort(l.begin(), l.end(), [&](int a, int b) { return itB[a].w < itB[b].w; });
+    g->tiles = (int64_t)itB.size();
+    const ListBlob A = flatten(listsA, itA), B = flatten(listsB, itB), C = flatten(listsC, itC);
+    auto r256 = [](size_t x) { return (size_t)round_up((int64_t)x, 256); };
+    const size_t sizes[7] = {r256(sizeof(int4) * A.items.size()), r256(sizeof(int) * A.off.size()),
+                             r256(sizeof(int4) * B.items.size()), r256(sizeof(int) * B.off.size()),
+                             r256(sizeof(int4) * C.items.size()), r256(sizeof(int) * C.off.size()),
+                             r256((size_t)persist_ctl_bytes())};
+    size_t tot = 0;
+    for (size_t z : sizes) tot += z;
+    GCHK(hipMalloc(&g->pbuf, tot));
+    char *q = g->pbuf;
+    const void *src[6] = {A.items.data(), A.off.data(), B.items.data(), B.off.data(), C.items.data(), C.off.data()};
+    const size_t len[6] = {sizeof(int4) * A.items.size(), sizeof(int) * A.off.size(), sizeof(int4) * B.items.size(),
+                           sizeof(int) * B.off.size(), sizeof(int4) * C.items.size(), sizeof(int) * C.off.size()};
+    char *at[7];
+    for (int i = 0; i < 7; i++) {
+        at[i] = q;
+        if (i < 6 && len[i]) GCHK(hipMemcpy(q, src[i], len[i], hipMemcpyHostToDevice));
+        q += sizes[i];
+    }
+    GCHK(hipMemset(at[6], 0, sizes[6]));   // once, outside any graph
+    GCHK(hipDeviceSynchronize());
+    CrtPersistLaunch &L = g->CPL;
+    L.itemsA = reinterpret_cast<const int4 *>(at[0]);
+    L.offA = reinterpret_cast<const int *>(at[1]);
+    L.itemsB = reinterpret_cast<const int4 *>(at[2]);
+    L.offB = reinterpret_cast<const int *>(at[3]);
+    L.itemsC = reinterpret_cast<const int4 *>(at[4]);
+    L.offC = reinterpret_cast<const int *>(at[5]);
+    L.ctl = at[6];
+    L.grid = G;
+    L.rounding = g->rounding;
+    g->PL.ctl = at[6];   // ozaie_group_status reads the control block here
+    return OZAIE_OK;
+}
+
 // CRT mode: per problem the largest exact bit count for its K (or the requested one), a
 // tile shape per size class, residue planes padded to the tile, and the item tables of
 // the four phases.
@@ -478,6 +607,13 @@ int group_build_crt(ozaie_group g, int force_cfg) {
     g->lastA.assign(n, nullptr);
     g->lastB.assign(n, nullptr);
     g->lastC.assign(n, nullptr);
+    if (g->persist) {
+        g->tiles = 0;
+        if ((rc = build_persist_crt(g, Kp))) return rc;
+        g->CPL.ops = g->d_ops;
+        g->CPL.probs = g->d_cprobs;
+        return OZAIE_OK;
+    }
     if ((rc = upload(&g->d_items_max, imax))) return rc;
     if ((rc = upload(&g->d_items_split, isplit))) return rc;
     if ((rc = upload(&g->d_items_rebuild, irebuild))) return rc;
@@ -593,6 +729,10 @@ int group_build(ozaie_group g, int force_cfg) {
 }
 
 int group_enqueue(ozaie_group g, hipStream_t st) {
+    if (g->crt && g->persist) {
+        GCHK(launch_group_crt_persistent(g->CPL, g->cc, st));
+        return OZAIE_OK;
+    }
     if (g->crt) {
         GCHK(launch_fill_i32(g->E_all, g->n_E, 0, st));
         GCHK(launch_group_maxexp(g->d_ops, g->d_items_max, g->n_max, st));
@@ -637,10 +777,6 @@ int group_new(ozaie_group *out, int count, const int64_t *M, const int64_t *N, c
             set_error_msg("CRT groups need 2 <= moduli <= 16 and 0 <= bits <= 62");
             return OZAIE_ERR_ARG;
         }
-        if (flags & OZAIE_GROUP_PERSISTENT) {
-            set_error_msg("the CRT mode has no persistent kernel");
-            return OZAIE_ERR_ARG;
-        }
         const int want = ((flags >> 4) & 15) - 1;
         if (want >= CCFG_COUNT) {
             set_error_msg("no such CRT tile shape");
@@ -677,8 +813,9 @@ int group_new(ozaie_group *out, int count, const int64_t *M, const int64_t *N, c
     g->fallback = false;
     if (!crt)
         for (int p = 0; p < count; p++) g->fallback |= !group_cfg_supported(g->dp[p], GCFG_D11);
-    // persistent single launch: every problem has a persistent tile body (any of 3, 7, 8, 9)
-    g->persist = !crt && !g->fallback && (flags & OZAIE_GROUP_PERSISTENT);
+    // persistent single launch: every problem has a persistent tile body (slice counts 3, 7,
+    // 8, 9; every CRT tile shape)
+    g->persist = !g->fallback && (flags & OZAIE_GROUP_PERSISTENT);
     if (!rc && g->fallback) {
         for (int p = 0; p < count && !rc; p++) {
             ozaie_plan pl = nullptr;

@@ -113,189 +113,6 @@ hipError_t launch_group_gemm(const GemmLaunch *probs, const int4 *items, int n, 
 // Items are assigned to workgroups on the host (balanced lists, tiles of one problem on
 // one XCD), so the only cross-workgroup synchronisation is the barrier.
 
-// Phase A item, run by ONE wave: the 32 rows r0 = rb*32 .. +32 of operand o.  First the
-// exponent maximum of each row over the whole contraction axis (into o.E), then the
-// split in tiles of 32 rows x 64 contraction indices (two slice blocks): the tile is read
-// coalesced along the contiguous axis, transposed through the wave's own LDS region
-// (32 x 65 doubles), and lane l emits the 16-byte chunk it owns in each block (row l % 32,
-// k 16 (l / 32) .. +16).  The reads of tile t+1 are in flight while tile t is emitted.
-// Rows past R read row R - 1 (always valid) and are masked, so loads carry no branches.
-constexpr int kWaveTileLd = 65;   // LDS row stride (doubles) of the per-wave tile
-
-template <bool KC>
-__device__ __forceinline__ void wave_gather(const GroupOperand &o, int64_t r0, int64_t k0, int lane, double (&v)[32]) {
-    if (KC) {
-        // load i: row r0 + i, contraction index k0 + lane
-        const int64_t k = k0 + lane;
-#pragma unroll
-        for (int i = 0; i < 32; i++) {
-            const int64_t r = r0 + i;
-            v[i] = (r < o.R && k < o.K) ? o.X[r * o.sr + k] : 0.0;
-        }
-    } else {
-        // load i: contraction index k0 + 2i + lane / 32, row r0 + lane % 32
-        const int64_t r = r0 + (lane & 31);
-#pragma unroll
-        for (int i = 0; i < 32; i++) {
-            const int64_t k = k0 + 2 * i + (lane >> 5);
-            v[i] = (r < o.R && k < o.K) ? o.X[k * o.sk + r] : 0.0;
-        }
-    }
-}
-
-template <bool KC>
-__device__ __forceinline__ void wave_stage(const double (&v)[32], double *wt, int lane) {
-#pragma unroll
-    for (int i = 0; i < 32; i++) {
-        if (KC) wt[i * kWaveTileLd + lane] = v[i];
-        else wt[(lane & 31) * kWaveTileLd + 2 * i + (lane >> 5)] = v[i];
-    }
-}
-
-// Digits of one 16-element chunk (row `row`, elements wt[row][c0 .. c0+16)) into the
-// lane's 16-byte chunk of d slice planes at dst.
-__device__ __forceinline__ void wave_emit_chunk(const double *wt, int row, int c0, int emax, int b, int d,
-                                                int8_t *__restrict__ dst, int64_t plane) {
-    uint64_t mx[16];
-    int bias[16];
-    uint32_t negm = 0;
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        const uint64_t u = (uint64_t)__double_as_longlong(wt[row * kWaveTileLd + c0 + i]);
-        const int e = (int)((u >> 52) & 0x7FF);
-        mx[i] = (u & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
-        bias[i] = emax - e;
-        negm |= (uint32_t)(u >> 63) << i;
-    }
-    emit_digits16(mx, bias, negm, b, d, dst, plane);
-}
-
-template <bool KC>
-__device__ __forceinline__ void wave_rowblock(const GroupOperand &o, int64_t rb, int b, int d, double *wt, int lane) {
-    const int64_t r0 = rb * 32;
-    // 1. exponent maxima (64 loads in flight per lane); lane l < 32 ends with row l's
-    int emax;
-    if (KC) {
-        const double *rowp[8];
-        int m[32];
-#pragma unroll
-        for (int i = 0; i < 32; i++) m[i] = 0;
-#pragma unroll
-        for (int gi = 0; gi < 4; gi++) {
-#pragma unroll
-            for (int i = 0; i < 8; i++) {
-                const int64_t r = r0 + gi * 8 + i;
-                rowp[i] = o.X + (r < o.R ? r : o.R - 1) * o.sr;
-            }
-            int64_t k = lane;
-            for (; k + 448 < o.K; k += 512) {
-                double a[8][8];
-#pragma unroll
-                for (int i = 0; i < 8; i++)
-#pragma unroll
-                    for (int u = 0; u < 8; u++) a[i][u] = rowp[i][k + 64 * u];
-#pragma unroll
-                for (int i = 0; i < 8; i++)
-#pragma unroll
-                    for (int u = 0; u < 8; u++) m[gi * 8 + i] = max(m[gi * 8 + i], field_of(a[i][u]));
-            }
-            for (; k < o.K; k += 64) {
-#pragma unroll
-                for (int i = 0; i < 8; i++) m[gi * 8 + i] = max(m[gi * 8 + i], field_of(rowp[i][k]));
-            }
-        }
-        // transpose-reduce through LDS: [32 rows][64 lanes]
-        int32_t *red = reinterpret_cast<int32_t *>(wt);
-#pragma unroll
-        for (int i = 0; i < 32; i++) red[i * 64 + lane] = m[i];
-        emax = 0;
-        if (lane < 32) {
-#pragma unroll
-            for (int q = 0; q < 64; q += 4) {
-                const int4 x = *reinterpret_cast<const int4 *>(red + lane * 64 + q);
-                emax = max(emax, max(max(x.x, x.y), max(x.z, x.w)));
-            }
-        }
-    } else {
-        // lane l: row r0 + l % 32, contraction indices of parity l / 32
-        const int64_t r = r0 + (lane & 31);
-        const double *col = o.X + (r < o.R ? r : o.R - 1);
-        int m = 0;
-        int64_t k = lane >> 5;
-        for (; k + 2 * 63 < o.K; k += 128) {
-            double a[64];
-#pragma unroll
-            for (int u = 0; u < 64; u++) a[u] = col[(k + 2 * u) * o.sk];
-#pragma unroll
-            for (int u = 0; u < 64; u++) m = max(m, field_of(a[u]));
-        }
-        for (; k < o.K; k += 2) m = max(m, field_of(col[k * o.sk]));
-        emax = max(m, __shfl_xor(m, 32, 64));
-    }
-    if (r0 + (lane & 31) >= o.R) emax = 0;
-    if (lane < 32) o.E[r0 + lane] = emax;
-    emax = __shfl(emax, lane & 31, 64);
-    // 2. split, tiles of 64 contraction indices (Kb is even), two tiles in flight
-    const int64_t nt = o.Kb / 2;
-    const int64_t plane = o.Rb * o.Kb * (int64_t)kBlkBytes;
-    const int row = lane & 31, half = lane >> 5;
-    auto emit = [&](int64_t t) {
-#pragma unroll
-        for (int blk = 0; blk < 2; blk++) {
-            int8_t *dst = o.S + (rb * o.Kb + 2 * t + blk) * (int64_t)kBlkBytes + lane * 16;
-            wave_emit_chunk(wt, row, 32 * blk + 16 * half, emax, b, d, dst, plane);
-        }
-    };
-    double v0[32], v1[32];
-    wave_gather<KC>(o, r0, 0, lane, v0);
-    if (nt > 1) wave_gather<KC>(o, r0, 64, lane, v1);
-    for (int64_t t = 0; t < nt; t += 2) {
-        wave_stage<KC>(v0, wt, lane);
-        if (t + 2 < nt) wave_gather<KC>(o, r0, (t + 2) * 64, lane, v0);
-        emit(t);
-        if (t + 1 < nt) {
-            wave_stage<KC>(v1, wt, lane);
-            if (t + 3 < nt) wave_gather<KC>(o, r0, (t + 3) * 64, lane, v1);
-            emit(t + 1);
-        }
-    }
-}
-
-struct PersistCtl {
-    int arrive;     // workgroups through the phase barrier
-    int done;       // workgroups finished (the last one resets the launch's words)
-    int error;      // this launch: a barrier wait exceeded its bound (every waiter leaves)
-    int reported;   // sticky across launches until ozaie_group_status reads and clears it
-    int pad[60];
-};
-
-constexpr int kSpinLimit = 1 << 20;   // barrier polls (each a device-coherent load + sleep)
-
-// Grid barrier: every wave makes its stores visible at agent scope, one thread per
-// workgroup arrives and polls; a wait longer than kSpinLimit polls (or an error raised by
-// another workgroup) sets the error flag and releases everyone.  Returns false on error.
-__device__ bool grid_barrier(PersistCtl *ctl, int G, int *flag_lds) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int ok = 1;
-        __hip_atomic_fetch_add(&ctl->arrive, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (int spin = 0;; spin++) {
-            if (__hip_atomic_load(&ctl->arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= G) break;
-            if (spin >= kSpinLimit || __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                __hip_atomic_store(&ctl->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ok = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-        }
-        *flag_lds = ok;
-    }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return *flag_lds != 0;
-}
-
 // Operands of one tile of phase B (all wave-uniform: the lane offset is added at the loads).
 struct PTile {
     const int8_t *pa, *pb;
@@ -467,19 +284,7 @@ void k_grp_persistent(const GroupOperand *__restrict__ ops, const int4 *__restri
         }
     }
 
-    // ---- exit: the last workgroup hands this launch's error to the sticky report word and
-    // resets the launch's words, so a failed launch does not fail the next one
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int old = __hip_atomic_fetch_add(&ctl->done, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == G - 1) {
-            if (__hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                __hip_atomic_store(&ctl->reported, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&ctl->error, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&ctl->arrive, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&ctl->done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
+    persist_exit(ctl, G);
 }
 
 bool persist_cfg_supported(int d, int cfg) {

@@ -221,6 +221,23 @@ hipError_t launch_group_crt_gemm(const CrtGroupProb *probs, const int4 *items, i
                                  const CrtConsts &c, hipStream_t st);
 hipError_t launch_group_crt_rebuild(const CrtGroupProb *probs, const int4 *items, int n, int rounding,
                                     const CrtConsts &c, hipStream_t st);
+// The whole CRT group as ONE persistent launch (one workgroup per CU): phase A exponent
+// maxima + residues per 32-row block of every operand (per-wave lists itemsA[offA[4w+v],
+// offA[4w+v+1]), items {operand, row block, 0, 0}), grid barrier, phase B modulus-GEMM
+// tiles with every modulus (per-workgroup lists, items {problem, tile row, tile col,
+// CrtCfg}, consecutive items of one shape), grid barrier, phase C rebuild (per-workgroup
+// lists, items {problem, first group, 0, 0} as for the rebuild launch).  ctl: as
+// PersistLaunch.ctl.
+struct CrtPersistLaunch {
+    const GroupOperand *ops;
+    const CrtGroupProb *probs;
+    const int4 *itemsA, *itemsB, *itemsC;
+    const int *offA, *offB, *offC;
+    void *ctl;
+    int grid, rounding;
+};
+hipError_t launch_group_crt_persistent(const CrtPersistLaunch &L, const CrtConsts &c, hipStream_t st);
+int crt_persist_per_cu();   // workgroups per CU of that launch (grid = this x CUs)
 
 // Synthetic (U - 0.5) 2^e data for timing runs.
 hipError_t launch_fill_random(double *p, int64_t n, uint32_t seed, hipStream_t st);

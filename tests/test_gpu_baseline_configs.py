@@ -209,14 +209,18 @@ def test_group512_modes_vs_single_plans_and_oracle(group512, d):
 def test_group512_crt16_vs_single_plans_and_oracle(group512):
     import torch
     shapes, As, Bs = group512
-    print("512 problems CRT 16 moduli", flush=True)
     ref = _single_plans(torch, shapes, As, Bs, 0, 16, True)
-    grp = oz.Group(shapes, 0, 16, rounding=oz.ROUND_NEAREST, crt=True)
-    Cs = [torch.full((m, n), float("nan"), dtype=torch.float64, device="cuda") for m, n, _ in shapes]
-    for rep in range(2):
-        grp(As, Bs, Cs)
-        torch.cuda.synchronize()
-        for i, (c, r) in enumerate(zip(Cs, ref)):
-            assert torch.equal(c.view(torch.int64), r.view(torch.int64)), (rep, i, shapes[i])
-    grp.close()
+    for persistent in (False, True):
+        print(f"512 problems CRT 16 moduli persistent={persistent}", flush=True)
+        grp = oz.Group(shapes, 0, 16, rounding=oz.ROUND_NEAREST, crt=True, persistent=persistent)
+        Cs = [torch.full((m, n), float("nan"), dtype=torch.float64, device="cuda") for m, n, _ in shapes]
+        for rep in range(2):
+            grp(As, Bs, Cs)
+            torch.cuda.synchronize()
+            assert grp.status() == 0
+            for i, (c, r) in enumerate(zip(Cs, ref)):
+                assert torch.equal(c.view(torch.int64), r.view(torch.int64)), (persistent, rep, i, shapes[i])
+        if persistent:
+            assert grp.query()["launches"] == 1
+        grp.close()
     _oracle_samples(shapes, As, Bs, Cs, 16, True, SEED + 16)

@@ -111,8 +111,6 @@ def test_crt_group_limits(gpu):
     with pytest.raises(oz.OzaieError):
         oz.Group([(64, 64, 64), (64, 64, 4096)], oz.crt_beta(16, 64), 16, crt=True)   # bits too many for K=4096
     with pytest.raises(oz.OzaieError):
-        oz.Group([(64, 64, 64)], 0, 16, crt=True, persistent=True)   # no persistent CRT kernel
-    with pytest.raises(oz.OzaieError):
         oz.Group([(64, 64, 64)], 0, 16, crt=True, crt_cfg=3)         # no such tile shape
 
 
@@ -143,3 +141,33 @@ def test_crt_group_moduli_per_workgroup(gpu, mpw):
             grp.close()
     finally:
         oz.set_option(oz.OPT_CRT_GROUP_MODULI, 0)
+
+
+@pytest.mark.parametrize("n,cfg,rounding", [(16, None, oz.ROUND_NEAREST), (16, 0, oz.ROUND_TRUNCATE),
+                                            (16, 2, oz.ROUND_NEAREST), (17, None, oz.ROUND_NEAREST),
+                                            (3, 1, oz.ROUND_TRUNCATE)])
+def test_crt_persistent_group_matches_single_plans(gpu, n, cfg, rounding):
+    """ONE persistent launch (residues, grid barrier, modulus GEMMs, grid barrier, rebuild):
+    every replay equal to the single CRT plans, ragged shapes included, per-class and forced
+    tile shapes, 17 moduli through the 192-bit rebuild."""
+    import torch
+    rng = np.random.default_rng(60 + n + (cfg or 0))
+    shp = EDGE + shapes(rng, 30)
+    As, Bs = inputs(torch, shp, 300 + n)
+    ref = single(torch, shp, As, Bs, 0, n, rounding)
+    print(f"persistent CRT group n={n} cfg={cfg}", flush=True)
+    grp = oz.Group(shp, 0, n, rounding=rounding, crt=True, crt_cfg=cfg, persistent=True)
+    q = grp.query()
+    assert q["launches"] == 1 and not q["fallback"]
+    if cfg is not None:
+        assert all(int(c) == cfg for c in grp.configs())
+    Cs = [torch.empty((m, nn), dtype=torch.float64, device="cuda") for m, nn, _ in shp]
+    for rep in range(3):
+        for c in Cs:
+            c.fill_(float("nan"))                      # eager work between replays
+        grp(As, Bs, Cs)
+        torch.cuda.synchronize()
+        assert grp.status() == 0
+        for i, (c, r) in enumerate(zip(Cs, ref)):
+            assert torch.equal(c.view(torch.int64), r.view(torch.int64)), (rep, i, shp[i])
+    grp.close()
