@@ -16,9 +16,11 @@
  *   bits        beta = floor((L - 2 - ceil(log2 K)) / 2), L = floor(log2 M), at most
  *               62: then K * 2^(2 beta) <= M / 4, so |C'| <= M / 4.
  *   scaling     the row (A) / column (B) maximum exponent field E as in the split
- *               scheme; q(x) = sign(x) * floor(M_x * 2^(beta - 53 - (E - e'))), |q| < 2^beta,
- *               e' = max(e, 1), i.e. q = sign(x) floor(|x| 2^(1022 + beta - E)) for every
- *               finite x (unlike the split scheme, subnormals are not off by a factor 2).
+ *               scheme; q(x) = sign(x) * round(M_x * 2^(beta - 53 - (E - e'))), |q| <= 2^beta,
+ *               e' = max(e, 1), i.e. q = sign(x) round(|x| 2^(1022 + beta - E)) for every
+ *               finite x (unlike the split scheme, subnormals are not off by a factor 2);
+ *               round = to nearest, ties away from zero (half the error of truncation and
+ *               no bias; |q| <= 2^beta keeps K * 2^(2 beta) as the bound of |C'|).
  *   residues    r_p(x) = q(x) mod p in [-(p-1)/2, (p-1)/2] for odd p, [-128, 127] for
  *               p = 256: int8.
  *   products    g_p = (sum_k r_p(a_ik) r_p(b_kj)) mod p in [0, p) (int32 sums are exact).
@@ -137,7 +139,9 @@ int64_t oro_crt_q(double x, int emax, int beta) {
     int e = (int)((u >> 52) & 0x7FF);
     uint64_t mx = (u & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
     int sh = 53 + (emax - (e ? e : 1)) - beta;   /* subnormals have exponent 1 - 1023 */
-    uint64_t Q = sh >= 64 ? 0 : (sh >= 0 ? mx >> sh : mx << (-sh));
+    /* nearest, ties away from zero; for sh >= 54, mx / 2^sh < 2^(53 - sh) <= 1/2 strictly
+     * (mx < 2^53), so those round to 0 */
+    uint64_t Q = sh >= 54 ? 0 : (sh >= 1 ? (mx + (1ull << (sh - 1))) >> sh : mx << (-sh));
     return (u >> 63) ? -(int64_t)Q : (int64_t)Q;
 }
 

@@ -6,7 +6,8 @@ namespace oz {
 
 // ------------------------------------------------------------------ CRT mode
 // An exact integer product instead of the triangle of slice products: each element
-// becomes q = sign(x) floor(|x| 2^(1022 + beta - E)) (|q| < 2^beta), the operands are
+// becomes q = sign(x) round(|x| 2^(1022 + beta - E)) (to nearest, ties away from zero;
+// |q| <= 2^beta), the operands are
 // multiplied modulo n pairwise-coprime moduli p <= 256 with one exact int8 GEMM each,
 // and the Chinese remainder theorem rebuilds sum_k q_a q_b exactly (|C'| <= M / 4).
 
@@ -28,7 +29,8 @@ __device__ __forceinline__ void crt_emit16(const double (&v)[16], int emax, int 
         const int e = (int)((u >> 52) & 0x7FF);
         const uint64_t mx = (u & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
         const int sh = 53 + emax - (e ? e : 1) - beta;   // subnormals: exponent 1 - 1023
-        const uint64_t Q = sh >= 64 ? 0ull : (sh >= 0 ? mx >> sh : mx << (-sh));
+        // nearest, ties away from zero (mx < 2^53: shifts of 54 and more round to 0)
+        const uint64_t Q = sh >= 54 ? 0ull : (sh >= 1 ? (mx + (1ull << (sh - 1))) >> sh : mx << (-sh));
         const bool neg = (u >> 63) != 0;
         const double h = (double)(uint32_t)(Q >> 31), l = (double)(uint32_t)(Q & 0x7FFFFFFFu);
         hi[i] = neg ? -h : h;
