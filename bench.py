@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--grouped-only", action="store_true",
                     help="only the 512-problem grouped section (prints its record as JSON)")
+    ap.add_argument("--accurate-only", action="store_true",
+                    help="only the reference's Mode.accurate section (prints its record as JSON)")
     ap.add_argument("--probe-only", action="store_true",
                     help="only the measured int8 MFMA ceiling (ozaie_probe_mfma), as one JSON line")
     ap.add_argument("--profile-run", action="store_true",
@@ -681,6 +683,9 @@ def main():
     n = args.size
     if args.profile_run:
         profile_run(args, torch, oz, n, slices)
+        return
+    if args.accurate_only:
+        print(json.dumps(accurate_section(args, torch, np, oz)), flush=True)
         return
     if args.probe_only:
         rec = {}

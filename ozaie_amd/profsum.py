@@ -3,6 +3,12 @@
     python -m ozaie_amd.profsum kernels DIR [--top N]
         per-kernel calls / average / min / max / share from the kernel trace
         (rocprofv3 --kernel-trace --output-format csv -d DIR)
+    python -m ozaie_amd.profsum counters DIR [--simds 1024 --xcds 8]
+        per-kernel averages of one --pmc run with SQ_VALU_MFMA_BUSY_CYCLES, SQ_WAVE_CYCLES,
+        SQ_WAIT_ANY, SQ_INSTS_LDS, SQ_LDS_BANK_CONFLICT, GRBM_GUI_ACTIVE (with --kernel-trace
+        in the same run, also the duration and the clock): cycles per XCD = GRBM_GUI_ACTIVE /
+        xcds, MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES / simds / cycles per XCD, wait =
+        SQ_WAIT_ANY / SQ_WAVE_CYCLES
     python -m ozaie_amd.profsum traffic FETCH_DIR WRITE_DIR [--json OUT]
         HBM bytes per dispatch of every oz:: kernel from two --pmc runs (FETCH_SIZE, then
         WRITE_SIZE, in KiB), scaled by what the same counters read for the 1 GiB device copy
@@ -50,6 +56,44 @@ def _pmc(d, counter):
     return [(did, names[did], per[did]) for did in sorted(per)]
 
 
+def counters(d, simds, xcds):
+    rows = _csv(d, "counter_collection.csv")
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    names = {}
+    for r in rows:
+        did = int(r["Dispatch_Id"])
+        per[did][r["Counter_Name"]] += float(r["Counter_Value"])
+        names[did] = r["Kernel_Name"]
+    dur = {}
+    try:
+        for r in _csv(d, "kernel_trace.csv"):
+            dur[int(r["Dispatch_Id"])] = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+    except SystemExit:
+        pass
+    agg = collections.OrderedDict()
+    for did in sorted(per):
+        if "oz::" not in names[did]:
+            continue
+        name = names[did].split("(")[0].replace("void ", "")
+        agg.setdefault(name, []).append((per[did], dur.get(did)))
+    out = [f"{'kernel':40s} {'calls':>5s} {'us':>9s} {'cycles/XCD':>11s} {'GHz':>5s} {'MFMA busy':>9s} "
+           f"{'wait':>6s} {'LDS insts':>10s} {'LDS confl':>9s}"]
+    for name, v in agg.items():
+        n = len(v)
+        mean = lambda key: sum(c.get(key, 0.0) for c, _ in v) / n
+        cyc = mean("GRBM_GUI_ACTIVE") / xcds
+        busy = mean("SQ_VALU_MFMA_BUSY_CYCLES") / simds / cyc if cyc else 0.0
+        wc = mean("SQ_WAVE_CYCLES")
+        wait = mean("SQ_WAIT_ANY") / wc if wc else 0.0
+        us = [t for _, t in v if t]
+        t = sum(us) / len(us) if us else None
+        ghz = cyc / (t * 1e3) if t else None
+        out.append(f"{name[:40]:40s} {n:5d} {t if t else 0:9.1f} {cyc:11.0f} {ghz if ghz else 0:5.2f} "
+                   f"{100 * busy:8.1f}% {100 * wait:5.1f}% {mean('SQ_INSTS_LDS'):10.0f} "
+                   f"{mean('SQ_LDS_BANK_CONFLICT'):9.0f}")
+    return "\n".join(out)
+
+
 def traffic(fetch_dir, write_dir):
     fetch = _pmc(fetch_dir, "FETCH_SIZE")
     write = _pmc(write_dir, "WRITE_SIZE")
@@ -79,6 +123,10 @@ def main(argv=None):
     k = sub.add_parser("kernels")
     k.add_argument("dir")
     k.add_argument("--top", type=int, default=25)
+    c = sub.add_parser("counters")
+    c.add_argument("dir")
+    c.add_argument("--simds", type=int, default=1024)
+    c.add_argument("--xcds", type=int, default=8)
     t = sub.add_parser("traffic")
     t.add_argument("fetch_dir")
     t.add_argument("write_dir")
@@ -87,6 +135,9 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "kernels":
         print(kernels(a.dir, a.top))
+        return 0
+    if a.cmd == "counters":
+        print(counters(a.dir, a.simds, a.xcds))
         return 0
     rec = traffic(a.fetch_dir, a.write_dir)
     rec["workload_n"] = a.n
