@@ -7,6 +7,7 @@ performance with device iterations/warmups 10/10 and medians (Program.cs:658-664
 module runs the same sweeps on the local GPU and appends JSON records to a file.
 
     python -m ozaie_amd.sweep accuracy --out acc.json
+    python -m ozaie_amd.sweep summary acc.json
     python -m ozaie_amd.sweep perf --sizes 1024,2048,4096 --out perf.json
 
 Errors are max / median |C - C*| / |C*| over sampled entries against the correctly
@@ -102,6 +103,61 @@ def accuracy(args):
                                       "samples": int(ns)})
 
 
+def int8_products(mode, K):
+    """int8 slice or residue products per output element of a preset (its cost measure, as the
+    reference ranks configurations by n_muls, Experiments/Program.cs:879-882)."""
+    b, d = mode_bd(mode, K)
+    if mode.startswith("crt"):
+        return d
+    if b <= 3 and d >= 8:                 # the paired-digit path the library runs
+        H = (d - 1) // 2 if d % 2 else (d - 2) // 2
+        P = H if d % 2 else H + 1
+        return P * (P + 1) // 2 + H + 1
+    return d * (d + 1) // 2
+
+
+def summary(args):
+    """Per (phi, K): native's max / median error, each preset's max and median over seeds, and
+    the matched preset: the fewest int8 products whose max error over the seeds is no worse
+    than native's max over the same seeds."""
+    recs = [r for r in load_records(args.inp) if r.get("kind") == "accuracy"]
+    modes = []
+    for r in recs:
+        if r["mode"] not in modes:
+            modes.append(r["mode"])
+    keys = sorted({(r["phi"], r["K"]) for r in recs})
+    lines = []
+    head = f"{'phi':>4s} {'K':>5s} {'seeds':>5s} {'native max':>10s} " + " ".join(f"{m:>11s}" for m in modes) + \
+           "   matched (int8 products)"
+    lines.append("max relative error over the seeds (median of the medians in the second table)")
+    lines.append(head)
+    med_lines = [f"{'phi':>4s} {'K':>5s} {'native med':>10s} " + " ".join(f"{m:>11s}" for m in modes)]
+    for phi, K in keys:
+        rs = [r for r in recs if r["phi"] == phi and r["K"] == K]
+        seeds = sorted({r["seed"] for r in rs})
+        nat = max(r["native_max_rel_err"] for r in rs)
+        natmed = sorted(r["native_median_rel_err"] for r in rs)[len(rs) // 2]
+        row, mrow, ok = [], [], []
+        for m in modes:
+            mr = [r for r in rs if r["mode"] == m]
+            if not mr:
+                row.append(f"{'-':>11s}")
+                mrow.append(f"{'-':>11s}")
+                continue
+            mx = max(r["max_rel_err"] for r in mr)
+            md = sorted(r["median_rel_err"] for r in mr)[len(mr) // 2]
+            row.append(f"{mx:11.2e}")
+            mrow.append(f"{md:11.2e}")
+            if mx <= nat:
+                ok.append((int8_products(m, K), m))
+        match = f"{min(ok)[1]} ({min(ok)[0]})" if ok else "none"
+        lines.append(f"{phi:4g} {K:5d} {len(seeds):5d} {nat:10.2e} " + " ".join(row) + f"   {match}")
+        med_lines.append(f"{phi:4g} {K:5d} {natmed:10.2e} " + " ".join(mrow))
+    text = "\n".join(lines + ["", "median relative error (median over the seeds)"] + med_lines)
+    print(text)
+    return text
+
+
 def perf(args):
     import numpy as np
     import torch
@@ -159,9 +215,13 @@ def main(argv=None):
     for s in (a, p):
         s.add_argument("--out", default=None, help="append JSON records to this file")
         s.add_argument("--resume", action="store_true", help="skip configurations already in --out")
+    m = sub.add_parser("summary", help="table of an accuracy sweep's records")
+    m.add_argument("inp")
     args = ap.parse_args(argv)
     if args.cmd == "accuracy":
         accuracy(args)
+    elif args.cmd == "summary":
+        summary(args)
     else:
         perf(args)
     return 0
