@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--grouped-only", action="store_true",
                     help="only the 512-problem grouped section (prints its record as JSON)")
+    ap.add_argument("--long-k-only", action="store_true",
+                    help="only the 8192 x 8192 x 32768 section (prints its record as JSON)")
     ap.add_argument("--accurate-only", action="store_true",
                     help="only the reference's Mode.accurate section (prints its record as JSON)")
     ap.add_argument("--probe-only", action="store_true",
@@ -386,7 +388,7 @@ def long_k(args, torch, np, oz, slices=(7, 8, 9)):
         2.0 * torch.randn(K, N, dtype=torch.float64, device="cuda:0", generator=g))
     C = torch.empty(M, N, dtype=torch.float64, device="cuda:0")
     flop = 2.0 * M * N * K
-    t_nat = time_steps(torch, lambda: torch.matmul(A, B, out=C), 3, 1)
+    t_nat = time_steps(torch, lambda: torch.matmul(A, B, out=C), 5, 2)
     rng = np.random.default_rng(args.seed)
     rows = np.repeat(rng.choice(M, 4, replace=False), 64)
     cols = np.tile(rng.choice(N, 64, replace=False), 4)
@@ -399,25 +401,44 @@ def long_k(args, torch, np, oz, slices=(7, 8, 9)):
     nat_err = rel_errors(np, C[ri, ci].cpu().numpy(), exact)
     rec = {"workload": "DGEMM M=N=8192 K=32768, phi=2", "native": {"ms": t_nat, "tflops": flop / t_nat / 1e9,
            "max_rel_err": nat_err[0], "median_rel_err": nat_err[1]}, "presets": []}
+    def phases(p):
+        """The library's own HIP-event phase times of 3 more calls (split, GEMM passes, whole
+        call; CRT: the whole call includes the rebuild): a slow call then shows whether its
+        time went into the kernels or into gaps between them."""
+        sp, gm, tot = [], [], []
+        for _ in range(3):
+            p(A, B, C)
+            a_, b_, c_ = p.phase_times()
+            sp.append(a_)
+            gm.append(b_)
+            tot.append(c_)
+        return {"split_ms": [round(x, 3) for x in sp], "gemm_ms": [round(x, 3) for x in gm],
+                "call_ms": [round(x, 3) for x in tot]}
+
+    oz.set_option(oz.OPT_PHASE_TIMING, 1)
     for d in slices:
         p = oz.Plan(M, N, K, 7, d)
-        t = time_steps(torch, lambda: p(A, B, C), 3, 1)
+        t = time_steps(torch, lambda: p(A, B, C), 5, 2)
         err = rel_errors(np, C[ri, ci].cpu().numpy(), exact)
         rec["presets"].append({"scheme": "split", "bits": 7, "slices": d, "ms": t, "tflops": flop / t / 1e9,
                                "speedup": t_nat / t,
                                "max_rel_err": err[0], "median_rel_err": err[1], "passes": p.info()["passes"],
-                               "matched": err[0] <= nat_err[0]})
-        log(args, f"  longK d={d}: {t:.1f} ms x{t_nat / t:.2f} err {err} passes {p.info()['passes']}")
+                               "matched": err[0] <= nat_err[0], **phases(p)})
+        log(args, f"  longK d={d}: {t:.1f} ms x{t_nat / t:.2f} err {err} passes {p.info()['passes']} "
+                  f"phases {rec['presets'][-1]['call_ms']}")
         p.close()
     for mods in [int(x) for x in getattr(args, "crt_long", "16,17").split(",") if x.strip()]:
         p = oz.Plan(M, N, K, 0, mods, rounding=oz.ROUND_NEAREST, crt=True)
-        t = time_steps(torch, lambda: p(A, B, C), 3, 1)
+        t = time_steps(torch, lambda: p(A, B, C), 5, 2)
         err = rel_errors(np, C[ri, ci].cpu().numpy(), exact)
         rec["presets"].append({"scheme": "crt", "moduli": mods, "bits": p.info()["bits"], "ms": t,
                                "tflops": flop / t / 1e9, "speedup": t_nat / t, "max_rel_err": err[0],
-                               "median_rel_err": err[1], "passes": 1, "matched": err[0] <= nat_err[0]})
-        log(args, f"  longK crt {mods} moduli ({p.info()['bits']} bits): {t:.1f} ms x{t_nat / t:.2f} err {err}")
+                               "median_rel_err": err[1], "passes": 1, "matched": err[0] <= nat_err[0],
+                               **phases(p)})
+        log(args, f"  longK crt {mods} moduli ({p.info()['bits']} bits): {t:.1f} ms x{t_nat / t:.2f} err {err} "
+                  f"phases {rec['presets'][-1]['call_ms']}")
         p.close()
+    oz.set_option(oz.OPT_PHASE_TIMING, 0)
     log(args, f"  longK native {t_nat:.1f} ms err {nat_err}")
     del A, B, C
     torch.cuda.empty_cache()
@@ -683,6 +704,9 @@ def main():
     n = args.size
     if args.profile_run:
         profile_run(args, torch, oz, n, slices)
+        return
+    if args.long_k_only:
+        print(json.dumps(long_k(args, torch, np, oz)), flush=True)
         return
     if args.accurate_only:
         print(json.dumps(accurate_section(args, torch, np, oz)), flush=True)
