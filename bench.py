@@ -206,8 +206,10 @@ def measure(args, torch, np, oz, n, steps, warmup, block, slices):
     return out, (Ah, Bh)
 
 
-def load_traffic(d, n):
-    """HBM bytes per launch of the fused GEMM from profiles/traffic.json, if measured."""
+def load_traffic(d, n, crt_mods=None, profiled_crt=(16, 17)):
+    """HBM bytes per launch of the preset's GEMM kernel from profiles/traffic.json, if measured
+    (bench.py --profile-run: two calls per preset, split presets first, then the CRT modulus
+    counts `profiled_crt` in order; the second call of the preset is read)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
@@ -217,9 +219,10 @@ def load_traffic(d, n):
             return None
         if d == "crt":
             hits = [x for x in t["dispatches"] if "k_crt_gemm" in x["kernel"]]
-        else:
-            hits = [x for x in t["dispatches"] if "k_gemm_" in x["kernel"] and f"<{d}," in x["kernel"]]
-        return hits[-1]["hbm_bytes"] if hits else None   # the second call of the preset
+            i = list(profiled_crt).index(crt_mods)
+            return hits[2 * i + 1]["hbm_bytes"] if len(hits) > 2 * i + 1 else None
+        hits = [x for x in t["dispatches"] if "k_gemm_" in x["kernel"] and f"<{d}," in x["kernel"]]
+        return hits[-1]["hbm_bytes"] if hits else None
     except Exception:
         return None
 
@@ -726,7 +729,7 @@ def main():
     crt = m.get("scheme") == "crt"
     d = m["moduli"] if crt else m["slices"]
     achieved = preset_ops(m, n) / m["gemm_ms"] / 1e9
-    traffic = load_traffic("crt" if crt else d, n)
+    traffic = load_traffic("crt" if crt else d, n, crt_mods=d if crt else None)
     # measured int8 MFMA ceiling of this card (register-only operands), beside the datasheet
     ceiling = {}
     for pat in ("zeros", "constant", "random"):
