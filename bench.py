@@ -55,10 +55,10 @@ def parse():
     ap.add_argument("--phi", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=12445)
     ap.add_argument("--slices", default="3,7,8,9", help="b=7 slice counts to measure")
-    ap.add_argument("--crt", default="16,17",
-                    help="CRT-mode modulus counts to measure ('' for none; 17: 192-bit rebuild, 3 more bits)")
-    ap.add_argument("--crt-long", default="16,17",
-                    help="CRT-mode modulus counts for the long-K section (17: 192-bit rebuild, more bits)")
+    ap.add_argument("--crt", default="14,15,16",
+                    help="CRT-mode modulus counts to measure ('' for none; 17-20: 192-bit rebuild)")
+    ap.add_argument("--crt-long", default="15,16",
+                    help="CRT-mode modulus counts for the long-K section (17-20: 192-bit rebuild)")
     ap.add_argument("--block", type=int, default=128, help="side of the sampled error block")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-target", action="store_true", help="skip the 4096^3 section")
@@ -105,6 +105,12 @@ def time_steps(torch, fn, steps, warmup):
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / steps
+
+
+def scaling_name(bits):
+    """CRT-mode operand scaling: bits 0 is the norm rule (power-of-two shift per row / column
+    from an exact bound on its Euclidean norm), else the max rule with that many bits."""
+    return "norm (Cauchy-Schwarz bound per row / column)" if bits == 0 else f"max rule, {bits} bits"
 
 
 def crt_list(args):
@@ -181,7 +187,8 @@ def measure(args, torch, np, oz, n, steps, warmup, block, slices):
         err = rel_errors(np, C[blk].cpu().numpy().ravel(), exact)
         info = plan.info()
         g_ms = float(np.mean(gemm))
-        rec = {"scheme": "crt", "moduli": mods, "bits": info["bits"], "ms": t, "tflops": flop / t / 1e9,
+        rec = {"scheme": "crt", "moduli": mods, "bits": info["bits"], "scaling": scaling_name(info["bits"]),
+               "ms": t, "tflops": flop / t / 1e9,
                "speedup": t_nat / t, "max_rel_err": err[0], "median_rel_err": err[1],
                "split_ms": float(np.mean(split)), "gemm_ms": g_ms,
                "rebuild_ms": float(np.mean(tot)) - float(np.mean(split)) - g_ms,
@@ -206,7 +213,7 @@ def measure(args, torch, np, oz, n, steps, warmup, block, slices):
     return out, (Ah, Bh)
 
 
-def load_traffic(d, n, crt_mods=None, profiled_crt=(16, 17)):
+def load_traffic(d, n, crt_mods=None, profiled_crt=(14, 15, 16)):
     """HBM bytes per launch of the preset's GEMM kernel from profiles/traffic.json, if measured
     (bench.py --profile-run: two calls per preset, split presets first, then the CRT modulus
     counts `profiled_crt` in order; the second call of the preset is read)."""
@@ -219,6 +226,8 @@ def load_traffic(d, n, crt_mods=None, profiled_crt=(16, 17)):
             return None
         if d == "crt":
             hits = [x for x in t["dispatches"] if "k_crt_gemm" in x["kernel"]]
+            if crt_mods not in profiled_crt:
+                return None
             i = list(profiled_crt).index(crt_mods)
             return hits[2 * i + 1]["hbm_bytes"] if len(hits) > 2 * i + 1 else None
         hits = [x for x in t["dispatches"] if "k_gemm_" in x["kernel"] and f"<{d}," in x["kernel"]]
@@ -430,11 +439,12 @@ def long_k(args, torch, np, oz, slices=(7, 8, 9)):
         log(args, f"  longK d={d}: {t:.1f} ms x{t_nat / t:.2f} err {err} passes {p.info()['passes']} "
                   f"phases {rec['presets'][-1]['call_ms']}")
         p.close()
-    for mods in [int(x) for x in getattr(args, "crt_long", "16,17").split(",") if x.strip()]:
+    for mods in [int(x) for x in getattr(args, "crt_long", "15,16").split(",") if x.strip()]:
         p = oz.Plan(M, N, K, 0, mods, rounding=oz.ROUND_NEAREST, crt=True)
         t = time_steps(torch, lambda: p(A, B, C), 5, 2)
         err = rel_errors(np, C[ri, ci].cpu().numpy(), exact)
-        rec["presets"].append({"scheme": "crt", "moduli": mods, "bits": p.info()["bits"], "ms": t,
+        rec["presets"].append({"scheme": "crt", "moduli": mods, "bits": p.info()["bits"],
+                               "scaling": scaling_name(p.info()["bits"]), "ms": t,
                                "tflops": flop / t / 1e9, "speedup": t_nat / t, "max_rel_err": err[0],
                                "median_rel_err": err[1], "passes": 1, "matched": err[0] <= nat_err[0],
                                **phases(p)})
@@ -599,7 +609,7 @@ def grouped(args, torch, np, oz, slices=(3, 7, 8, 9), count=512):
         table = [int(x) for x in oz.autotune(0, mods, reps=2, crt=True)]
         t_tune = time.time() - t0
         entry = {"label": f"crt{mods}", "scheme": "crt", "moduli": mods,
-                 "bits": "per problem: crt_beta(moduli, K)", "autotune_table": table, "autotune_s": t_tune}
+                 "bits": 0, "scaling": scaling_name(0), "autotune_table": table, "autotune_s": t_tune}
         run_modes(entry, lambda pers: oz.Group(shapes, 0, mods, rounding=oz.ROUND_NEAREST, crt=True,
                                                persistent=pers))
         rec["presets"].append(entry)
@@ -739,7 +749,7 @@ def main():
             log(args, f"  mfma probe {pat}: {e!r}")
     log(args, f"  int8 MFMA ceiling (TOP/s): {ceiling}")
     kname = (f"CRT modulus GEMM (k_crt_gemm_lds<4, 4>), {d} moduli" if crt else f"fused slice GEMM, d={d}")
-    cfg = ({"scheme": "crt", "moduli": d, "bits": m["bits"], "rounding": "nearest"} if crt
+    cfg = ({"scheme": "crt", "moduli": d, "bits": m["bits"], "scaling": m["scaling"], "rounding": "nearest"} if crt
            else {"scheme": "split", "bits": 7, "slices": d})
     # the reference rule (b = 7 split scheme): its fastest preset at matched error, and the
     # reference's own fast preset (d = 8), flat so that every record keeps them

@@ -45,9 +45,10 @@ extern "C" {
 #define OZAIE_MODE_FAST3 2
 #define OZAIE_MODE_ACCURATE7 3
 #define OZAIE_MODE_CRT 4        /* no reference counterpart: exact integer product of the
-                                   operands scaled to the largest bit count that 16 int8
-                                   residue GEMMs rebuild exactly (ozaie_plan_create, flag
-                                   OZAIE_PLAN_CRT); K <= 65536                           */
+                                   operands, each row of A and column of B scaled by a
+                                   power of two from an exact bound on its Euclidean norm
+                                   so that 16 int8 residue GEMMs rebuild it exactly
+                                   (ozaie_plan_create, flag OZAIE_PLAN_CRT); K <= 65536   */
 
 /* ---- options (ozaie_set_option) */
 #define OZAIE_OPT_ROUNDING 1    /* OZAIE_ROUND_TRUNCATE (default) or OZAIE_ROUND_NEAREST   */
@@ -84,7 +85,8 @@ int32_t ozaie_get_option(int option);
 
 /* Number of (b, d) and passes of the current global plan:
  * out[0]=b out[1]=d out[2]=k-chunk out[3]=passes out[4]=kernel (0 fused, 1 group,
- * 2 CRT mode: then b = bits per operand and d = number of moduli, 3 paired digits: b <= 3
+ * 2 CRT mode: then b = the max rule's bits per operand (0: the norm rule) and d = number
+ * of moduli, 3 paired digits: b <= 3
  * with d >= 8, adjacent digits paired into one int8 digit, the same C)                  */
 int32_t ozaie_plan_info(int32_t *out5);
 
@@ -97,8 +99,13 @@ typedef struct ozaie_plan_s *ozaie_plan;
 /* Create a plan for C (M x N) = op(A) (M x K) * op(B) (K x N) with b bits and d slices.
  * flags: bit 0 -> nearest-even rounding (default truncation).
  * flags bit 2 (OZAIE_PLAN_CRT) -> CRT mode: `slices` = number of moduli (2..20), `bits`
- * = bits per operand (0 = the largest the moduli rebuild exactly for this K).  No
- * reference counterpart; K <= 65536.                                                   */
+ * = 0 for the norm rule: per row / column q = round(x 2^s) with the largest s that the
+ * Cauchy-Schwarz bound ||q_a|| ||q_b|| <= M / 4 allows, from an exact integer bound on the
+ * norm (about log2(max|x| sqrt(K) / ||x||) bits more than the max rule; for a few moduli,
+ * M 2^20 <= K (2^20 + 2^40), the max rule with the largest bit count instead); or `bits`
+ * = beta > 0 for the max rule: |q| <= 2^beta from the row / column maximum exponent, beta
+ * no more than the moduli rebuild exactly for this K.  No reference counterpart;
+ * K <= 65536.                                                                           */
 #define OZAIE_PLAN_NEAREST 1
 #define OZAIE_PLAN_CRT 4
 int32_t ozaie_plan_create(ozaie_plan *plan, int64_t M, int64_t N, int64_t K, int bits,
@@ -155,12 +162,12 @@ int32_t ozaie_dgemm_noalloc(int64_t M, int64_t N, int64_t K, int bits, int slice
  * forced persistent tile configuration + 1 (0: per size class; ids as for
  * ozaie_autotune), applied to the problems whose slice count has it.
  * flags bit 3 (OZAIE_GROUP_CRT): the CRT mode for every problem (no reference
- * counterpart): `slices` = moduli (2..20), `bits` = 0 for the largest exact bit count of
- * each problem's K (crt_beta(moduli, K_i)) or a fixed count no larger than any problem's;
+ * counterpart): `slices` = moduli (2..20), `bits` = 0 for the norm rule per problem (as
+ * ozaie_plan_create) or a fixed max-rule bit count no larger than any problem's allows;
  * K_i <= 65536.  Four phase launches plus one modulus-GEMM launch per tile shape (ids as
  * for ozaie_autotune_crt; flags bits 4..7 = forced shape + 1), same graph capture; each
  * C_i is bit-identical to a CRT plan of that problem.  With bit 2 as well, ONE persistent
- * launch (two workgroups per CU): exponent maxima + residues of every operand, a grid
+ * launch (two workgroups per CU): shifts + residues of every operand, a grid
  * barrier, every modulus-GEMM tile (each problem in its size class's tile shape, 256 x 256
  * run as 128 x 128 tiles, all moduli per tile), a second grid barrier, the rebuild of
  * every C_i.
@@ -214,7 +221,8 @@ int32_t ozaie_tune_set_crt(int moduli, const int32_t *table9);
 
 /* ---- inspection (reference full_dump equivalent, whole_array_iron_ozaki.py:485-558)  */
 
-/* CRT-mode intermediates of C = A * B (row-major device A, B), as computed on the GPU:
+/* CRT-mode intermediates of C = A * B (row-major device A, B; `bits` as for
+ * ozaie_plan_create), as computed on the GPU:
  * SA_out int8[moduli][M][K] and SB_out int8[moduli][N][K] residues (host), G_out
  * uint8[moduli][M][N] residue products mod p (host).  No reference counterpart.        */
 int32_t ozaie_debug_crt(const double *A, const double *B, int64_t M, int64_t N, int64_t K, int moduli,

@@ -18,6 +18,8 @@ Function map (reference file:line each restates):
   ozaki_dgemm               the whole pipeline, include/oz_aie.h:311-349
   mpfr_dot / mpfr_sample    common.h:164-232 (MPFR, 106 bits)
   naive_dgemm               common.h:234-254
+  crt_*                     the library's CRT mode (no reference counterpart; the
+                            arithmetic is stated in crt_oracle.c)
 
 Parity status: every function above is pinned by at least one independent check in
 tests/test_oracle.py (word-level vs closed-form split, numpy int64 GEMMs for bins,
@@ -78,8 +80,17 @@ def _setup(L):
     L.oro_crt_rebuild.argtypes = [P, i64, i32, P]
     L.oro_crt_to_double.argtypes = [P, i32, i32]
     L.oro_crt_to_double.restype = ctypes.c_double
-    L.oro_crt_dgemm.argtypes = [i64, i64, i64, P, i64, P, i64, P, i64, i32, i32]
+    L.oro_crt_dgemm.argtypes = [i64, i64, i64, P, i64, P, i64, P, i64, i32, i32, i32]
     L.oro_crt_dgemm.restype = i32
+    L.oro_crt_q_sigma.argtypes = [ctypes.c_double, i32]
+    L.oro_crt_q_sigma.restype = i64
+    L.oro_crt_split_sigma.argtypes = [P, i64, i64, i64, i64, P, i32, P]
+    L.oro_crt_normbound.argtypes = [P, i64, i64, i64, i64, P, P]
+    L.oro_crt_norm_m.argtypes = [ctypes.c_uint64, i32, i64]
+    L.oro_crt_norm_m.restype = i32
+    L.oro_crt_sigma_norm.argtypes = [P, i64, i64, i64, i64, i32, i64, P]
+    L.oro_crt_sigma_norm.restype = i32
+    L.oro_crt_sigma_max.argtypes = [P, i64, i64, i64, i64, i32, P]
     if hasattr(L, "oro_mpfr_dot"):
         L.oro_mpfr_dot.argtypes = [P, i64, P, i64, i64, i32]
         L.oro_mpfr_dot.restype = ctypes.c_double
@@ -297,14 +308,60 @@ def crt_to_double(v, e2, rounding=1):
     return lib().oro_crt_to_double(_p(C3), int(e2), int(rounding))
 
 
-def crt_dgemm(A, B, n=16, rounding=1):
-    """End-to-end CPU CRT-mode DGEMM."""
+def crt_dgemm(A, B, n=16, rounding=1, bits=0):
+    """End-to-end CPU CRT-mode DGEMM (bits 0: the norm rule, else the max rule)."""
     A = np.ascontiguousarray(A, dtype=np.float64)
     B = np.ascontiguousarray(B, dtype=np.float64)
     M, K = A.shape
     N = B.shape[1]
     C = np.zeros((M, N), dtype=np.float64)
-    rc = lib().oro_crt_dgemm(M, N, K, _p(A), K, _p(B), N, _p(C), N, int(n), int(rounding))
+    rc = lib().oro_crt_dgemm(M, N, K, _p(A), K, _p(B), N, _p(C), N, int(n), int(bits), int(rounding))
     if rc:
         raise ValueError("no bit count fits these moduli and K")
     return C
+
+
+def _rows_view(X, axis_contract):
+    X = np.asarray(X, dtype=np.float64)
+    if axis_contract == 1:
+        rows, K = X.shape
+        return X, rows, K, X.strides[0] // 8, X.strides[1] // 8
+    K, rows = X.shape
+    return X, rows, K, X.strides[1] // 8, X.strides[0] // 8
+
+
+def crt_q_sigma(x, sigma):
+    return lib().oro_crt_q_sigma(float(x), int(sigma))
+
+
+def crt_normbound(X, axis_contract=1):
+    """(E^ int32[rows], S uint64[rows]) of the norm rule: ||x||^2 <= S 4^(E^ - 1038)."""
+    X, rows, K, sr, sc = _rows_view(X, axis_contract)
+    Eh = np.zeros(rows, dtype=np.int32)
+    S = np.zeros(rows, dtype=np.uint64)
+    lib().oro_crt_normbound(_p(X), rows, K, sr, sc, _p(Eh), _p(S))
+    return Eh, S
+
+
+def crt_norm_m(S, n, K):
+    return lib().oro_crt_norm_m(int(S), int(n), int(K))
+
+
+def crt_sigma(X, n, axis_contract=1, bits=0, K=None):
+    """Per-row shifts sigma: the norm rule (bits 0) or the max rule (beta = bits)."""
+    X, rows, Kx, sr, sc = _rows_view(X, axis_contract)
+    sg = np.zeros(rows, dtype=np.int32)
+    if bits:
+        lib().oro_crt_sigma_max(_p(X), rows, Kx, sr, sc, int(bits), _p(sg))
+    elif lib().oro_crt_sigma_norm(_p(X), rows, Kx, sr, sc, int(n), int(K or Kx), _p(sg)):
+        raise ValueError("no bit count fits these moduli and K")
+    return sg
+
+
+def crt_split_sigma(X, sigma, n, axis_contract=1):
+    """Residues int8[n][rows][K] of X for per-row shifts sigma."""
+    X, rows, K, sr, sc = _rows_view(X, axis_contract)
+    sigma = np.ascontiguousarray(sigma, dtype=np.int32)
+    S = np.zeros((n, rows, K), dtype=np.int8)
+    lib().oro_crt_split_sigma(_p(X), rows, K, sr, sc, _p(sigma), int(n), _p(S))
+    return S

@@ -13,14 +13,29 @@
  *   moduli      p = 256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211, 199,
  *               197, 193, 191, 181, 179, 173, 167 (the first n, 2 <= n <= 20; pairwise
  *               coprime); M = prod p (below 2^156).
- *   bits        beta = floor((L - 2 - ceil(log2 K)) / 2), L = floor(log2 M), at most
- *               62: then K * 2^(2 beta) <= M / 4, so |C'| <= M / 4.
- *   scaling     the row (A) / column (B) maximum exponent field E as in the split
- *               scheme; q(x) = sign(x) * round(M_x * 2^(beta - 53 - (E - e'))), |q| <= 2^beta,
- *               e' = max(e, 1), i.e. q = sign(x) round(|x| 2^(1022 + beta - E)) for every
- *               finite x (unlike the split scheme, subnormals are not off by a factor 2);
- *               round = to nearest, ties away from zero (half the error of truncation and
- *               no bias; |q| <= 2^beta keeps K * 2^(2 beta) as the bound of |C'|).
+ *   shift       every row of A and column of B has an integer shift sigma, and each
+ *               element becomes q(x) = sign(x) round(M_x 2^(e' - sigma)), e' = max(e, 1),
+ *               M_x its 53-bit significand, i.e. q = sign(x) round(|x| 2^(1075 - sigma))
+ *               for every finite x (unlike the split scheme, subnormals are not off by a
+ *               factor 2); round = to nearest, ties away from zero.  Two rules for sigma:
+ *   max rule    (explicit bits beta) sigma = 53 + E - beta with E the row's maximum
+ *               exponent field as in the split scheme: |q| <= 2^beta, and with
+ *               beta <= floor((L - 2 - ceil(log2 K)) / 2), L = floor(log2 M), at most 62,
+ *               K 2^(2 beta) <= M / 4 bounds |C'| by M / 4.
+ *   norm rule   (bits 0, the default where M 2^20 > K (2^20 + 2^40); below that, i.e. for
+ *               a few moduli, bits 0 means the max rule with the largest beta) from an
+ *               exact integer bound on the row's Euclidean
+ *               norm.  Chunks of 16 consecutive contraction indices (from index 0): for a
+ *               chunk with a nonzero element, Ec = max e' over them and
+ *               S_c = sum u^2, u = ceil(M_x / 2^(37 + Ec - e')) = ceil(|x| / 2^(Ec - 1038))
+ *               <= 2^16; per row E^ = max Ec, S = sum_c ceil(S_c / 4^(E^ - Ec)), so that
+ *               ||x||^2 <= S 4^(E^ - 1038).  m = the largest integer <= 46 with
+ *               S (2^22 + 4) 4^m <= M 2^20 - K (2^20 + 2^40), and sigma = 37 + E^ - m
+ *               (sigma = 0 for a zero row).  Then ||q|| <= 2^m sqrt(S) + sqrt(K) / 2 and,
+ *               by 2ab <= a^2 2^-20 + b^2 2^20, ||q||^2 <= M / 4 for every row and column,
+ *               so |C'| <= ||q_a|| ||q_b|| <= M / 4 (Cauchy-Schwarz); every |q| <= 2^62.
+ *               It gives the operands about log2(max |x| sqrt(K) / ||x||) more bits than
+ *               the max rule: 4 at K = 8192 for (U - 1/2) exp(N(0, 1)) entries.
  *   residues    r_p(x) = q(x) mod p in [-(p-1)/2, (p-1)/2] for odd p, [-128, 127] for
  *               p = 256: int8.
  *   products    g_p = (sum_k r_p(a_ik) r_p(b_kj)) mod p in [0, p) (int32 sums are exact).
@@ -29,8 +44,8 @@
  *               C' = S - k M (mod 2^192) read as a signed 192-bit integer.  Because
  *               |C'| <= M/4 and the error of t is below 2^-30, k is exact and C' is the
  *               exact integer sum_k q(a_ik) q(b_kj).
- *   output      C = C' * 2^(E_r + E_c - 2044 - 2 beta), truncated toward zero or rounded
- *               to nearest-even to 53 bits.
+ *   output      C = C' * 2^(sigma_r + sigma_c - 2150) (max rule: 2^(E_r + E_c - 2044 -
+ *               2 beta)), truncated toward zero or rounded to nearest-even to 53 bits.
  * Pinned in tests/test_oracle_crt.py against Python integer arithmetic (residues, CRT
  * identity, exact products and the final scaling) and the exact rational product.
  */
@@ -133,16 +148,152 @@ static u192 crt_W(int n, int i) {
 /* W_p as three words each (low first), f_p = fl(W_p / M), M as three words. */
 void oro_crt_consts(int n, uint64_t *W, double *f, uint64_t *Mw);
 
-/* Signed integer q(x) of one element for row/column exponent field emax. */
-int64_t oro_crt_q(double x, int emax, int beta) {
+/* Signed integer q(x) of one element for the row/column shift sigma (sigma - e' >= -9). */
+int64_t oro_crt_q_sigma(double x, int sigma) {
     uint64_t u = dbits(x);
     int e = (int)((u >> 52) & 0x7FF);
     uint64_t mx = (u & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
-    int sh = 53 + (emax - (e ? e : 1)) - beta;   /* subnormals have exponent 1 - 1023 */
+    int sh = sigma - (e ? e : 1);   /* subnormals have exponent 1 - 1023 */
     /* nearest, ties away from zero; for sh >= 54, mx / 2^sh < 2^(53 - sh) <= 1/2 strictly
      * (mx < 2^53), so those round to 0 */
     uint64_t Q = sh >= 54 ? 0 : (sh >= 1 ? (mx + (1ull << (sh - 1))) >> sh : mx << (-sh));
     return (u >> 63) ? -(int64_t)Q : (int64_t)Q;
+}
+
+/* The max rule: q(x) for row/column exponent field emax and beta bits. */
+int64_t oro_crt_q(double x, int emax, int beta) { return oro_crt_q_sigma(x, 53 + emax - beta); }
+
+/* ---- the norm rule */
+#define CRT_CHUNK 16
+#define CRT_MCAP 46
+
+static int fieldp(double x) {   /* e' = max(e, 1) */
+    int e = (int)((dbits(x) >> 52) & 0x7FF);
+    return e ? e : 1;
+}
+
+/* Per row: E^ and the integer norm bound S (||x||^2 <= S 4^(E^ - 1038)); a zero row gives
+ * E^ = 0, S = 0. */
+void oro_crt_normbound(const double *X, int64_t rows, int64_t cols, int64_t sr, int64_t sc, int32_t *Ehat,
+                       uint64_t *S) {
+    const int64_t nch = (cols + CRT_CHUNK - 1) / CRT_CHUNK;
+    #pragma omp parallel
+    {
+        int *Ec = (int *)malloc(sizeof(int) * (size_t)(nch > 0 ? nch : 1));
+        uint64_t *Sc = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(nch > 0 ? nch : 1));
+        #pragma omp for schedule(static)
+        for (int64_t r = 0; r < rows; r++) {
+            int Eh = 0;
+            for (int64_t c = 0; c < nch; c++) {
+                int em = 0;
+                for (int64_t k = c * CRT_CHUNK; k < cols && k < (c + 1) * CRT_CHUNK; k++) {
+                    const double x = X[r * sr + k * sc];
+                    if (x != 0.0 && fieldp(x) > em) em = fieldp(x);
+                }
+                uint64_t s = 0;
+                for (int64_t k = c * CRT_CHUNK; em && k < cols && k < (c + 1) * CRT_CHUNK; k++) {
+                    const double x = X[r * sr + k * sc];
+                    if (x == 0.0) continue;
+                    const uint64_t u = dbits(x);
+                    const int e = (int)((u >> 52) & 0x7FF);
+                    const uint64_t mx = (u & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
+                    const int t = 37 + em - fieldp(x);   /* >= 37 */
+                    const uint64_t v = t >= 64 ? 1 : (mx + ((1ull << t) - 1)) >> t;   /* ceil, <= 2^16 */
+                    s += v * v;
+                }
+                Ec[c] = em;
+                Sc[c] = s;
+                if (em > Eh) Eh = em;
+            }
+            uint64_t tot = 0;
+            for (int64_t c = 0; c < nch; c++) {
+                if (!Sc[c]) continue;
+                const int d2 = 2 * (Eh - Ec[c]);   /* S_c < 2^37: the ceiling is 1 from 2^37 on */
+                tot += d2 >= 40 ? 1 : (Sc[c] + ((1ull << d2) - 1)) >> d2;
+            }
+            Ehat[r] = Eh;
+            S[r] = tot;
+        }
+        free(Ec);
+        free(Sc);
+    }
+}
+
+static u192 u192_shl(u192 a, int s) {   /* s < 192 */
+    u192 r = {{0, 0, 0}};
+    const int w = s / 64, b = s % 64;
+    for (int i = 2; i >= 0; i--) {
+        if (i - w < 0) continue;
+        uint64_t v = a.w[i - w] << b;
+        if (b && i - w - 1 >= 0) v |= a.w[i - w - 1] >> (64 - b);
+        r.w[i] = v;
+    }
+    return r;
+}
+
+static int u192_le(u192 a, u192 b) {
+    for (int i = 2; i >= 0; i--)
+        if (a.w[i] != b.w[i]) return a.w[i] < b.w[i];
+    return 1;
+}
+
+/* R = M 2^20 - K (2^20 + 2^40); returns 0 if it is not positive. */
+static int crt_norm_R(int n, int64_t K, u192 *R) {
+    const u192 MR = u192_shl(crt_M(n), 20);
+    const u192 t = u192_mul_small(u192_small(((uint64_t)1 << 20) + ((uint64_t)1 << 40)), (uint64_t)K);
+    if (u192_le(MR, t)) return 0;
+    *R = u192_add(MR, u192_neg(t));
+    return 1;
+}
+
+/* S (2^22 + 4) 4^m <= R */
+static int norm_fits(u128 Sp, u192 R, int m) {
+    if (m >= 0) {
+        u192 a = {{(uint64_t)Sp, (uint64_t)(Sp >> 64), 0}};
+        return u192_le(u192_shl(a, 2 * m), R);
+    }
+    const int nb = u192_bitlen(R);
+    if (nb + (-2 * m) > 72) return 1;   /* R 4^-m >= 2^72 > Sp */
+    const u128 r = ((u128)R.w[1] << 64 | R.w[0]) << (-2 * m);
+    return Sp <= r;
+}
+
+/* m of the norm rule for bound S (> 0), or -1000 if R <= 0. */
+int oro_crt_norm_m(uint64_t S, int n, int64_t K) {
+    u192 R;
+    if (!crt_norm_R(n, K, &R)) return -1000;
+    const u128 Sp = (u128)S * ((1u << 22) + 4);
+    int bs = 0;
+    for (u128 v = Sp; v; v >>= 1) bs++;
+    int m = (u192_bitlen(R) - bs) / 2 + 1;
+    if (m > CRT_MCAP) m = CRT_MCAP;
+    while (!norm_fits(Sp, R, m)) m--;
+    return m;
+}
+
+/* Norm-rule shifts of a rows x cols operand (contraction along cols); -1 if none fits. */
+int oro_crt_sigma_norm(const double *X, int64_t rows, int64_t cols, int64_t sr, int64_t sc, int n, int64_t K,
+                       int32_t *sigma) {
+    u192 R;
+    if (n < 2 || n > CRT_MAXMOD || !crt_norm_R(n, K, &R)) return -1;
+    int32_t *Eh = (int32_t *)malloc(sizeof(int32_t) * (size_t)(rows > 0 ? rows : 1));
+    uint64_t *S = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(rows > 0 ? rows : 1));
+    oro_crt_normbound(X, rows, cols, sr, sc, Eh, S);
+    for (int64_t r = 0; r < rows; r++)
+        sigma[r] = S[r] ? 37 + Eh[r] - oro_crt_norm_m(S[r], n, K) : 0;
+    free(Eh);
+    free(S);
+    return 0;
+}
+
+/* Max-rule shifts: sigma = 53 + E - beta. */
+void oro_crt_sigma_max(const double *X, int64_t rows, int64_t cols, int64_t sr, int64_t sc, int beta,
+                       int32_t *sigma) {
+    extern void oro_max_exp(const double *, int64_t, int64_t, int64_t, int64_t, int16_t *);
+    int16_t *E = (int16_t *)malloc(sizeof(int16_t) * (size_t)(rows > 0 ? rows : 1));
+    oro_max_exp(X, rows, cols, sr, sc, E);
+    for (int64_t r = 0; r < rows; r++) sigma[r] = 53 + E[r] - beta;
+    free(E);
 }
 
 static inline int sym_mod(int64_t q, int p) {
@@ -152,16 +303,25 @@ static inline int sym_mod(int64_t q, int p) {
     return (int)(r > half ? r - p : r);
 }
 
-/* Residues of a rows x cols operand (element (r, c) at X[r*sr + c*sc], exponent field E[r]
- * per row) into S[i][r][c] (int8), i < n. */
-void oro_crt_split(const double *X, int64_t rows, int64_t cols, int64_t sr, int64_t sc, const int16_t *E,
-                   int beta, int n, int8_t *S) {
+/* Residues of a rows x cols operand (element (r, c) at X[r*sr + c*sc], shift sigma[r] per
+ * row) into S[i][r][c] (int8), i < n. */
+void oro_crt_split_sigma(const double *X, int64_t rows, int64_t cols, int64_t sr, int64_t sc,
+                         const int32_t *sigma, int n, int8_t *S) {
     #pragma omp parallel for schedule(static)
     for (int64_t r = 0; r < rows; r++)
         for (int64_t c = 0; c < cols; c++) {
-            int64_t q = oro_crt_q(X[r * sr + c * sc], E[r], beta);
+            int64_t q = oro_crt_q_sigma(X[r * sr + c * sc], sigma[r]);
             for (int i = 0; i < n; i++) S[((int64_t)i * rows + r) * cols + c] = (int8_t)sym_mod(q, crt_p[i]);
         }
+}
+
+/* The same with the max rule: exponent field E[r] per row, beta bits. */
+void oro_crt_split(const double *X, int64_t rows, int64_t cols, int64_t sr, int64_t sc, const int16_t *E,
+                   int beta, int n, int8_t *S) {
+    int32_t *sg = (int32_t *)malloc(sizeof(int32_t) * (size_t)(rows > 0 ? rows : 1));
+    for (int64_t r = 0; r < rows; r++) sg[r] = 53 + E[r] - beta;
+    oro_crt_split_sigma(X, rows, cols, sr, sc, sg, n, S);
+    free(sg);
 }
 
 /* g_p = (sum_k SA_p[r][k] SB_p[c][k]) mod p in [0, p): G[i][r][c] (uint8). */
@@ -257,22 +417,32 @@ double oro_crt_to_double(const int64_t *C3, int e2, int rounding) {
     return neg ? -v : v;
 }
 
-/* End-to-end CRT-mode DGEMM, row-major A (M x K, lda), B (K x N, ldb), C (M x N, ldc).
- * Returns 0, or -1 if no bit count fits (n, K). */
+/* End-to-end CRT-mode DGEMM, row-major A (M x K, lda), B (K x N, ldb), C (M x N, ldc);
+ * bits 0: the norm rule (or the max rule with the largest beta where the norm rule's bound
+ * has no room), else the max rule with beta = bits.  Returns 0, or -1 if the
+ * moduli do not fit (n, K, bits). */
 int oro_crt_dgemm(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
-                  double *C, int64_t ldc, int n, int rounding) {
-    int beta = oro_crt_beta(n, K);
-    if (beta < 0) return -1;
-    int16_t *Er = (int16_t *)malloc(sizeof(int16_t) * (size_t)(M > 0 ? M : 1));
-    int16_t *Ec = (int16_t *)malloc(sizeof(int16_t) * (size_t)(N > 0 ? N : 1));
+                  double *C, int64_t ldc, int n, int bits, int rounding) {
+    const int bmax = oro_crt_beta(n, K);
+    if (bmax < 0 || bits < 0 || bits > bmax) return -1;
+    u192 R;
+    if (bits == 0 && !crt_norm_R(n, K, &R)) bits = bmax;   /* few moduli: the max rule */
+    int32_t *Er = (int32_t *)malloc(sizeof(int32_t) * (size_t)(M > 0 ? M : 1));
+    int32_t *Ec = (int32_t *)malloc(sizeof(int32_t) * (size_t)(N > 0 ? N : 1));
     int8_t *SA = (int8_t *)malloc((size_t)n * M * K + 1);
     int8_t *SB = (int8_t *)malloc((size_t)n * N * K + 1);
     uint8_t *G = (uint8_t *)malloc((size_t)n * M * N + 1);
-    extern void oro_max_exp(const double *, int64_t, int64_t, int64_t, int64_t, int16_t *);
-    oro_max_exp(A, M, K, lda, 1, Er);
-    oro_max_exp(B, N, K, 1, ldb, Ec);
-    oro_crt_split(A, M, K, lda, 1, Er, beta, n, SA);
-    oro_crt_split(B, N, K, 1, ldb, Ec, beta, n, SB);
+    if (bits == 0) {
+        if (oro_crt_sigma_norm(A, M, K, lda, 1, n, K, Er) || oro_crt_sigma_norm(B, N, K, 1, ldb, n, K, Ec)) {
+            free(Er); free(Ec); free(SA); free(SB); free(G);
+            return -1;
+        }
+    } else {
+        oro_crt_sigma_max(A, M, K, lda, 1, bits, Er);
+        oro_crt_sigma_max(B, N, K, 1, ldb, bits, Ec);
+    }
+    oro_crt_split_sigma(A, M, K, lda, 1, Er, n, SA);
+    oro_crt_split_sigma(B, N, K, 1, ldb, Ec, n, SB);
     oro_crt_products(SA, SB, M, N, K, n, G);
     u192 Wt[CRT_MAXMOD], Mt;
     double ft[CRT_MAXMOD];
@@ -282,7 +452,7 @@ int oro_crt_dgemm(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
         for (int64_t c = 0; c < N; c++) {
             int64_t C3[3];
             rebuild_with(Wt, ft, Mt, G + r * N + c, M * N, n, C3);
-            C[r * ldc + c] = oro_crt_to_double(C3, Er[r] + Ec[c] - 2044 - 2 * beta, rounding);
+            C[r * ldc + c] = oro_crt_to_double(C3, Er[r] + Ec[c] - 2150, rounding);
         }
     free(Er); free(Ec); free(SA); free(SB); free(G);
     return 0;

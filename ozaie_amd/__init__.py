@@ -201,7 +201,7 @@ def mode_slices(mode, K):
         lk = min(max(int(math.log2(K)), 7), 14)
         return table[lk]
     if mode == Mode.crt:
-        return crt_beta(16, K), 16
+        return 0, 16          # bits 0: shifts by the norm rule, 16 moduli
     return {Mode.fast: (7, 8), Mode.fast3: (7, 3), Mode.accurate7: (7, 7)}[mode]
 
 

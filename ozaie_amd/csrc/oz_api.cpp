@@ -144,11 +144,21 @@ int crt_beta(int n, int64_t K) {
     return beta >= 1 ? beta : -1;
 }
 
+// The norm rule needs R = M 2^20 - K (2^20 + 2^40) > 0.
+bool crt_norm_ok(int n, int64_t K) {
+    if (n < 2 || n > kCrtMaxMod || K < 1 || K > 65536) return false;
+    const Wide M = crt_M(n);
+    if (M.w[2] || (M.w[1] >> 44)) return true;   // M 2^20 >= 2^128 > K (2^20 + 2^40)
+    const unsigned __int128 MR = (((unsigned __int128)M.w[1] << 64) | M.w[0]) << 20;
+    return MR > (unsigned __int128)K * ((1ull << 20) + (1ull << 40));
+}
+
 void crt_consts(int n, int beta, CrtConsts &c) {
     std::memset(&c, 0, sizeof(c));
     c.n = n;
     c.beta = beta;
     const Wide M = crt_M(n);
+    for (int i = 0; i < 3; i++) c.Mw[i] = M.w[i];
     for (int i = 0; i < n; i++) {
         const int p = kCrtModuli[i];
         Wide Mi = M;
@@ -192,7 +202,8 @@ struct ozaie_plan_s {
     CrtConsts cc{};
     uint32_t *R = nullptr;  // CRT residue products [d][Mp/4][Np]
     int8_t *SA = nullptr, *SB = nullptr;
-    int32_t *Er = nullptr, *Ec = nullptr;
+    int32_t *Er = nullptr, *Ec = nullptr;   // CRT mode: the shifts sigma
+    uint64_t *stA = nullptr, *stB = nullptr; // CRT mode: chunk statistics of A and B
     __int128 *T = nullptr;
     // staging for host pointers (reference-compatible entry point)
     double *dA = nullptr, *dB = nullptr, *dC = nullptr;
@@ -217,6 +228,8 @@ void plan_free(ozaie_plan p) {
     if (p->T) (void)hipFree(p->T);
     if (p->R) (void)hipFree(p->R);
     if (p->X) (void)hipFree(p->X);
+    if (p->stA) (void)hipFree(p->stA);
+    if (p->stB) (void)hipFree(p->stB);
     if (p->dA) (void)hipFree(p->dA);
     if (p->dB) (void)hipFree(p->dB);
     if (p->dC) (void)hipFree(p->dC);
@@ -296,7 +309,8 @@ int plan_configure(ozaie_plan p, int64_t M, int64_t N, int64_t K, int b, int d, 
     return OZAIE_OK;
 }
 
-// CRT mode: n moduli, beta bits per operand (0: the largest exact one for K).
+// CRT mode: n moduli, beta bits per operand (the max rule), or 0: the norm rule (the max
+// rule with the largest beta where M 2^20 <= K (2^20 + 2^40), i.e. for a few moduli).
 int plan_configure_crt(ozaie_plan p, int64_t M, int64_t N, int64_t K, int beta, int n, int rounding) {
     if (M < 1 || N < 1 || K < 1) { g_last_error = "dimensions must be >= 1"; return OZAIE_ERR_DIMS; }
     if (M > (1ll << 31) - 512 || N > (1ll << 31) - 512) { g_last_error = "dimensions too large"; return OZAIE_ERR_DIMS; }
@@ -304,8 +318,8 @@ int plan_configure_crt(ozaie_plan p, int64_t M, int64_t N, int64_t K, int beta, 
     if (n < 2 || n > kCrtMaxMod) { g_last_error = "CRT mode needs 2 <= moduli <= 20"; return OZAIE_ERR_ARG; }
     const int bmax = crt_beta(n, K);
     if (bmax < 1) { g_last_error = "no exact bit count for these moduli and K"; return OZAIE_ERR_ARG; }
-    if (beta == 0) beta = bmax;
-    if (beta < 1 || beta > bmax) { g_last_error = "CRT bits exceed the exact range of the moduli"; return OZAIE_ERR_ARG; }
+    if (beta < 0 || beta > bmax) { g_last_error = "CRT bits exceed the exact range of the moduli"; return OZAIE_ERR_ARG; }
+    if (beta == 0 && !crt_norm_ok(n, K)) beta = bmax;   // few moduli: the norm rule has no room
     p->M = M; p->N = N; p->K = K; p->b = beta; p->d = n; p->rounding = rounding;
     p->crt = true;
     crt_consts(n, beta, p->cc);
@@ -331,7 +345,11 @@ int plan_alloc(ozaie_plan p, bool staging) {
     HIPCHK(hipMalloc(&p->Ec, sizeof(int32_t) * p->Np));
     if (!p->crt && (int64_t)p->nchunks * p->ngroups > 1)
         HIPCHK(hipMalloc(&p->T, sizeof(__int128) * (size_t)p->M * p->N));
-    if (p->crt) HIPCHK(hipMalloc(&p->R, (size_t)p->d * p->Mp * p->Np));
+    if (p->crt) {
+        HIPCHK(hipMalloc(&p->R, (size_t)p->d * p->Mp * p->Np));
+        HIPCHK(hipMalloc(&p->stA, sizeof(uint64_t) * (size_t)p->Mp * crt_chunks(p->Kb)));
+        HIPCHK(hipMalloc(&p->stB, sizeof(uint64_t) * (size_t)p->Np * crt_chunks(p->Kb)));
+    }
     if (p->pairs) HIPCHK(hipMalloc(&p->X, sizeof(int32_t) * (size_t)p->Mp * p->Np));
     if (staging) {
         HIPCHK(hipMalloc(&p->dA, sizeof(double) * (size_t)p->M * p->K));
@@ -391,17 +409,23 @@ GemmLaunch base_launch(const ozaie_plan p) {
 // through the memory-side cache (256 MB).  Measured: 2-3% off the split-scheme split phase
 // at 4096^3, no change for the CRT residues or at 8192^3.
 int run_split(ozaie_plan p, const OperandView &va, const OperandView &vb, hipStream_t st) {
-    HIPCHK(launch_maxexp(va, p->Er, p->Mp, st));
-    if (p->crt)
+    if (p->crt) {   // chunk statistics, shifts, residues
+        const int64_t nch = crt_chunks(p->Kb);
+        HIPCHK(launch_crt_stats(va, p->stA, p->Mp, p->Kp, st));
+        HIPCHK(launch_crt_scale(p->stA, p->Mp, nch, p->K, p->cc, p->Er, st));
         HIPCHK(launch_crt_split(va, p->Er, p->cc, p->SA, p->Mp, p->Kp, st));
-    else if (p->pairs)
+        HIPCHK(launch_crt_stats(vb, p->stB, p->Np, p->Kp, st));
+        HIPCHK(launch_crt_scale(p->stB, p->Np, nch, p->K, p->cc, p->Ec, st));
+        HIPCHK(launch_crt_split(vb, p->Ec, p->cc, p->SB, p->Np, p->Kp, st));
+        return OZAIE_OK;
+    }
+    HIPCHK(launch_maxexp(va, p->Er, p->Mp, st));
+    if (p->pairs)
         HIPCHK(launch_split_pairs(va, p->Er, p->b, p->d, p->SA, p->Mp, p->Kp, false, st));
     else
         HIPCHK(launch_split(va, p->Er, p->b, p->d, p->SA, p->Mp, p->Kp, st));
     HIPCHK(launch_maxexp(vb, p->Ec, p->Np, st));
-    if (p->crt)
-        HIPCHK(launch_crt_split(vb, p->Ec, p->cc, p->SB, p->Np, p->Kp, st));
-    else if (p->pairs)
+    if (p->pairs)
         HIPCHK(launch_split_pairs(vb, p->Ec, p->b, p->d, p->SB, p->Np, p->Kp, true, st));
     else
         HIPCHK(launch_split(vb, p->Ec, p->b, p->d, p->SB, p->Np, p->Kp, st));
@@ -507,7 +531,11 @@ int64_t noalloc_bytes(const ozaie_plan p) {
     take(4 * p->Mp);
     take(4 * p->Np);
     if (!p->crt && (int64_t)p->nchunks * p->ngroups > 1) take(16 * p->M * p->N);
-    if (p->crt) take((int64_t)p->d * p->Mp * p->Np);
+    if (p->crt) {
+        take((int64_t)p->d * p->Mp * p->Np);
+        take(8 * p->Mp * crt_chunks(p->Kb));
+        take(8 * p->Np * crt_chunks(p->Kb));
+    }
     if (p->pairs) take(4 * p->Mp * p->Np);
     return off;
 }
@@ -695,7 +723,11 @@ int32_t ozaie_dgemm_noalloc(int64_t M, int64_t N, int64_t K, int bits, int slice
     p.Er = reinterpret_cast<int32_t *>(take(4 * p.Mp));
     p.Ec = reinterpret_cast<int32_t *>(take(4 * p.Np));
     if (!p.crt && (int64_t)p.nchunks * p.ngroups > 1) p.T = reinterpret_cast<__int128 *>(take(16 * p.M * p.N));
-    if (p.crt) p.R = reinterpret_cast<uint32_t *>(take((int64_t)p.d * p.Mp * p.Np));
+    if (p.crt) {
+        p.R = reinterpret_cast<uint32_t *>(take((int64_t)p.d * p.Mp * p.Np));
+        p.stA = reinterpret_cast<uint64_t *>(take(8 * p.Mp * crt_chunks(p.Kb)));
+        p.stB = reinterpret_cast<uint64_t *>(take(8 * p.Np * crt_chunks(p.Kb)));
+    }
     if (p.pairs) p.X = reinterpret_cast<int32_t *>(take(4 * p.Mp * p.Np));
     // Every intermediate is fully written before it is read (exponent arrays that are
     // combined with atomicMax are cleared by a fill kernel first), so the previous call's

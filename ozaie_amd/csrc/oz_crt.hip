@@ -5,21 +5,142 @@
 namespace oz {
 
 // ------------------------------------------------------------------ CRT mode
-// An exact integer product instead of the triangle of slice products: each element
-// becomes q = sign(x) round(|x| 2^(1022 + beta - E)) (to nearest, ties away from zero;
-// |q| <= 2^beta), the operands are
-// multiplied modulo n pairwise-coprime moduli p <= 256 with one exact int8 GEMM each,
-// and the Chinese remainder theorem rebuilds sum_k q_a q_b exactly (|C'| <= M / 4).
+// An exact integer product instead of the triangle of slice products: each element of a
+// row with shift sigma becomes q = sign(x) round(|x| 2^(1075 - sigma)) (to nearest, ties
+// away from zero), the operands are multiplied modulo n pairwise-coprime moduli p <= 256
+// with one exact int8 GEMM each, and the Chinese remainder theorem rebuilds
+// sum_k q_a q_b exactly (|C'| <= M / 4).
+//
+// Shifts.  Max rule (bits beta > 0): sigma = 53 + E - beta, E the row's maximum exponent
+// field, so |q| <= 2^beta and K 2^(2 beta) <= M / 4 bounds |C'|.  Norm rule (the
+// default): from an exact integer bound S on the row's Euclidean norm, ||x||^2 <=
+// S 4^(E^ - 1038), built from chunks of 16 contraction indices: per chunk Ec = max(max
+// exponent field, 1) and S_c = sum ceil(|x| / 2^(Ec - 1038))^2 (each ceiling <= 2^16),
+// per row E^ = max Ec and S = sum_c ceil(S_c / 4^(E^ - Ec)).  m is the largest integer
+// <= 46 with S (2^22 + 4) 4^m <= M 2^20 - K (2^20 + 2^40), and sigma = 37 + E^ - m.
+// Then (2^m sqrt(S) + sqrt(K) / 2)^2 <= M / 4 (2ab <= a^2 2^-20 + b^2 2^20), which bounds
+// ||q|| for every row and column, and |C'| <= ||q_a|| ||q_b|| <= M / 4 (Cauchy-Schwarz);
+// |q| <= 2^(16 + m) <= 2^62.  All of it is integer arithmetic: the same on every run,
+// layout and launch shape.
 
-// Residues of 16 consecutive elements v[i] of one row (row maximum exponent field emax,
-// beta bits) into one 16-byte chunk per modulus plane at dst (planes `plane` bytes apart).
+__host__ __device__ inline uint64_t crt_bits(double x) {
+    uint64_t u;
+    __builtin_memcpy(&u, &x, 8);
+    return u;
+}
+
+// One chunk of 16 elements: (max exponent field << 40) | S_c (S_c < 2^37; 0 for a chunk
+// of zeros).
+__host__ __device__ inline uint64_t crt_chunk_stat(const double (&v)[16]) {
+    int em = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const int e = (int)((crt_bits(v[i]) >> 52) & 0x7FF);
+        em = e > em ? e : em;
+    }
+    const int ec = em > 1 ? em : 1;
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint64_t u = crt_bits(v[i]);
+        const int e = (int)((u >> 52) & 0x7FF);
+        const uint64_t mx = (u & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
+        const int t = 37 + ec - (e ? e : 1);   // >= 37
+        const uint32_t w = mx == 0 ? 0u : (t >= 64 ? 1u : (uint32_t)((mx + ((1ull << t) - 1)) >> t));   // <= 2^16
+        s += (uint64_t)w * w;
+    }
+    return ((uint64_t)em << 40) | s;
+}
+
+// ceil(S_c / 4^(eh - max(E_c, 1))) of one chunk word (0 for a chunk of zeros).
+__host__ __device__ inline uint64_t crt_chunk_term(uint64_t w, int eh) {
+    const uint64_t sc = w & ((1ull << 40) - 1);
+    if (!sc) return 0;
+    const int ec = (int)(w >> 40);
+    const int d2 = 2 * (eh - (ec > 1 ? ec : 1));   // S_c < 2^37: the ceiling is 1 from 2^37 on
+    return d2 >= 40 ? 1 : (sc + ((1ull << d2) - 1)) >> d2;
+}
+
+__host__ __device__ inline int bitlen128(u128 a) {
+    const uint64_t h = (uint64_t)(a >> 64), l = (uint64_t)a;
+    return h ? 128 - __builtin_clzll(h) : (l ? 64 - __builtin_clzll(l) : 0);
+}
+
+// m of the norm rule for bound S > 0 and contraction length K (M 2^20 > K (2^20 + 2^40)).
+__host__ __device__ inline int crt_norm_m(uint64_t S, int64_t K, const CrtConsts &c) {
+    // R = M 2^20 - K (2^20 + 2^40), three words
+    const uint64_t t = (uint64_t)K * ((1ull << 20) + (1ull << 40));   // K <= 2^16: < 2^57
+    uint64_t r0 = c.Mw[0] << 20, r1 = (c.Mw[1] << 20) | (c.Mw[0] >> 44), r2 = (c.Mw[2] << 20) | (c.Mw[1] >> 44);
+    const uint64_t b0 = r0 < t;
+    r0 -= t;
+    const uint64_t b1 = b0 & (uint64_t)(r1 == 0);
+    r1 -= b0;
+    r2 -= b1;
+    const int bR = r2 ? 128 + 64 - __builtin_clzll(r2) : bitlen128(((u128)r1 << 64) | r0);
+    const u128 Sp = (u128)S * ((1u << 22) + 4);   // < 2^71
+    const int bS = bitlen128(Sp);
+    int m = (bR - bS) / 2 + 1;
+    if (m > 46) m = 46;
+    for (;; m--) {
+        bool fits;
+        if (m >= 0) {   // Sp << 2m < 2^163: three words
+            const int sh = 2 * m;
+            const uint64_t s0 = (uint64_t)Sp, s1 = (uint64_t)(Sp >> 64);
+            uint64_t a0, a1, a2;
+            if (sh >= 128) { a0 = 0; a1 = 0; a2 = s0 << (sh - 128); }
+            else if (sh >= 64) {
+                a0 = 0;
+                a1 = s0 << (sh - 64);
+                a2 = (sh > 64 ? s0 >> (128 - sh) : 0) | (s1 << (sh - 64));
+            } else {
+                a0 = s0 << sh;
+                a1 = (sh ? s0 >> (64 - sh) : 0) | (s1 << sh);
+                a2 = sh ? s1 >> (64 - sh) : 0;
+            }
+            fits = a2 != r2 ? a2 < r2 : (a1 != r1 ? a1 < r1 : a0 <= r0);
+        } else if (bR - 2 * m > 72) {
+            fits = true;   // R 4^-m >= 2^72 > Sp
+        } else {
+            fits = Sp <= ((((u128)r1 << 64) | r0) << (-2 * m));
+        }
+        if (fits) return m;
+    }
+}
+
+// sigma of one row from its chunk words st[0, nch) (whole wave; every lane returns it).
+__device__ __forceinline__ int crt_row_sigma(const uint64_t *__restrict__ st, int64_t nch, int64_t K, int beta,
+                                             const CrtConsts &c, int lane) {
+    int em = 0;
+    bool nz = false;
+    for (int64_t i = lane; i < nch; i += 64) {
+        const uint64_t w = st[i];
+        em = max(em, (int)(w >> 40));
+        nz |= (w & ((1ull << 40) - 1)) != 0;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) em = max(em, __shfl_xor(em, o, 64));
+    if (beta) return 53 + em - beta;
+    if (!__any(nz)) return 0;   // a row of zeros
+    const int eh = em > 1 ? em : 1;
+    uint64_t S = 0;
+    for (int64_t i = lane; i < nch; i += 64) S += crt_chunk_term(st[i], eh);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)S, o, 64), hi = __shfl_xor((uint32_t)(S >> 32), o, 64);
+        S += ((uint64_t)hi << 32) | lo;
+    }
+    return 37 + eh - crt_norm_m(S, K, c);
+}
+
+// Residues of 16 consecutive elements v[i] of one row (shift sigma, sigma - e' >= -9) into
+// one 16-byte chunk per modulus plane at dst (planes `plane` bytes apart).
 // With q = +-(hi 2^31 + lo), the value x = +-(hi (2^31 mod p) + lo) is an exact double
 // (|x| < 2^40) congruent to q, and x - p n, n the integer nearest x / p, is its residue in
 // [-(p-1)/2, (p-1)/2] for odd p: x / p is never within 1 / (2p) of a half-integer, far
 // above the error of the product x fl(1/p), which fma(x, fl(1/p), 1.5 2^52) rounds to an
 // integer; the residue plus 1.5 2^52 holds it in two's complement in its low word.  For
 // p = 256 (plane 0) the residue is the low byte of q in two's complement.
-__device__ __forceinline__ void crt_emit16(const double (&v)[16], int emax, int beta, const CrtConsts &c,
+__device__ __forceinline__ void crt_emit16(const double (&v)[16], int sigma, const CrtConsts &c,
                                            int8_t *__restrict__ dst, int64_t plane) {
     double hi[16], lo[16];
     uint32_t low8[4] = {0u, 0u, 0u, 0u};
@@ -28,7 +149,7 @@ __device__ __forceinline__ void crt_emit16(const double (&v)[16], int emax, int 
         const uint64_t u = (uint64_t)__double_as_longlong(v[i]);
         const int e = (int)((u >> 52) & 0x7FF);
         const uint64_t mx = (u & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
-        const int sh = 53 + emax - (e ? e : 1) - beta;   // subnormals: exponent 1 - 1023
+        const int sh = sigma - (e ? e : 1);   // subnormals: exponent 1 - 1023
         // nearest, ties away from zero (mx < 2^53: shifts of 54 and more round to 0)
         const uint64_t Q = sh >= 54 ? 0ull : (sh >= 1 ? (mx + (1ull << (sh - 1))) >> sh : mx << (-sh));
         const bool neg = (u >> 63) != 0;
@@ -63,9 +184,9 @@ __device__ __forceinline__ void crt_emit16(const double (&v)[16], int emax, int 
 // Residues of the staged tile: thread (row, q) owns 16 consecutive k of one row and
 // writes one 16-byte lane chunk per modulus plane.
 template <bool KC>
-__device__ __forceinline__ void crt_emit(const double *tile, int64_t R, const int32_t *__restrict__ Erow,
-                                         const CrtConsts &c, int beta, int8_t *__restrict__ S, int64_t Rb,
-                                         int64_t Kb, int64_t rb, int64_t kt) {
+__device__ __forceinline__ void crt_emit(const double *tile, int64_t R, const int32_t *__restrict__ Srow,
+                                         const CrtConsts &c, int8_t *__restrict__ S, int64_t Rb, int64_t Kb,
+                                         int64_t rb, int64_t kt) {
     constexpr int TK = 128;
     constexpr int LDW = KC ? (TK + 1) : (32 + 1);
     const int t = threadIdx.x;
@@ -73,37 +194,93 @@ __device__ __forceinline__ void crt_emit(const double *tile, int64_t R, const in
     const int row = t & 31, q = t >> 5;
     const int64_t kb = kt * 4 + (q >> 1);
     if (kb >= Kb) return;
-    const int emax = (r0 + row < R) ? Erow[row] : 0;
+    const int sigma = (r0 + row < R) ? Srow[row] : 0;
     double v[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) v[i] = KC ? tile[row * LDW + 16 * q + i] : tile[(16 * q + i) * LDW + row];
     const int64_t plane = Rb * Kb * (int64_t)kBlkBytes;
-    crt_emit16(v, emax, beta, c, S + (rb * Kb + kb) * (int64_t)kBlkBytes + (q & 1) * 512 + row * 16, plane);
+    crt_emit16(v, sigma, c, S + (rb * Kb + kb) * (int64_t)kBlkBytes + (q & 1) * 512 + row * 16, plane);
+}
+
+// Chunk words of the staged tile: thread (row, q) owns the chunk kt * 8 + q of its row.
+template <bool KC>
+__device__ __forceinline__ void crt_stats_emit(const double *tile, uint64_t *__restrict__ st, int64_t nch,
+                                               int64_t rb, int64_t kt) {
+    constexpr int TK = 128;
+    constexpr int LDW = KC ? (TK + 1) : (32 + 1);
+    const int t = threadIdx.x;
+    const int row = t & 31, q = t >> 5;
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) v[i] = KC ? tile[row * LDW + 16 * q + i] : tile[(16 * q + i) * LDW + row];
+    st[(rb * 32 + row) * nch + kt * 8 + q] = crt_chunk_stat(v);
+}
+
+template <bool KC>
+__global__ __launch_bounds__(256) void k_crt_stats(const double *__restrict__ X, int64_t sr, int64_t sk, int64_t R,
+                                                   int64_t K, uint64_t *__restrict__ st, int64_t nch) {
+    __shared__ double tile[kSplitLds / 8];
+    double v[16];
+    split_gather<KC>(X, sr, sk, R, K, blockIdx.y, blockIdx.x, v);
+    split_stage<KC>(v, tile);
+    __syncthreads();
+    crt_stats_emit<KC>(tile, st, nch, blockIdx.y, blockIdx.x);
+}
+
+hipError_t launch_crt_stats(const OperandView &op, uint64_t *stats, int64_t Rp, int64_t Kp, hipStream_t st) {
+    const int64_t Rb = Rp / kBlk, Kb = Kp / kBlk;
+    const int64_t nkt = (Kb + 3) / 4;
+    if (Rb > 65535 || nkt > 0x7fffffff) return hipErrorInvalidValue;
+    if (op.sk == 1)
+        hipLaunchKernelGGL(k_crt_stats<true>, dim3((unsigned)nkt, (unsigned)Rb), dim3(256), 0, st, op.X, op.sr,
+                           op.sk, op.R, op.K, stats, crt_chunks(Kb));
+    else
+        hipLaunchKernelGGL(k_crt_stats<false>, dim3((unsigned)nkt, (unsigned)Rb), dim3(256), 0, st, op.X, op.sr,
+                           op.sk, op.R, op.K, stats, crt_chunks(Kb));
+    return hipGetLastError();
+}
+
+// Wave per row (four rows per workgroup).
+__global__ __launch_bounds__(256) void k_crt_scale(const uint64_t *__restrict__ st, int64_t Rp, int64_t nch,
+                                                   int64_t K, CrtConsts c, int32_t *__restrict__ sigma) {
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= Rp) return;   // whole wave
+    const int s = crt_row_sigma(st + r * nch, nch, K, c.beta, c, threadIdx.x & 63);
+    if ((threadIdx.x & 63) == 0) sigma[r] = s;
+}
+
+hipError_t launch_crt_scale(const uint64_t *stats, int64_t Rp, int64_t nch, int64_t K, const CrtConsts &c,
+                            int32_t *sigma, hipStream_t st) {
+    const int64_t nb = (Rp + 3) / 4;
+    if (nb <= 0) return hipSuccess;
+    if (nb > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_crt_scale, dim3((unsigned)nb), dim3(256), 0, st, stats, Rp, nch, K, c, sigma);
+    return hipGetLastError();
 }
 
 template <bool KC>
 __global__ __launch_bounds__(256) void k_crt_split(const double *__restrict__ X, int64_t sr, int64_t sk,
-                                                   int64_t R, int64_t K, const int32_t *__restrict__ E,
+                                                   int64_t R, int64_t K, const int32_t *__restrict__ sigma,
                                                    int8_t *__restrict__ S, int64_t Rb, int64_t Kb, CrtConsts c) {
     __shared__ double tile[kSplitLds / 8];
     double v[16];
     split_gather<KC>(X, sr, sk, R, K, blockIdx.y, blockIdx.x, v);
     split_stage<KC>(v, tile);
     __syncthreads();
-    crt_emit<KC>(tile, R, E + (int64_t)blockIdx.y * 32, c, c.beta, S, Rb, Kb, blockIdx.y, blockIdx.x);
+    crt_emit<KC>(tile, R, sigma + (int64_t)blockIdx.y * 32, c, S, Rb, Kb, blockIdx.y, blockIdx.x);
 }
 
-hipError_t launch_crt_split(const OperandView &op, const int32_t *E, const CrtConsts &c, int8_t *S, int64_t Rp,
+hipError_t launch_crt_split(const OperandView &op, const int32_t *sigma, const CrtConsts &c, int8_t *S, int64_t Rp,
                             int64_t Kp, hipStream_t st) {
     const int64_t Rb = Rp / kBlk, Kb = Kp / kBlk;
     const int64_t nkt = (Kb + 3) / 4;
     if (Rb > 65535 || nkt > 0x7fffffff) return hipErrorInvalidValue;
     if (op.sk == 1)
         hipLaunchKernelGGL(k_crt_split<true>, dim3((unsigned)nkt, (unsigned)Rb), dim3(256), 0, st, op.X, op.sr,
-                           op.sk, op.R, op.K, E, S, Rb, Kb, c);
+                           op.sk, op.R, op.K, sigma, S, Rb, Kb, c);
     else
         hipLaunchKernelGGL(k_crt_split<false>, dim3((unsigned)nkt, (unsigned)Rb), dim3(256), 0, st, op.X, op.sr,
-                           op.sk, op.R, op.K, E, S, Rb, Kb, c);
+                           op.sk, op.R, op.K, sigma, S, Rb, Kb, c);
     return hipGetLastError();
 }
 
@@ -305,7 +482,7 @@ __device__ __forceinline__ double crt_wide_to_double(uint64_t w0, uint64_t w1, u
 // (then the same power-of-two scaling as the truncating path).
 template <bool WIDE>
 __device__ __forceinline__ void crt_rebuild_quad(const uint32_t (&wd)[kCrtMaxMod], const CrtConsts &c, int rows,
-                                                 const int32_t *__restrict__ Er4, int ec, int beta, int rounding,
+                                                 const int32_t *__restrict__ Er4, int ec, int rounding,
                                                  double (&out)[4]) {
     constexpr int NG = (WIDE ? kCrtMaxMod : kCrtNarrowMod) / 4;   // groups of four moduli
     constexpr int NL = WIDE ? kCrtLimbs : 16;                      // byte limbs of C'
@@ -359,7 +536,7 @@ __device__ __forceinline__ void crt_rebuild_quad(const uint32_t (&wd)[kCrtMaxMod
             const uint32_t hi = L[e][4 * h + 2] + (L[e][4 * h + 3] << 8);
             Q[h] = (uint64_t)lo + ((uint64_t)hi << 16);                      // below 2^48
         }
-        const int e2 = Er4[e] + ec - 2044 - 2 * beta;
+        const int e2 = Er4[e] + ec - 2150;   // shifts sigma_r + sigma_c
         if (!WIDE) {
             const u128 S = (u128)Q[0] + ((u128)Q[1] << 32) + ((u128)Q[2] << 64) + ((u128)Q[3] << 96) - (u128)128;
             out[e] = crt_to_double((i128)S, e2, rounding);
@@ -388,7 +565,7 @@ template <bool WIDE>
 __device__ __forceinline__ void crt_rebuild_cols(const uint32_t *__restrict__ Rq, int64_t plane, int64_t col0,
                                                  int64_t N, int rows, const CrtConsts &c,
                                                  const int32_t *__restrict__ Er4, const int32_t *__restrict__ Ec,
-                                                 int beta, int rounding, double *__restrict__ Cq, int64_t ldc) {
+                                                 int rounding, double *__restrict__ Cq, int64_t ldc) {
     uint4 w4[kCrtMaxMod];
 #pragma unroll
     for (int j = 0; j < kCrtMaxMod; j++)
@@ -404,7 +581,7 @@ __device__ __forceinline__ void crt_rebuild_cols(const uint32_t *__restrict__ Rq
             const uint32_t v = cc == 0 ? w4[j].x : (cc == 1 ? w4[j].y : (cc == 2 ? w4[j].z : w4[j].w));
             wd[j] = v ^ 0x80808080u;   // planes j >= n: zero words
         }
-        crt_rebuild_quad<WIDE>(wd, c, rows, Er4, Ec[col0 + cc], beta, rounding, out[cc]);
+        crt_rebuild_quad<WIDE>(wd, c, rows, Er4, Ec[col0 + cc], rounding, out[cc]);
     }
     const bool full = col0 + 3 < N;
 #pragma unroll
@@ -434,7 +611,7 @@ __global__ __launch_bounds__(256) void k_crt_rebuild(const uint32_t *__restrict_
     if (col0 >= N) return;
     const int64_t rest = M - row4 * 4;
     crt_rebuild_cols<WIDE>(R + row4 * Np, (Mp / 4) * Np, col0, N, rest < 4 ? (int)rest : 4, c, Er + row4 * 4, Ec,
-                           c.beta, rounding, C + row4 * 4 * ldc, ldc);
+                           rounding, C + row4 * 4 * ldc, ldc);
 }
 
 hipError_t launch_crt_rebuild(const uint32_t *R, int64_t Mp, int64_t Np, int64_t M, int64_t N, const int32_t *Er,
@@ -454,36 +631,83 @@ hipError_t launch_crt_rebuild(const uint32_t *R, int64_t Mp, int64_t Np, int64_t
 }
 
 // ------------------------------------------------------------------ grouped CRT
-// Many independent products in one launch per phase: exponent maxima (the grouped
-// split-scheme kernel), residues of every operand with its problem's bit count, one
-// modulus GEMM launch per tile shape over items {problem, tile row, tile col} x modulus,
-// and the rebuild over items {problem, first element}.
+// Many independent products in one launch per phase: chunk statistics and shifts of
+// every operand (each problem's rule), residues, one modulus GEMM launch per tile shape
+// over items {problem, tile row, tile col} x modulus, and the rebuild over items
+// {problem, first element}.
 
-__global__ __launch_bounds__(256) void k_grp_crt_split(const GroupOperand *__restrict__ ops,
-                                                       const CrtGroupProb *__restrict__ probs,
-                                                       const int4 *__restrict__ items, CrtConsts c) {
+__global__ __launch_bounds__(256) void k_grp_crt_stats(const GroupOperand *__restrict__ ops,
+                                                       const int4 *__restrict__ items) {
     __shared__ double tile[kSplitLds / 8];
     const int4 it = items[blockIdx.x];
     const GroupOperand &o = ops[it.x];
-    const int beta = probs[it.x >> 1].beta;
     double v[16];
     if (o.sk == 1) {
         split_gather<true>(o.X, o.sr, o.sk, o.R, o.K, it.y, it.z, v);
         split_stage<true>(v, tile);
         __syncthreads();
-        crt_emit<true>(tile, o.R, o.E + (int64_t)it.y * 32, c, beta, o.S, o.Rb, o.Kb, it.y, it.z);
+        crt_stats_emit<true>(tile, o.st, crt_chunks(o.Kb), it.y, it.z);
     } else {
         split_gather<false>(o.X, o.sr, o.sk, o.R, o.K, it.y, it.z, v);
         split_stage<false>(v, tile);
         __syncthreads();
-        crt_emit<false>(tile, o.R, o.E + (int64_t)it.y * 32, c, beta, o.S, o.Rb, o.Kb, it.y, it.z);
+        crt_stats_emit<false>(tile, o.st, crt_chunks(o.Kb), it.y, it.z);
+    }
+}
+
+hipError_t launch_group_crt_stats(const GroupOperand *ops, const int4 *items, int n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_grp_crt_stats, dim3((unsigned)n), dim3(256), 0, st, ops, items);
+    return hipGetLastError();
+}
+
+// Items {operand, row block}: wave w takes rows 8 w .. 8 w + 7 of the block.
+__global__ __launch_bounds__(256) void k_grp_crt_scale(const GroupOperand *__restrict__ ops,
+                                                       const CrtGroupProb *__restrict__ probs,
+                                                       const int4 *__restrict__ items, CrtConsts c) {
+    const int4 it = items[blockIdx.x];
+    const GroupOperand &o = ops[it.x];
+    const int beta = probs[it.x >> 1].beta;
+    const int64_t nch = crt_chunks(o.Kb);
+    const int lane = threadIdx.x & 63;
+    for (int i = 0; i < 8; i++) {
+        const int64_t r = (int64_t)it.y * 32 + (threadIdx.x >> 6) * 8 + i;
+        const int s = crt_row_sigma(o.st + r * nch, nch, o.K, beta, c, lane);
+        if (lane == 0) o.E[r] = s;
+    }
+}
+
+hipError_t launch_group_crt_scale(const GroupOperand *ops, const CrtGroupProb *probs, const int4 *items, int n,
+                                  const CrtConsts &c, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_grp_crt_scale, dim3((unsigned)n), dim3(256), 0, st, ops, probs, items, c);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_grp_crt_split(const GroupOperand *__restrict__ ops,
+                                                       const int4 *__restrict__ items, CrtConsts c) {
+    __shared__ double tile[kSplitLds / 8];
+    const int4 it = items[blockIdx.x];
+    const GroupOperand &o = ops[it.x];
+    double v[16];
+    if (o.sk == 1) {
+        split_gather<true>(o.X, o.sr, o.sk, o.R, o.K, it.y, it.z, v);
+        split_stage<true>(v, tile);
+        __syncthreads();
+        crt_emit<true>(tile, o.R, o.E + (int64_t)it.y * 32, c, o.S, o.Rb, o.Kb, it.y, it.z);
+    } else {
+        split_gather<false>(o.X, o.sr, o.sk, o.R, o.K, it.y, it.z, v);
+        split_stage<false>(v, tile);
+        __syncthreads();
+        crt_emit<false>(tile, o.R, o.E + (int64_t)it.y * 32, c, o.S, o.Rb, o.Kb, it.y, it.z);
     }
 }
 
 hipError_t launch_group_crt_split(const GroupOperand *ops, const CrtGroupProb *probs, const int4 *items, int n,
                                   const CrtConsts &c, hipStream_t st) {
+    (void)probs;
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_grp_crt_split, dim3((unsigned)n), dim3(256), 0, st, ops, probs, items, c);
+    hipLaunchKernelGGL(k_grp_crt_split, dim3((unsigned)n), dim3(256), 0, st, ops, items, c);
     return hipGetLastError();
 }
 
@@ -557,7 +781,7 @@ __global__ __launch_bounds__(256) void k_grp_crt_rebuild(const CrtGroupProb *__r
     const int64_t col0 = 4 * (int64_t)(idx - row4 * nc4);
     const int rest = pr.M - 4 * (int)row4;
     crt_rebuild_cols<WIDE>(pr.R + (int64_t)row4 * pr.Np, pr.RbA * (kBlk / 4) * pr.Np, col0, pr.N, rest < 4 ? rest : 4,
-                           c, pr.Er + 4 * (int64_t)row4, pr.Ec, pr.beta, rounding,
+                           c, pr.Er + 4 * (int64_t)row4, pr.Ec, rounding,
                            pr.C + 4 * (int64_t)row4 * pr.ldc, pr.ldc);
 }
 
@@ -611,6 +835,40 @@ __device__ __forceinline__ void crt_persist_tiles(const CrtGroupProb *__restrict
     }
 }
 
+// One row block of the persistent launch's phase A: the row maxima (one pass over the
+// block), for the norm rule the chunk terms of every row (a second pass, each lane's
+// chunks as the split kernel's, the two halves of a row summed), the shifts into o.E, and
+// the residues (a third pass).
+template <bool KC>
+__device__ __forceinline__ void crt_wave_rowblock(const GroupOperand &o, int64_t rb, int beta, const CrtConsts &c,
+                                                  double *wt, int lane) {
+    const int emax = wave_rowmax<KC>(o, rb, wt, lane);
+    int sigma;
+    if (beta) {
+        sigma = 53 + emax - beta;
+    } else {
+        const int eh = emax > 1 ? emax : 1;
+        uint64_t S = 0;
+        wave_tiles<KC>(o, rb, wt, lane, [&](const double *w_, int row, int c0, int64_t) {
+            double v[16];
+#pragma unroll
+            for (int e = 0; e < 16; e++) v[e] = w_[row * kWaveTileLd + c0 + e];
+            S += crt_chunk_term(crt_chunk_stat(v), eh);
+        });
+        const uint32_t lo = __shfl_xor((uint32_t)S, 32, 64), hi = __shfl_xor((uint32_t)(S >> 32), 32, 64);
+        S += ((uint64_t)hi << 32) | lo;
+        sigma = S ? 37 + eh - crt_norm_m(S, o.K, c) : 0;
+    }
+    if (lane < 32) o.E[rb * 32 + lane] = sigma;
+    const int64_t plane = o.Rb * o.Kb * (int64_t)kBlkBytes;
+    wave_tiles<KC>(o, rb, wt, lane, [&](const double *w_, int row, int c0, int64_t kb) {
+        double v[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) v[e] = w_[row * kWaveTileLd + c0 + e];
+        crt_emit16(v, sigma, c, o.S + (rb * o.Kb + kb) * (int64_t)kBlkBytes + lane * 16, plane);
+    });
+}
+
 // Two workgroups per CU (two waves per SIMD, 256 registers each): the memory-bound
 // residue and rebuild phases and the small tiles' contraction loops need the second wave
 // to hide latency; the 256 x 256 tile (256 accumulator registers) does not fit and runs as
@@ -623,7 +881,7 @@ void k_grp_crt_persistent(CrtPersistLaunch L, CrtConsts c) {
     const int G = gridDim.x;
     PersistCtl *ctl = reinterpret_cast<PersistCtl *>(L.ctl);
 
-    // ---- phase A: exponent maxima + residues, one 32-row block per item, per-wave lists
+    // ---- phase A: shifts + residues, one 32-row block per item, per-wave lists
     {
         const int lane = threadIdx.x & 63;
         const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -632,16 +890,8 @@ void k_grp_crt_persistent(CrtPersistLaunch L, CrtConsts c) {
         for (int i = L.offA[slot]; i < L.offA[slot + 1]; i++) {
             const int4 it = L.itemsA[i];
             const GroupOperand &o = L.ops[it.x];
-            const int beta = L.probs[it.x >> 1].beta;
-            const int64_t plane = o.Rb * o.Kb * (int64_t)kBlkBytes;
-            auto emit = [&](const double *w_, int row, int c0, int emax, int8_t *dst) {
-                double v[16];
-#pragma unroll
-                for (int e = 0; e < 16; e++) v[e] = w_[row * kWaveTileLd + c0 + e];
-                crt_emit16(v, emax, beta, c, dst, plane);
-            };
-            if (o.sk == 1) wave_rowblock_t<true>(o, it.y, wt, lane, emit);
-            else wave_rowblock_t<false>(o, it.y, wt, lane, emit);
+            if (o.sk == 1) crt_wave_rowblock<true>(o, it.y, L.probs[it.x >> 1].beta, c, wt, lane);
+            else crt_wave_rowblock<false>(o, it.y, L.probs[it.x >> 1].beta, c, wt, lane);
         }
     }
     bool ok = grid_barrier(ctl, G, &flag_lds);
@@ -676,7 +926,7 @@ void k_grp_crt_persistent(CrtPersistLaunch L, CrtConsts c) {
                 const int64_t col0 = 4 * (int64_t)(idx - row4 * nc4);
                 const int rest = pr.M - 4 * (int)row4;
                 crt_rebuild_cols<WIDE>(pr.R + (int64_t)row4 * pr.Np, pr.RbA * (kBlk / 4) * pr.Np, col0, pr.N,
-                                       rest < 4 ? rest : 4, c, pr.Er + 4 * (int64_t)row4, pr.Ec, pr.beta, L.rounding,
+                                       rest < 4 ? rest : 4, c, pr.Er + 4 * (int64_t)row4, pr.Ec, L.rounding,
                                        pr.C + 4 * (int64_t)row4 * pr.ldc, pr.ldc);
             }
         }

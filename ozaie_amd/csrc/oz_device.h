@@ -1006,13 +1006,10 @@ __device__ __forceinline__ void wave_emit_chunk(const double *wt, int row, int c
     emit_digits16(mx, bias, negm, b, d, dst, plane);
 }
 
-// One row block: exponent maxima of its 32 rows over the whole contraction axis (into
-// o.E), then every 64-index tile of the contraction axis staged through the wave's LDS
-// tile, `emit(wt, row, c0, emax, dst)` writing the lane's 16-byte chunk of each slice
-// block (dst: its chunk in plane 0 of block (rb, kb)).  Kb may be odd (the last tile then
-// holds one block).
-template <bool KC, typename Emit>
-__device__ __forceinline__ void wave_rowblock_t(const GroupOperand &o, int64_t rb, double *wt, int lane, Emit emit) {
+// Exponent maxima of the 32 rows of row block rb over the whole contraction axis: every
+// lane returns the maximum of its row l % 32 (0 for rows at or beyond R).
+template <bool KC>
+__device__ __forceinline__ int wave_rowmax(const GroupOperand &o, int64_t rb, double *wt, int lane) {
     const int64_t r0 = rb * 32;
     // 1. exponent maxima (64 loads in flight per lane); lane l < 32 ends with row l's
     int emax;
@@ -1074,17 +1071,23 @@ __device__ __forceinline__ void wave_rowblock_t(const GroupOperand &o, int64_t r
         emax = max(m, __shfl_xor(m, 32, 64));
     }
     if (r0 + (lane & 31) >= o.R) emax = 0;
-    if (lane < 32) o.E[r0 + lane] = emax;
-    emax = __shfl(emax, lane & 31, 64);
-    // 2. tiles of 64 contraction indices, two tiles in flight
+    return __shfl(emax, lane & 31, 64);
+}
+
+// Every 64-index tile of the contraction axis of row block rb staged through the wave's
+// LDS tile, `chunk(wt, row, c0, kb)` called for each 16-element chunk the lane owns (row
+// l % 32, elements wt[row][c0 .. c0 + 16), contraction block kb; c0 = 32 (kb % 2) +
+// 16 (l / 32)).  Kb may be odd (the last tile then holds one block).
+template <bool KC, typename Chunk>
+__device__ __forceinline__ void wave_tiles(const GroupOperand &o, int64_t rb, double *wt, int lane, Chunk chunk) {
+    const int64_t r0 = rb * 32;
     const int64_t nt = (o.Kb + 1) / 2;
     const int row = lane & 31, half = lane >> 5;
     auto emit_tile = [&](int64_t t) {
 #pragma unroll
         for (int blk = 0; blk < 2; blk++) {
             if (2 * t + blk >= o.Kb) break;   // wave-uniform
-            int8_t *dst = o.S + (rb * o.Kb + 2 * t + blk) * (int64_t)kBlkBytes + lane * 16;
-            emit(wt, row, 32 * blk + 16 * half, emax, dst);
+            chunk(wt, row, 32 * blk + 16 * half, 2 * t + blk);
         }
     };
     double v0[32], v1[32];
@@ -1100,6 +1103,18 @@ __device__ __forceinline__ void wave_rowblock_t(const GroupOperand &o, int64_t r
             emit_tile(t + 1);
         }
     }
+}
+
+// One row block: exponent maxima of its 32 rows (into o.E), then `emit(wt, row, c0,
+// emax, dst)` for each 16-element chunk the lane owns (dst: the lane's 16-byte chunk in
+// plane 0 of block (rb, kb)).
+template <bool KC, typename Emit>
+__device__ __forceinline__ void wave_rowblock_t(const GroupOperand &o, int64_t rb, double *wt, int lane, Emit emit) {
+    const int emax = wave_rowmax<KC>(o, rb, wt, lane);
+    if (lane < 32) o.E[rb * 32 + lane] = emax;
+    wave_tiles<KC>(o, rb, wt, lane, [&](const double *w_, int row, int c0, int64_t kb) {
+        emit(w_, row, c0, emax, o.S + (rb * o.Kb + kb) * (int64_t)kBlkBytes + lane * 16);
+    });
 }
 
 // The split scheme's row block: d slice planes of b-bit digits.

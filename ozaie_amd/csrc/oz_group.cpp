@@ -503,13 +503,13 @@ int build_persist_crt(ozaie_group g, const std::vector<int64_t> &Kp) {
     return OZAIE_OK;
 }
 
-// CRT mode: per problem the largest exact bit count for its K (or the requested one), a
-// tile shape per size class, residue planes padded to the tile, and the item tables of
-// the four phases.
+// CRT mode: per problem the norm rule (bits 0) or the requested bit count, a tile shape
+// per size class, residue planes padded to the tile, chunk statistics (per-phase launches
+// only), and the item tables of the phases.
 int group_build_crt(ozaie_group g, int force_cfg) {
     const int n = g->count, nm = g->d;
     std::vector<int64_t> Mp(n), Np(n), Kp(n);
-    int64_t nE = 0, s_bytes = 0, r_bytes = 0;
+    int64_t nE = 0, s_bytes = 0, r_bytes = 0, t_bytes = 0;
     g->beta.assign(n, 0);
     for (int p = 0; p < n; p++) {
         if (g->K[p] > 65536) {
@@ -517,8 +517,8 @@ int group_build_crt(ozaie_group g, int force_cfg) {
             return OZAIE_ERR_DIMS;
         }
         const int bmax = crt_beta(nm, g->K[p]);
-        const int beta = g->b ? g->b : bmax;
-        if (bmax < 1 || beta < 1 || beta > bmax) {
+        int beta = g->b;   // 0: the norm rule
+        if (bmax < 1 || beta < 0 || beta > bmax) {
             set_error_msg("CRT bits exceed the exact range of the moduli for a problem's K");
             return OZAIE_ERR_ARG;
         }
@@ -526,6 +526,7 @@ int group_build_crt(ozaie_group g, int force_cfg) {
             set_error_msg("grouped CRT problems must have ceil(M / 4) N < 2^31");
             return OZAIE_ERR_DIMS;
         }
+        if (beta == 0 && !crt_norm_ok(nm, g->K[p])) beta = bmax;   // as a CRT plan
         g->beta[p] = beta;
         const int c = pick_crt_cfg(nm, g->M[p], g->N[p], g->K[p], force_cfg);
         g->cfg[p] = c;
@@ -535,11 +536,13 @@ int group_build_crt(ozaie_group g, int force_cfg) {
         nE += Mp[p] + Np[p];
         s_bytes += (int64_t)nm * (Mp[p] + Np[p]) * Kp[p];
         r_bytes += (int64_t)nm * Mp[p] * Np[p];
+        if (!g->persist) t_bytes += 8 * (Mp[p] + Np[p]) * crt_chunks(Kp[p] / kBlk);
     }
-    crt_consts(nm, 0, g->cc);   // per-problem bit counts travel in the descriptors
+    crt_consts(nm, 0, g->cc);   // per-problem rules travel in the descriptors
     const int64_t e_bytes = round_up(nE * 4, 1024);
     s_bytes = round_up(s_bytes, 1024);
-    GCHK(hipMalloc(&g->ws, (size_t)(e_bytes + s_bytes + r_bytes)));
+    r_bytes = round_up(r_bytes, 1024);
+    GCHK(hipMalloc(&g->ws, (size_t)(e_bytes + s_bytes + r_bytes + t_bytes)));
     g->E_all = reinterpret_cast<int32_t *>(g->ws);
     g->n_E = nE;
     GCHK(hipHostMalloc(&g->h_ops, sizeof(GroupOperand) * 2 * n));
@@ -548,7 +551,7 @@ int group_build_crt(ozaie_group g, int force_cfg) {
     std::memset(g->h_cprobs, 0, sizeof(CrtGroupProb) * n);
     std::vector<int4> imax, isplit, irebuild;
     std::vector<int4> igemm[CCFG_COUNT];
-    int64_t eoff = 0, soff = e_bytes, roff = e_bytes + s_bytes;
+    int64_t eoff = 0, soff = e_bytes, roff = e_bytes + s_bytes, toff = e_bytes + s_bytes + r_bytes;
     for (int p = 0; p < n; p++) {
         GroupOperand &a = g->h_ops[2 * p], &bo = g->h_ops[2 * p + 1];
         a.R = g->M[p]; a.K = g->K[p]; a.Rp = Mp[p]; a.sr = g->K[p]; a.sk = 1;
@@ -559,19 +562,20 @@ int group_build_crt(ozaie_group g, int force_cfg) {
         bo.S = reinterpret_cast<int8_t *>(g->ws + soff); soff += (int64_t)nm * Np[p] * Kp[p];
         a.Rb = Mp[p] / kBlk; bo.Rb = Np[p] / kBlk;
         a.Kb = bo.Kb = Kp[p] / kBlk;
-        for (int64_t q = 0; q < (Mp[p] + 3) / 4; q++) imax.push_back(int4{2 * p, (int)q, 0, 0});
-        const int64_t kper = 256;
-        for (int64_t cx = 0; cx < (g->N[p] + 255) / 256; cx++)
-            for (int64_t ky = 0; ky < (g->K[p] + kper - 1) / kper; ky++)
-                imax.push_back(int4{2 * p + 1, (int)cx, (int)ky, (int)kper});
-        // residues of the row blocks that hold real rows only: the blocks padding a
-        // problem to its tile shape feed only outputs at or beyond M rows / N columns,
-        // which the modulus GEMM does not store and the rebuild does not read
+        if (!g->persist) {
+            a.st = reinterpret_cast<uint64_t *>(g->ws + toff); toff += 8 * Mp[p] * crt_chunks(a.Kb);
+            bo.st = reinterpret_cast<uint64_t *>(g->ws + toff); toff += 8 * Np[p] * crt_chunks(bo.Kb);
+        }
+        // statistics, shifts and residues of the row blocks that hold real rows only: the
+        // blocks padding a problem to its tile shape feed only outputs at or beyond M rows /
+        // N columns, which the modulus GEMM does not store and the rebuild does not read
         for (int op = 0; op < 2; op++) {
             const GroupOperand &o = g->h_ops[2 * p + op];
-            for (int64_t rb = 0; rb < (o.R + kBlk - 1) / kBlk; rb++)
+            for (int64_t rb = 0; rb < (o.R + kBlk - 1) / kBlk; rb++) {
+                imax.push_back(int4{2 * p + op, (int)rb, 0, 0});
                 for (int64_t kt = 0; kt < (o.Kb + 3) / 4; kt++)
                     isplit.push_back(int4{2 * p + op, (int)rb, (int)kt, 0});
+            }
         }
         CrtGroupProb &cp = g->h_cprobs[p];
         cp.SA = a.S; cp.SB = bo.S; cp.Er = a.E; cp.Ec = bo.E;
@@ -734,8 +738,8 @@ int group_enqueue(ozaie_group g, hipStream_t st) {
         return OZAIE_OK;
     }
     if (g->crt) {
-        GCHK(launch_fill_i32(g->E_all, g->n_E, 0, st));
-        GCHK(launch_group_maxexp(g->d_ops, g->d_items_max, g->n_max, st));
+        GCHK(launch_group_crt_stats(g->d_ops, g->d_items_split, g->n_split, st));
+        GCHK(launch_group_crt_scale(g->d_ops, g->d_cprobs, g->d_items_max, g->n_max, g->cc, st));
         GCHK(launch_group_crt_split(g->d_ops, g->d_cprobs, g->d_items_split, g->n_split, g->cc, st));
         for (int c = 0; c < CCFG_COUNT; c++)
             GCHK(launch_group_crt_gemm(g->d_cprobs, g->d_items_crt[c], g->n_crt[c], c, g->crt_mpw[c], g->cc, st));

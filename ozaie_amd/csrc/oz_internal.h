@@ -74,6 +74,7 @@ struct GroupOperand {
     int8_t *S;
     int64_t Rb, Kb;
     int d, pad;         // slice count of this operand's problem (split scheme)
+    uint64_t *st;       // CRT mode: chunk statistics [Rp][crt_chunks(Kb)]
 };
 
 // Tile configurations of the grouped fused GEMM (wave tile (32 WM) x (32 WN)).
@@ -163,7 +164,8 @@ constexpr int kCrtLimbs = 24;        // byte limbs of the 192-bit rebuild
 constexpr int kCrtRowPad = 256;   // the modulus GEMM's workgroup tile is 256 x 256
 constexpr int kCrtPF = 8;         // K pads to 32 kCrtPF (register ring depth of the per-wave modulus GEMM)
 struct CrtConsts {
-    int n, beta;
+    int n, beta;               // beta 0: the norm rule (per-row shifts from norm bounds)
+    uint64_t Mw[3];            // M = prod p, three words low first
     double p[kCrtMaxMod];      // modulus
     double inv[kCrtMaxMod];    // 1 / p
     double c31[kCrtMaxMod];    // 2^31 mod p
@@ -178,9 +180,20 @@ struct CrtConsts {
 // Modulus list, bits per operand for (n, K) (or -1), and the constants.
 int crt_modulus(int i);
 int crt_beta(int n, int64_t K);
+bool crt_norm_ok(int n, int64_t K);   // the norm rule's bound M 2^20 - K (2^20 + 2^40) > 0
 void crt_consts(int n, int beta, CrtConsts &c);
-// Residues S[j][rb][kb][1024] (int8, slice-block layout) of one operand.
-hipError_t launch_crt_split(const OperandView &op, const int32_t *E, const CrtConsts &c, int8_t *S, int64_t Rp,
+// Per-row shifts sigma (q = round(|x| 2^(1075 - sigma))) come from statistics of chunks of
+// kCrtChunk contraction indices: per chunk one word, (max exponent field << 40) | S_c with
+// S_c the sum of squared 16-bit ceilings of the chunk's magnitudes; [Rp][crt_chunks(Kb)].
+constexpr int kCrtChunk = 16;
+__host__ __device__ inline int64_t crt_chunks(int64_t Kb) { return (Kb + 3) / 4 * 8; }   // 8 per split tile of 128
+hipError_t launch_crt_stats(const OperandView &op, uint64_t *stats, int64_t Rp, int64_t Kp, hipStream_t st);
+// sigma[r] for r < Rp from the statistics: the norm rule for c.beta = 0 (contraction
+// length K), else sigma = 53 + E - c.beta.
+hipError_t launch_crt_scale(const uint64_t *stats, int64_t Rp, int64_t nch, int64_t K, const CrtConsts &c,
+                            int32_t *sigma, hipStream_t st);
+// Residues S[j][rb][kb][1024] (int8, slice-block layout) of one operand with shifts sigma.
+hipError_t launch_crt_split(const OperandView &op, const int32_t *sigma, const CrtConsts &c, int8_t *S, int64_t Rp,
                             int64_t Kp, hipStream_t st);
 // g_j = A_j B_j^T mod p_j for every modulus: R[j][Mp/4][Np] words of four rows, one signed
 // byte each (congruent to g_j, |byte| <= p_j / 2).
@@ -191,7 +204,7 @@ struct CrtGemm {
     int variant;   // 0: fragments shared through LDS, 1: per-wave loads (OZAIE_OPT_KERNEL 3)
 };
 hipError_t launch_crt_gemm(const CrtGemm &g, const CrtConsts &c, hipStream_t st);
-// C = rebuild(R) * 2^(E_r + E_c - 2044 - 2 beta), truncated (rounding 0) or nearest (1).
+// C = rebuild(R) * 2^(sigma_r + sigma_c - 2150), truncated (rounding 0) or nearest (1).
 hipError_t launch_crt_rebuild(const uint32_t *R, int64_t Mp, int64_t Np, int64_t M, int64_t N, const int32_t *Er,
                               const int32_t *Ec, double *C, int64_t ldc, int rounding, const CrtConsts &c,
                               hipStream_t st);
@@ -205,7 +218,7 @@ struct CrtGroupProb {
     uint32_t *R;
     double *C;
     int64_t RbA, RbB, Kb, Np, ldc;
-    int M, N, beta, pad;
+    int M, N, beta, pad;   // beta 0: the norm rule, else sigma = 53 + E - beta
 };
 // Tile shapes of the grouped modulus GEMM: 256 x 256, 128 x 128, 64 x 64 per workgroup.
 enum CrtCfg { CCFG_44 = 0, CCFG_22 = 1, CCFG_11 = 2, CCFG_COUNT = 3 };
@@ -215,14 +228,18 @@ int crt_group_cfg_wn(int cfg);   // column blocks of 64 per tile
 // is o / 2), {problem, tile row, tile col, 0} (GEMM: workgroup x runs item x for moduli
 // [mpw y, mpw y + mpw); problem -1 = empty slot) and {problem, first (row quad, four
 // columns) group, 0, 0} (rebuild, 256 groups per item).
+// Chunk statistics over the split items; shifts over items {operand, row block, 0, 0}.
+hipError_t launch_group_crt_stats(const GroupOperand *ops, const int4 *items, int n, hipStream_t st);
+hipError_t launch_group_crt_scale(const GroupOperand *ops, const CrtGroupProb *probs, const int4 *items, int n,
+                                  const CrtConsts &c, hipStream_t st);
 hipError_t launch_group_crt_split(const GroupOperand *ops, const CrtGroupProb *probs, const int4 *items, int n,
                                   const CrtConsts &c, hipStream_t st);
 hipError_t launch_group_crt_gemm(const CrtGroupProb *probs, const int4 *items, int n, int cfg, int mpw,
                                  const CrtConsts &c, hipStream_t st);
 hipError_t launch_group_crt_rebuild(const CrtGroupProb *probs, const int4 *items, int n, int rounding,
                                     const CrtConsts &c, hipStream_t st);
-// The whole CRT group as ONE persistent launch (one workgroup per CU): phase A exponent
-// maxima + residues per 32-row block of every operand (per-wave lists itemsA[offA[4w+v],
+// The whole CRT group as ONE persistent launch (one workgroup per CU): phase A shifts
+// (exponent maxima, norm bounds) + residues per 32-row block of every operand (per-wave lists itemsA[offA[4w+v],
 // offA[4w+v+1]), items {operand, row block, 0, 0}), grid barrier, phase B modulus-GEMM
 // tiles with every modulus (per-workgroup lists, items {problem, tile row, tile col,
 // CrtCfg}, consecutive items of one shape), grid barrier, phase C rebuild (per-workgroup

@@ -9,11 +9,15 @@ src/programming_examples/basic/matrix_multiplication/test.cpp:479-520, common.h:
 
   square 4096^3, phi = 1: b = 7 with 3, 7, 8 slices (truncating) and the CRT mode with 16
     moduli (nearest), a 64 x 64 block;
+  square 8192^3, phi = 1 (the headline workload): the CRT mode with 14 and 15 moduli and
+    b = 7, d = 9, 4 sampled rows x 64 sampled columns;
   8192 x 8192 x 32768, phi = 2: b = 7, d = 7 (two int32-safe contraction passes) and the
-    CRT mode with 17 moduli (192-bit rebuild), 4 sampled rows x 64 sampled columns;
+    CRT mode with 15 and 17 moduli (128- and 192-bit rebuilds), 4 sampled rows x 64 sampled
+    columns;
   512 problems with M, N, K log-uniform in [32, 2048], phi = 1: the grouped multi-launch
-    and persistent modes, every problem equal to its own single plan, and 8 sampled
-    problems equal to the oracle on sampled rows and columns.
+    and persistent modes (b = 7 with 3, 7, 8, 9 slices; the CRT mode with 14 and 16
+    moduli), every problem equal to its own single plan, and 8 sampled problems equal to
+    the oracle on sampled rows and columns.
 """
 import numpy as np
 import pytest
@@ -85,6 +89,58 @@ def test_square4096_crt16_block_vs_oracle(square4096):
 
 
 @pytest.fixture(scope="module")
+def square8192(gpu):
+    import torch
+    n = 8192
+    g = torch.Generator(device="cuda")
+    g.manual_seed(SEED + 8)
+    A = gen(torch, g, n, n, 1.0)
+    B = gen(torch, g, n, n, 1.0)
+    rng = np.random.default_rng(SEED + 8)
+    rows = np.sort(rng.choice(n, 4, replace=False))
+    cols = np.sort(rng.choice(n, 64, replace=False))
+    Ar = A[torch.from_numpy(rows).cuda()].cpu().numpy()
+    Bc = B[:, torch.from_numpy(cols).cuda()].cpu().numpy()
+    yield A, B, rows, cols, Ar, Bc
+    del A, B
+    torch.cuda.empty_cache()
+
+
+def _sampled(torch, C, rows, cols):
+    return C[torch.from_numpy(rows).cuda()][:, torch.from_numpy(cols).cuda()].cpu().numpy()
+
+
+@pytest.mark.parametrize("n", [14, 15])
+def test_square8192_crt_vs_oracle(square8192, n):
+    import torch
+    A, B, rows, cols, Ar, Bc = square8192
+    print(f"8192^3 phi=1 CRT {n} moduli", flush=True)
+    p = oz.Plan(8192, 8192, 8192, 0, n, rounding=oz.ROUND_NEAREST, crt=True)
+    assert p.info()["bits"] == 0                                  # the norm rule
+    C = p(A, B)
+    torch.cuda.synchronize()
+    got = _sampled(torch, C, rows, cols)
+    del C
+    p.close()
+    assert np.array_equal(bits(got), bits(oracle.crt_dgemm(Ar, Bc, n, oz.ROUND_NEAREST)))
+
+
+def test_square8192_split_d9_vs_oracle(square8192):
+    import torch
+    A, B, rows, cols, Ar, Bc = square8192
+    print("8192^3 phi=1 b=7 d=9", flush=True)
+    p = oz.Plan(8192, 8192, 8192, 7, 9)
+    C = p(A, B)
+    torch.cuda.synchronize()
+    got = _sampled(torch, C, rows, cols)
+    del C
+    p.close()
+    ref, ov = oracle.ozaki_dgemm(Ar, Bc, 7, 9)
+    assert ov == 0
+    assert np.array_equal(bits(got), bits(ref))
+
+
+@pytest.fixture(scope="module")
 def long_k(gpu):
     import torch
     M = N = 8192
@@ -120,18 +176,19 @@ def test_long_k_d7_two_passes_vs_oracle(long_k):
     assert np.array_equal(bits(got), bits(ref))
 
 
-def test_long_k_crt17_vs_oracle(long_k):
+@pytest.mark.parametrize("n", [15, 17])
+def test_long_k_crt_vs_oracle(long_k, n):
     import torch
     A, B, rows, cols, Ar, Bc = long_k
-    print("8192x8192x32768 phi=2 CRT 17 moduli", flush=True)
-    p = oz.Plan(8192, 8192, 32768, 0, 17, rounding=oz.ROUND_NEAREST, crt=True)
-    assert p.info()["bits"] == oracle.crt_beta(17, 32768)
+    print(f"8192x8192x32768 phi=2 CRT {n} moduli", flush=True)
+    p = oz.Plan(8192, 8192, 32768, 0, n, rounding=oz.ROUND_NEAREST, crt=True)
+    assert p.info()["bits"] == 0                                  # the norm rule
     C = p(A, B)
     torch.cuda.synchronize()
     got = C[torch.from_numpy(rows).cuda()][:, torch.from_numpy(cols).cuda()].cpu().numpy()
     del C
     p.close()
-    ref = oracle.crt_dgemm(Ar, Bc, 17, oz.ROUND_NEAREST)
+    ref = oracle.crt_dgemm(Ar, Bc, n, oz.ROUND_NEAREST)
     assert np.array_equal(bits(got), bits(ref))
 
 
@@ -206,13 +263,14 @@ def test_group512_modes_vs_single_plans_and_oracle(group512, d):
     _oracle_samples(shapes, As, Bs, Cs, d, False, SEED + d)
 
 
-def test_group512_crt16_vs_single_plans_and_oracle(group512):
+@pytest.mark.parametrize("nm", [14, 16])
+def test_group512_crt_vs_single_plans_and_oracle(group512, nm):
     import torch
     shapes, As, Bs = group512
-    ref = _single_plans(torch, shapes, As, Bs, 0, 16, True)
+    ref = _single_plans(torch, shapes, As, Bs, 0, nm, True)
     for persistent in (False, True):
-        print(f"512 problems CRT 16 moduli persistent={persistent}", flush=True)
-        grp = oz.Group(shapes, 0, 16, rounding=oz.ROUND_NEAREST, crt=True, persistent=persistent)
+        print(f"512 problems CRT {nm} moduli persistent={persistent}", flush=True)
+        grp = oz.Group(shapes, 0, nm, rounding=oz.ROUND_NEAREST, crt=True, persistent=persistent)
         Cs = [torch.full((m, n), float("nan"), dtype=torch.float64, device="cuda") for m, n, _ in shapes]
         for rep in range(2):
             grp(As, Bs, Cs)
@@ -223,4 +281,4 @@ def test_group512_crt16_vs_single_plans_and_oracle(group512):
         if persistent:
             assert grp.query()["launches"] == 1
         grp.close()
-    _oracle_samples(shapes, As, Bs, Cs, 16, True, SEED + 16)
+    _oracle_samples(shapes, As, Bs, Cs, nm, True, SEED + nm)

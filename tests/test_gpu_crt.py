@@ -1,10 +1,12 @@
 """CRT mode on the GPU (through the C ABI) against the oracle's CRT restatement.
 
-Bar: bit-exact residues of both operands, bit-exact residue products for every modulus,
-and bit-exact FP64 C (truncating and nearest-even), including transposed operands, fewer
-moduli, 17-20 moduli (products beyond 2^127, rebuilt in 192 bits), explicit bit counts,
-the reference-style entry point with host arrays, and the K range limits; at 1024^3 the
-mode's error against the exact product is below native DGEMM's."""
+Bar: bit-exact shifts and residues of both operands under the norm rule (the default) and
+the max rule (explicit bits), bit-exact residue products for every modulus, and
+bit-exact FP64 C (truncating and nearest-even), including transposed operands, fewer
+moduli (where the norm rule has no room, the max rule), 17-20 moduli (products beyond
+2^127, rebuilt in 192 bits), explicit bit counts, the reference-style entry point with
+host arrays, and the K range limits; at 1024^3 the mode's error against the exact
+product is below native DGEMM's."""
 import numpy as np
 import pytest
 
@@ -24,6 +26,18 @@ def bits(x):
     return np.ascontiguousarray(x).view(np.int64)
 
 
+def default_bits(n, K):
+    """What bits 0 selects: the norm rule (0), or for a few moduli the max rule's largest
+    bit count."""
+    return 0 if oracle.crt_norm_m(1, n, K) != -1000 else oracle.crt_beta(n, K)
+
+
+def ref_residues(X, n, axis, bits, K):
+    if bits:
+        return oracle.crt_split(X, oracle.max_exp(X, axis), bits, n, axis)
+    return oracle.crt_split_sigma(X, oracle.crt_sigma(X, n, axis, K=K), n, axis)
+
+
 def edge_matrix(rng, m, n, phi=2.0):
     X = gen_inputs(rng, m, n, phi)
     X[0, :6] = [0.0, -0.0, 1.0, 5e-324, 2.2250738585072014e-308, -2.2250738585072009e-308]
@@ -35,18 +49,17 @@ def edge_matrix(rng, m, n, phi=2.0):
     return X
 
 
+@pytest.mark.parametrize("rule", ["default", "max"])
 @pytest.mark.parametrize("M,N,K,n", [(37, 45, 100, 16), (70, 33, 300, 16), (9, 260, 64, 5), (130, 7, 129, 2),
-                                     (33, 40, 200, 17), (20, 21, 70, 20)])
-def test_crt_stages_bit_exact(gpu, M, N, K, n):
+                                     (33, 40, 200, 17), (20, 21, 70, 20), (40, 36, 2000, 16)])
+def test_crt_stages_bit_exact(gpu, M, N, K, n, rule):
     rng = np.random.default_rng(M * 7 + K)
     A = edge_matrix(rng, M, K)
     B = np.ascontiguousarray(edge_matrix(rng, N, K).T)
-    beta = oracle.crt_beta(n, K)
-    SA, SB, G = oz.debug_crt(dev(A), dev(B), n, 0)
-    Ea = oracle.max_exp(A, 1)
-    Eb = oracle.max_exp(B, 0)
-    SAo = oracle.crt_split(A, Ea, beta, n, 1)
-    SBo = oracle.crt_split(B, Eb, beta, n, 0)
+    bits = default_bits(n, K) if rule == "default" else oracle.crt_beta(n, K)
+    SA, SB, G = oz.debug_crt(dev(A), dev(B), n, 0 if rule == "default" else bits)
+    SAo = ref_residues(A, n, 1, bits, K)
+    SBo = ref_residues(B, n, 0, bits, K)
     assert np.array_equal(SA, SAo)
     assert np.array_equal(SB, SBo)
     assert np.array_equal(G, oracle.crt_products(SAo, SBo))
@@ -64,7 +77,7 @@ def test_crt_C_bit_exact(gpu, M, N, K, n, phi, rounding):
     A[0, 0] = 0.0
     p = oz.Plan(M, N, K, 0, n, rounding=rounding, crt=True)
     info = p.info()
-    assert info["kernel"] == "crt" and info["bits"] == oracle.crt_beta(n, K) and info["slices"] == n
+    assert info["kernel"] == "crt" and info["bits"] == default_bits(n, K) and info["slices"] == n
     C = p(dev(A), dev(B))
     torch.cuda.synchronize()
     ref = oracle.crt_dgemm(A, B, n, rounding)

@@ -2,8 +2,11 @@
 
 The mode has no reference counterpart, so every pin is mathematics checked with Python
 integers and exact rationals: the moduli and reconstruction constants, the bit-count
-bound, the integer scaling q(x), the residues, the residue products, the CRT identity,
-and end-to-end C against the exact integer product of the scaled operands."""
+bound of the max rule, the norm rule's bound on each row's Euclidean norm and its shift
+(largest by its own integer condition, and every row and column of the scaled operands
+within sqrt(M / 4)), the integer scaling q(x), the residues, the residue products, the
+CRT identity, and end-to-end C against the exact integer product of the scaled operands
+under both rules."""
 import math
 import random
 from fractions import Fraction
@@ -118,129 +121,158 @@ def test_rebuild_is_the_crt_inverse(n):
         assert oracle.crt_rebuild(g) == v
 
 
-def _exact_C(A, B, n, rounding):
+def _q_sigma_exact(x, sigma):
+    """sign(x) * round(|x| * 2^(1075 - sigma)), ties away from zero, exact rationals."""
+    if x == 0:
+        return 0
+    v = Fraction(abs(float(x))) * Fraction(2) ** (1075 - sigma) + Fraction(1, 2)
+    q = v.numerator // v.denominator
+    return -q if x < 0 else q
+
+
+def _sigmas(A, B, n, bits):
+    """Shifts of the rows of A and the columns of B: the max rule from the exponent fields
+    (sigma = 53 + E - beta), the norm rule from the oracle (pinned by the tests below)."""
+    if bits:
+        return ([53 + max(_field(v) for v in row) - bits for row in A],
+                [53 + max(_field(v) for v in col) - bits for col in B.T])
+    return [int(v) for v in oracle.crt_sigma(A, n, 1)], [int(v) for v in oracle.crt_sigma(B, n, 0)]
+
+
+def _exact_C(A, B, n, rounding, bits):
     """C from the exact integer product of the scaled operands, by Python integers."""
-    beta = oracle.crt_beta(n, A.shape[1])
-    Er = [max(_field(v) for v in row) for row in A]
-    Ec = [max(_field(v) for v in col) for col in B.T]
+    sr, sc = _sigmas(A, B, n, bits)
     C = np.zeros((A.shape[0], B.shape[1]))
     for r in range(A.shape[0]):
-        qa = [_q_exact(v, Er[r], beta) for v in A[r]]
+        qa = [_q_sigma_exact(v, sr[r]) for v in A[r]]
         for c in range(B.shape[1]):
-            qb = [_q_exact(v, Ec[c], beta) for v in B[:, c]]
+            qb = [_q_sigma_exact(v, sc[c]) for v in B[:, c]]
             Cp = sum(x * y for x, y in zip(qa, qb))
-            C[r, c] = oracle.crt_to_double(Cp, Er[r] + Ec[c] - 2044 - 2 * beta, rounding)
+            C[r, c] = oracle.crt_to_double(Cp, sr[r] + sc[c] - 2150, rounding)
     return C
 
 
+@pytest.mark.parametrize("rule", ["norm", "max"])
 @pytest.mark.parametrize("n,phi,shape", [(16, 1.0, (6, 50, 5)), (16, 4.0, (5, 129, 4)), (9, 0.5, (4, 64, 6)),
                                          (3, 1.0, (3, 8, 3)), (17, 2.0, (4, 70, 5)), (20, 1.0, (3, 40, 4))])
-def test_crt_dgemm_equals_exact_integer_product(n, phi, shape):
+def test_crt_dgemm_equals_exact_integer_product(n, phi, shape, rule):
     M_, K, N_ = shape
     rng = np.random.default_rng(int(phi * 10) + n)
     A = (rng.random((M_, K)) - 0.5) * np.exp(phi * rng.standard_normal((M_, K)))
     B = (rng.random((K, N_)) - 0.5) * np.exp(phi * rng.standard_normal((K, N_)))
+    bits = 0 if rule == "norm" else oracle.crt_beta(n, K)
     for rounding in (0, 1):
-        C = oracle.crt_dgemm(A, B, n, rounding)
-        ref = _exact_C(A, B, n, rounding)
+        C = oracle.crt_dgemm(A, B, n, rounding, bits)
+        ref = _exact_C(A, B, n, rounding, bits)
         assert np.array_equal(C.view(np.int64), ref.view(np.int64))
 
 
-def test_conversion_rounding_closed_form():
-    rnd = random.Random(5)
-    for _ in range(600):
-        v = rnd.randrange(-2 ** 126, 2 ** 126) if _ % 2 else rnd.randrange(-2 ** 190, 2 ** 190)
-        e2 = rnd.randrange(-1100, -900)
-        for rounding in (0, 1):
-            got = oracle.crt_to_double(v, e2, rounding)
-            exact = Fraction(v) * Fraction(2) ** e2
-            if rounding == 1:
-                assert got == float(exact)                       # Fraction -> float rounds to nearest-even
-            else:
-                a = abs(v)
-                sh = max(0, a.bit_length() - 53)
-                t = (a >> sh) << sh
-                assert got == float(Fraction(t if v >= 0 else -t) * Fraction(2) ** e2)
+def _norm_rows(rng):
+    """Rows with the cases the chunked bound must handle: wide exponent ranges, zeros,
+    subnormals, a lone huge element, a chunk boundary inside the row, a row of zeros."""
+    rows = []
+    for phi, K in ((0.1, 64), (1.0, 300), (4.0, 257), (20.0, 90), (1.0, 16), (1.0, 1), (2.0, 1000)):
+        rows.append((rng.random(K) - 0.5) * np.exp(phi * rng.standard_normal(K)))
+    r = (rng.random(200) - 0.5)
+    r[:30] = 5e-324 * rng.integers(-400, 400, size=30)
+    r[100] = 1.5e300
+    rows.append(r)
+    rows.append(5e-324 * rng.integers(-9, 9, size=70).astype(np.float64))
+    rows.append(np.zeros(40))
+    return rows
 
 
-def test_crt_error_bound_vs_exact_product():
-    """|C - A B| <= sum_k (|a| d_b + |b| d_a + d_a d_b) + one rounding, d = 2^(E - 1022 - beta) / 2:
-    each scaled operand is within half a grid step of the exact one.  (The rounding of each
-    scaled element itself is pinned by test_q_closed_form_and_bound.)"""
-    rng = np.random.default_rng(6)
-    A = (rng.random((4, 300)) - 0.5) * np.exp(rng.standard_normal((4, 300)))
-    B = (rng.random((300, 3)) - 0.5) * np.exp(rng.standard_normal((300, 3)))
-    n = 16
-    beta = oracle.crt_beta(n, 300)
-    C = oracle.crt_dgemm(A, B, n, 1)
-    for r in range(4):
-        Er = max(_field(v) for v in A[r])
-        da = Fraction(2) ** (Er - 1022 - beta) / 2
-        for c in range(3):
-            Ec = max(_field(v) for v in B[:, c])
-            db = Fraction(2) ** (Ec - 1022 - beta) / 2
-            exact = sum(Fraction(float(x)) * Fraction(float(y)) for x, y in zip(A[r], B[:, c]))
-            bound = sum(abs(Fraction(float(x))) * db + abs(Fraction(float(y))) * da + da * db
-                        for x, y in zip(A[r], B[:, c]))
-            bound += abs(exact) * Fraction(2) ** -52
-            assert abs(Fraction(float(C[r, c])) - exact) <= bound
-
-
-@pytest.mark.parametrize("n", [2, 9, 16, 17, 20])
-def test_rebuild_from_offset_symmetric_residues(n):
-    """The GPU stores each residue product as a signed byte r (r = g mod p, -128 <= r <= 127,
-    128 kept as -128 for p = 256) and rebuilds from u = r + 128: because sum_j W_j = 1 (mod M),
-    C' = sum u W - nearest(sum u f) M - 128 for every |C'| <= M / 4."""
-    ps = oracle.crt_moduli(n)
-    M = math.prod(ps)
-    W, f, M2 = oracle.crt_consts(n)
-    assert M2 == M and sum(W) % M == 1
-    rnd = random.Random(100 + n)
-    vals = [0, 1, -1, M // 4, -(M // 4), 128, -128] + [rnd.randrange(-(M // 4), M // 4 + 1) for _ in range(400)]
-    for v in vals:
-        u = []
-        for p in ps:
-            r = v % p
-            r = r - p if r > p // 2 else r            # symmetric, in [-(p-1)/2, (p-1)/2] or [-128, 128]
-            r = ((r & 0xFF) ^ 0x80) - 128             # stored as one byte, read back signed
-            assert (r - v) % p == 0
-            u.append(r + 128)
-        t = 0.0
-        for uj, fj in zip(u, f):
-            t += float(uj) * fj
-        k = round(t)
-        R = 2 ** (128 if n <= 16 else 192)
-        S = (sum(uj * wj for uj, wj in zip(u, W)) - k * M - 128) % R
-        assert (S - R if S >= R // 2 else S) == v
+def test_norm_bound_valid_and_tight():
+    """||x||^2 <= S 4^(E^ - 1038) (valid), and S U^2 <= ||x||^2 + 2 U sum|x| + 17 nch U^2 with
+    U = 2^(E^ - 1038) and nch the number of 16-index chunks (each 16-bit ceiling adds at
+    most 2 |x| U_c + U_c^2, U_c <= U, and each chunk's rescaling one unit U^2);
+    E^ = max over the nonzero elements of max(e, 1)."""
+    rng = np.random.default_rng(11)
+    for x in _norm_rows(rng):
+        Eh, S = oracle.crt_normbound(x[None, :], 1)
+        Eh, S = int(Eh[0]), int(S[0])
+        nz = [v for v in x if v != 0]
+        if not nz:
+            assert S == 0
+            continue
+        assert Eh == max(max(_field(v), 1) for v in nz)
+        U = Fraction(2) ** (Eh - 1038)
+        nrm = sum(Fraction(float(v)) ** 2 for v in x)
+        absum = sum(abs(Fraction(float(v))) for v in x)
+        nch = (len(x) + 15) // 16
+        assert nrm <= S * U * U
+        assert S * U * U <= nrm + 2 * U * absum + 17 * nch * U * U
 
 
 @pytest.mark.parametrize("n", [2, 5, 9, 16, 17, 20])
-def test_rebuild_byte_limbs_and_fixed_point_quotient(n):
-    """The GPU rebuild's arithmetic against the exact CRT inverse: u = r + 128 per modulus,
-    L_i = sum_j u_j w_ji over the byte limbs w_ji of W_j, the quotient
-    k = floor((sum_j u_j q_j + 2^15) / 2^16) with q_j = floor(2^16 fl(W_j / M)) (< 2^16), and
-    C' = (sum_i (L_i + k m_i) 256^i - 128) mod 2^(8 nl) with m_i the bytes of 2^(8 nl) - M:
-    nl = 16 limbs (128 bits) for n <= 16 moduli, 24 (192 bits) above."""
-    ps = oracle.crt_moduli(n)
-    W, f, M = oracle.crt_consts(n)
-    nl = 16 if n <= 16 else 24
-    wb = [[(w >> (8 * i)) & 0xFF for i in range(nl)] for w in W]
-    q = [int(math.floor(fj * 65536.0)) for fj in f]
-    assert all(0 <= qj < 65536 for qj in q)
-    mneg = [((2 ** (8 * nl) - M) >> (8 * i)) & 0xFF for i in range(nl)]
-    rnd = random.Random(200 + n)
-    vals = [0, 1, -1, M // 4, -(M // 4), 127, -128] + [rnd.randrange(-(M // 4), M // 4 + 1) for _ in range(400)]
-    for v in vals:
-        u = []
-        for p in ps:
-            r = v % p
-            r = r - p if r > p // 2 else r
-            u.append((((r & 0xFF) ^ 0x80) - 128) + 128)
-        t = sum(uj * qj for uj, qj in zip(u, q))
-        assert t < 2 ** 29
-        k = (t + 32768) >> 16
-        L = [sum(u[j] * wb[j][i] for j in range(n)) + k * mneg[i] for i in range(nl)]
-        assert max(L) < 2 ** 22
-        R = 2 ** (8 * nl)
-        S = (sum(Li << (8 * i) for i, Li in enumerate(L)) - 128) % R
-        assert (S - R if S >= R // 2 else S) == v
+def test_norm_m_is_the_largest_by_its_condition(n):
+    """m = the largest integer <= 46 with S (2^22 + 4) 4^m <= M 2^20 - K (2^20 + 2^40)."""
+    M = math.prod(oracle.crt_moduli(n))
+    rnd = random.Random(n)
+    for K in (1, 16, 300, 8192, 65536):
+        R = M * 2 ** 20 - K * (2 ** 20 + 2 ** 40)
+        if R <= 0:
+            assert oracle.crt_norm_m(1, n, K) == -1000
+            continue
+        for S in [1, 2, 3, 2 ** 32, 2 ** 36, 2 ** 48 - 1] + [rnd.randrange(1, 2 ** 48) for _ in range(40)]:
+            m = oracle.crt_norm_m(S, n, K)
+            fits = lambda mm: (S * (2 ** 22 + 4) * 4 ** mm <= R) if mm >= 0 else (S * (2 ** 22 + 4) <= R * 4 ** -mm)
+            assert m <= 46 and fits(m)
+            assert m == 46 or not fits(m + 1)
+
+
+@pytest.mark.parametrize("n,K", [(16, 8192), (16, 300), (9, 100), (17, 5000), (20, 64), (4, 20)])
+def test_norm_rule_bounds_every_row_and_column(n, K):
+    """With the norm rule's shifts, every row of the scaled integers has ||q||^2 <= M / 4 (so
+    |C'| <= ||q_a|| ||q_b|| <= M / 4 by Cauchy-Schwarz, the rebuild's range), every
+    |q| <= 2^62, and q is the closed form round(|x| 2^(1075 - sigma))."""
+    rng = np.random.default_rng(K + n)
+    M = math.prod(oracle.crt_moduli(n))
+    rows = [(rng.random(K) - 0.5) * np.exp(phi * rng.standard_normal(K)) for phi in (0.1, 1.0, 4.0)]
+    spike = np.full(K, 1e-3)
+    spike[K // 2] = 1e5                       # one element carries the norm
+    flat = np.full(K, -0.75)                  # every element at the row maximum
+    rows += [spike, flat]
+    X = np.array(rows)
+    sg = oracle.crt_sigma(X, n, 1, K=K)
+    for x, s in zip(X, sg):
+        q = [oracle.crt_q_sigma(v, int(s)) for v in x]
+        idx = rng.integers(0, K, size=min(K, 64))
+        for i in idx:
+            assert q[i] == _q_sigma_exact(x[i], int(s))
+        assert max(abs(v) for v in q) <= 2 ** 62
+        assert 4 * sum(v * v for v in q) <= M
+
+
+def test_few_moduli_fall_back_to_the_max_rule():
+    """Where M 2^20 <= K (2^20 + 2^40) the norm rule has no room and bits 0 means the max rule
+    with the largest exact bit count."""
+    rng = np.random.default_rng(8)
+    A = rng.random((3, 20)) - 0.5
+    B = rng.random((20, 4)) - 0.5
+    assert oracle.crt_norm_m(1, 3, 20) == -1000 and oracle.crt_beta(3, 20) >= 1
+    for rounding in (0, 1):
+        C0 = oracle.crt_dgemm(A, B, 3, rounding, 0)
+        Cm = oracle.crt_dgemm(A, B, 3, rounding, oracle.crt_beta(3, 20))
+        assert np.array_equal(C0.view(np.int64), Cm.view(np.int64))
+
+
+def test_norm_rule_gains_bits_on_wide_rows():
+    """On (U - 1/2) exp(N(0, 1)) rows of K = 8192 the norm rule's scale exceeds the max
+    rule's by log2(max |x| sqrt(K) / ||x||) - 1/2 to within the two rules' floors (about 4
+    bits here); a row whose norm one element carries gains log2 sqrt(K), and on a flat row
+    (every element at the maximum, the norm rule's worst case) it is at most one bit below."""
+    rng = np.random.default_rng(21)
+    K, n = 8192, 16
+    X = (rng.random((7, K)) - 0.5) * np.exp(rng.standard_normal((7, K)))
+    X[5] = -0.75
+    X[6] = 1e-9
+    X[6, 7] = 3.0
+    beta = oracle.crt_beta(n, K)
+    smax = np.array([53 + max(_field(v) for v in row) - beta for row in X])
+    gain = smax - oracle.crt_sigma(X, n, 1)                # bits the norm rule adds
+    pred = [math.log2(np.abs(row).max() * math.sqrt(K) / np.linalg.norm(row)) - 0.5 for row in X]
+    for g, p in zip(gain[:5], pred[:5]):
+        assert abs(g - p) <= 1.5 and g >= 3
+    assert gain[5] >= -1
+    assert gain[6] >= 6
