@@ -232,11 +232,34 @@ void lpt_assign(const std::vector<int64_t> &cost, const std::vector<int> &order,
     }
 }
 
+// The tile sequence (problems by decreasing cost, each problem's tiles in band order) split
+// into one part per XCD: runs of `run` neighbouring tiles of a problem go to the XCD with
+// the least work so far, so tiles that share rows of A or columns of B run on one XCD's
+// L2, and every XCD gets the same mix of large and small problems (cutting the sequence
+// into contiguous parts gave the last XCD all the smallest problems, whose cost the model
+// underestimates).  Returns the item indices of each part, in sequence order.
+std::vector<std::vector<int>> xcd_parts(const std::vector<int4> &it, const std::vector<int64_t> &cost, int nx,
+                                        int run) {
+    std::vector<std::vector<int>> parts(nx);
+    std::vector<int64_t> load(nx, 0);
+    for (size_t i = 0; i < it.size();) {
+        size_t e = i;
+        while (e < it.size() && e < i + (size_t)run && it[e].x == it[i].x) e++;
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        for (size_t u = i; u < e; u++) {
+            parts[x].push_back((int)u);
+            load[x] += cost[u];
+        }
+        i = e;
+    }
+    return parts;
+}
+
 // Item lists of the persistent kernel.  Phase A: one item per 32-row block of every
-// operand, costed by its contraction length, balanced over all waves (four per workgroup).  Phase B: the
-// tiles of every problem in band order, problems by decreasing tile cost; the tile
-// sequence is cut into 8 contiguous parts of equal cost, one per XCD (workgroup w runs on
-// XCD w mod 8), and each part is balanced over that XCD's workgroups.
+// operand, costed by its contraction length, balanced over all waves (four per workgroup).
+// Phase B: the tiles of every problem in band order, problems by decreasing tile cost,
+// dealt to the XCDs by xcd_parts (workgroup w runs on XCD w mod 8); each XCD's part is
+// balanced over that XCD's workgroups.
 int build_persist(ozaie_group g, const std::vector<int64_t> &Kp) {
     const int n = g->count;
     int dev = 0, ncu = 0;
@@ -284,22 +307,12 @@ int build_persist(ozaie_group g, const std::vector<int64_t> &Kp) {
         }
     }
     g->tiles = (int64_t)itB.size();
-    int64_t total = 0;
-    for (int64_t c : costB) total += c;
     const int nx = (G % 8 == 0) ? 8 : 1;
-    size_t pos = 0;
-    int64_t acc = 0;
+    const std::vector<std::vector<int>> parts = xcd_parts(itB, costB, nx, 8);
     for (int x = 0; x < nx; x++) {
-        std::vector<int> part;
-        const int64_t target = (x == nx - 1) ? INT64_MAX : total * (x + 1) / nx;
-        while (pos < itB.size() && acc < target) {
-            part.push_back((int)pos);
-            acc += costB[pos];
-            pos++;
-        }
         std::vector<int> wgs;
         for (int w = x; w < G; w += nx) wgs.push_back(w);
-        lpt_assign(costB, part, wgs, listsB);
+        lpt_assign(costB, parts[x], wgs, listsB);
     }
     // flatten: [itemsA][offA][itemsB][offB][ctl]
     std::vector<int4> flatA, flatB;
@@ -395,9 +408,9 @@ ListBlob flatten(const std::vector<std::vector<int>> &lists, const std::vector<i
 
 // Item lists of the persistent CRT group: phase A as for the split scheme (row blocks of
 // real rows, per wave, longest first), phase B the tiles of every problem (each with all
-// moduli) in band order, problems by decreasing contraction length, cut into 8 parts of
-// equal cost (one per XCD) and balanced over each XCD's workgroups, each workgroup's list
-// ordered by tile shape; phase C the rebuild items balanced over all workgroups.
+// moduli) in band order, problems by decreasing contraction length, dealt to the XCDs by
+// xcd_parts and balanced over each XCD's workgroups, each workgroup's list ordered by tile
+// shape; phase C the rebuild items balanced over all workgroups.
 int build_persist_crt(ozaie_group g, const std::vector<int64_t> &Kp) {
     const int n = g->count;
     int dev = 0, ncu = 0;
@@ -448,22 +461,12 @@ int build_persist_crt(ozaie_group g, const std::vector<int64_t> &Kp) {
     std::vector<std::vector<int>> listsA(4 * G), listsB(G), listsC(G);
     lpt_assign(costA, order_of(costA), slots, listsA);
     lpt_assign(costC, order_of(costC), wgs, listsC);
-    int64_t total = 0;
-    for (int64_t c : costB) total += c;
     const int nx = (G % 8 == 0) ? 8 : 1;
-    size_t pos = 0;
-    int64_t acc = 0;
+    const std::vector<std::vector<int>> parts = xcd_parts(itB, costB, nx, 8);
     for (int x = 0; x < nx; x++) {
-        std::vector<int> part;
-        const int64_t target = (x == nx - 1) ? INT64_MAX : total * (x + 1) / nx;
-        while (pos < itB.size() && acc < target) {
-            part.push_back((int)pos);
-            acc += costB[pos];
-            pos++;
-        }
         std::vector<int> xw;
         for (int w = x; w < G; w += nx) xw.push_back(w);
-        lpt_assign(costB, part, xw, listsB);
+        lpt_assign(costB, parts[x], xw, listsB);
     }
     for (auto &l : listsB)
         std::stable_sort(l.begin(), l.end(), [&](int a, int b) { return itB[a].w < itB[b].w; });
