@@ -255,8 +255,25 @@ std::vector<std::vector<int>> xcd_parts(const std::vector<int4> &it, const std::
     return parts;
 }
 
-// Item lists of the persistent kernel.  Phase A: one item per 32-row block of every
-// operand, costed by its contraction length, balanced over all waves (four per workgroup).
+// Duration of one persistent GEMM tile in ns: a + c per contraction block, fitted per
+// (slice count, tile shape) to per-tile timings of the 512-problem set (MI355X, rocm 7.2;
+// kernel-side s_memrealtime stamps).  The cost does not scale with tile area x slice
+// products: small tiles are latency-bound, and the epilogue is a fixed cost per tile.
+int64_t persist_tile_cost(int d, int cfg, int64_t Kp) {
+    struct Fit { int d, cfg, per_block, fixed; };
+    static const Fit fits[] = {{3, GCFG_D11, 643, 2810},  {3, GCFG_D12, 630, 6890},  {3, GCFG_D22, 821, 13640},
+                               {7, GCFG_D11, 1037, 5940}, {7, GCFG_D12, 1420, 14280}, {8, GCFG_D11, 1100, 7400},
+                               {9, GCFG_D11, 1172, 8860}};
+    const int64_t blocks = Kp / kBlk;
+    for (const Fit &f : fits)
+        if (f.d == d && f.cfg == cfg) return f.per_block * blocks + f.fixed;
+    const int64_t area = (int64_t)(group_cfg_tile_m(cfg) / 64) * (group_cfg_tile_n(cfg) / 64);
+    return (blocks + 8) * area * (d * (d + 1) / 2) * 40;   // shapes without a fit
+}
+
+// Item lists of the persistent kernel.  Phase A: one item per 32-row block (of real rows)
+// of every operand, costed by its contraction length, balanced over all waves (four per
+// workgroup).
 // Phase B: the tiles of every problem in band order, problems by decreasing tile cost,
 // dealt to the XCDs by xcd_parts (workgroup w runs on XCD w mod 8); each XCD's part is
 // balanced over that XCD's workgroups.
@@ -272,9 +289,12 @@ int build_persist(ozaie_group g, const std::vector<int64_t> &Kp) {
     for (int p = 0; p < n; p++)
         for (int op = 0; op < 2; op++) {
             const GroupOperand &o = g->h_ops[2 * p + op];
-            for (int64_t rb = 0; rb < o.Rb; rb++) {
+            // row blocks of real rows only: the padding rows' slices feed only outputs the
+            // tiles do not store
+            for (int64_t rb = 0; rb * kBlk < o.R; rb++) {
                 itA.push_back(int4{2 * p + op, (int)rb, 0, 0});
-                costA.push_back(o.K + 256);
+                // per index as A; the transposed gather of B costs ~90 indices more per item
+                costA.push_back(o.K + (op ? 110 : 40));
             }
         }
     std::vector<int> orderA(itA.size());
@@ -290,20 +310,19 @@ int build_persist(ozaie_group g, const std::vector<int64_t> &Kp) {
     std::vector<int> porder(n);
     for (int p = 0; p < n; p++) porder[p] = p;
     std::stable_sort(porder.begin(), porder.end(), [&](int a, int b) {
-        return Kp[a] * g->dp[a] * (g->dp[a] + 1) > Kp[b] * g->dp[b] * (g->dp[b] + 1);
+        return persist_tile_cost(g->dp[a], g->cfg[a], Kp[a]) > persist_tile_cost(g->dp[b], g->cfg[b], Kp[b]);
     });
     std::vector<int4> itB;
     std::vector<int64_t> costB;
     for (int p : porder) {
         const int c = g->cfg[p], dd = g->dp[p];
         const int tm_rows = group_cfg_tile_m(c), tn_cols = group_cfg_tile_n(c);
-        const int64_t area = (int64_t)(tm_rows / 64) * (tn_cols / 64);
         const int64_t ntm = (g->M[p] + tm_rows - 1) / tm_rows, ntn = (g->N[p] + tn_cols - 1) / tn_cols;
         for (int64_t t = 0; t < ntm * ntn; t++) {
             int tm, tn;
             band_map((int)t, (int)ntm, (int)ntn, tm, tn);
             itB.push_back(int4{p, tm, tn, persist_code(dd, c)});
-            costB.push_back((Kp[p] / kBlk + 4) * area * (dd * (dd + 1) / 2));
+            costB.push_back(persist_tile_cost(dd, c, Kp[p]));
         }
     }
     g->tiles = (int64_t)itB.size();
@@ -424,7 +443,8 @@ int build_persist_crt(ozaie_group g, const std::vector<int64_t> &Kp) {
             const GroupOperand &o = g->h_ops[2 * p + op];
             for (int64_t rb = 0; rb < (o.R + kBlk - 1) / kBlk; rb++) {
                 itA.push_back(int4{2 * p + op, (int)rb, 0, 0});
-                costA.push_back(o.K + 256);
+                // per index as A; the transposed gather of B costs ~90 indices more per item
+                costA.push_back(o.K + (op ? 110 : 40));
             }
         }
     std::vector<int> porder(n);
