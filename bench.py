@@ -19,7 +19,8 @@ The JSON line also carries:
                 duration (HIP events recorded by the library on the stream it launches on)
                 against the 5.0 POP/s dense int8 peak, and against the ceiling the matrix
                 cores of this card sustain with register-only operands (ozaie_probe_mfma:
-                zero, constant and toggling random operands);
+                zero, constant and toggling random operands) and beside the vendor's int8
+                GEMM (hipBLASLt through torch._int_mm) on the same card and size;
                 traffic = HBM bytes per launch from the rocprofv3 FETCH/WRITE summaries in
                 profiles/ when present
   cpu_baseline  the oracle's OpenMP restatement of the reference rule (b = 7, the matched
@@ -105,6 +106,26 @@ def time_steps(torch, fn, steps, warmup):
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / steps
+
+
+def vendor_int8_gemm(torch, n, steps=10):
+    """The vendor's int8 GEMM on the same card for comparison with the modulus GEMM: an
+    n x n x n product of random int8 operands through torch._int_mm (hipBLASLt, int32
+    result), A row-major and B column-major (its fastest layout here) and B row-major,
+    TOP/s over `steps` calls after 3 warm-ups.  Reported beside the roofline, never
+    required."""
+    try:
+        a = torch.randint(-128, 128, (n, n), dtype=torch.int8, device="cuda:0")
+        b = torch.randint(-128, 128, (n, n), dtype=torch.int8, device="cuda:0")
+        out = {}
+        for name, bb in (("b_col_major", b.t().contiguous().t()), ("b_row_major", b)):
+            ms = time_steps(torch, lambda: torch._int_mm(a, bb), steps, 3)
+            out[name + "_tops"] = 2.0 * n ** 3 / ms / 1e9
+        del a, b
+        torch.cuda.empty_cache()
+        return out
+    except Exception as e:  # reported beside value, never required
+        return {"error": repr(e)}
 
 
 def scaling_name(bits):
@@ -748,6 +769,8 @@ def main():
         except Exception as e:  # reported beside value, never required
             log(args, f"  mfma probe {pat}: {e!r}")
     log(args, f"  int8 MFMA ceiling (TOP/s): {ceiling}")
+    vendor = vendor_int8_gemm(torch, n)
+    log(args, f"  vendor int8 GEMM (hipBLASLt, torch._int_mm) at {n}^3: {vendor}")
     kname = (f"CRT modulus GEMM (k_crt_gemm_lds<4, 4>), {d} moduli" if crt else f"fused slice GEMM, d={d}")
     cfg = ({"scheme": "crt", "moduli": d, "bits": m["bits"], "scaling": m["scaling"], "rounding": "nearest"} if crt
            else {"scheme": "split", "bits": 7, "slices": d})
@@ -797,7 +820,8 @@ def main():
                      "measured_ceiling_random": ceiling.get("random"),
                      "measured_ceiling_constant": ceiling.get("constant"),
                      "measured_ceiling_zeros": ceiling.get("zeros"),
-                     "frac_of_measured_ceiling": (achieved / ceiling["random"]) if ceiling.get("random") else None},
+                     "frac_of_measured_ceiling": (achieved / ceiling["random"]) if ceiling.get("random") else None,
+                     "vendor_int8_gemm": vendor},
     }
     try:
         line["host_pointer"] = host_pointer(np, oz, Ah, Bh, n, crt, d)

@@ -228,6 +228,14 @@ int32_t ozaie_tune_set_crt(int moduli, const int32_t *table9);
 int32_t ozaie_debug_crt(const double *A, const double *B, int64_t M, int64_t N, int64_t K, int moduli,
                         int bits, int8_t *SA_out, int8_t *SB_out, uint8_t *G_out);
 
+/* The epilogue's conversion of exact integers T (= T_hi 2^64 + T_lo, two's complement) to
+ * FP64, C = T 2^(es - 2044 - bits (slices + 1)), as computed on the GPU: out[i] through the
+ * 128-bit path, out64[i] through the 64-bit path when T fits int64 (else 0).  rounding:
+ * OZAIE_ROUND_TRUNCATE (the reference's host conversion, include/oz_aie.h:92-150) or
+ * OZAIE_ROUND_NEAREST.  Host arrays of n entries.                                       */
+int32_t ozaie_debug_convert(const int64_t *T_hi, const uint64_t *T_lo, const int32_t *es, int64_t n, int bits,
+                            int slices, int rounding, double *out, double *out64);
+
 /* Exponents and slices of one operand, as computed on the GPU, in plain layout.
  * X is a device pointer to a rows x K view: element (r, k) at X[r*sr + k*sk].
  * E_out: int32[rows] (host), S_out: int8[d][rows][K] (host).                          */

@@ -60,6 +60,7 @@ EXPORTS = [
     "ozAIE_setup", "ozAIE_setup_slices", "ozAIE_DGEMM", "ozaie_set_option", "ozaie_get_option",
     "ozaie_plan_info", "ozaie_release", "ozaie_plan_create", "ozaie_plan_destroy",
     "ozaie_plan_query", "ozaie_dgemm", "ozaie_debug_split", "ozaie_debug_bins", "ozaie_debug_crt",
+    "ozaie_debug_convert",
     "ozaie_paired_output", "ozaie_last_error", "ozaie_version", "ozaie_plan_phase_times",
     "ozaie_exact_entries", "ozaie_group_create", "ozaie_group_run", "ozaie_group_query",
     "ozaie_group_configs", "ozaie_group_status", "ozaie_group_destroy", "ozaie_dgemm_grouped", "ozaie_autotune",
@@ -130,6 +131,8 @@ def _declare(L):
     L.ozaie_debug_split.restype = i32
     L.ozaie_debug_crt.argtypes = [P, P, i64, i64, i64, c_int, c_int, P, P, P]
     L.ozaie_debug_crt.restype = i32
+    L.ozaie_debug_convert.argtypes = [P, P, P, i64, c_int, c_int, c_int, P, P]
+    L.ozaie_debug_convert.restype = i32
     L.ozaie_debug_bins.argtypes = [i64, i64, i64, P, P, c_int, c_int, c_int, P]
     L.ozaie_debug_bins.restype = i32
     L.ozaie_paired_output.argtypes = [i64, i64, i64, P, P, c_int, c_int, P]
@@ -375,6 +378,20 @@ def debug_split(X, bits, slices, contract_axis=1):
     _check(lib().ozaie_debug_split(_ptr(X), rows, K, sr, sk, bits, slices, _ptr(E), _ptr(S)),
            "ozaie_debug_split")
     return E, S
+
+
+def debug_convert(T_hi, T_lo, es, bits, slices, rounding=ROUND_TRUNCATE):
+    """The GPU epilogue's FP64 of exact integers T = T_hi 2^64 + T_lo (int64 / uint64 host
+    arrays, two's complement) scaled by 2^(es - 2044 - bits (slices + 1)): (through the
+    128-bit path, through the 64-bit path where T fits int64, else 0)."""
+    hi = np.ascontiguousarray(T_hi, dtype=np.int64)
+    lo = np.ascontiguousarray(T_lo, dtype=np.uint64)
+    e = np.ascontiguousarray(es, dtype=np.int32)
+    out = np.zeros(hi.shape[0], dtype=np.float64)
+    out64 = np.zeros(hi.shape[0], dtype=np.float64)
+    _check(lib().ozaie_debug_convert(_ptr(hi), _ptr(lo), _ptr(e), hi.shape[0], bits, slices, rounding,
+                                     _ptr(out), _ptr(out64)), "ozaie_debug_convert")
+    return out, out64
 
 
 def debug_crt(A, B, moduli=16, bits=0):

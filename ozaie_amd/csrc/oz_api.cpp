@@ -745,6 +745,33 @@ int32_t ozaie_plan_phase_times(ozaie_plan plan, float *out3) {
     return OZAIE_OK;
 }
 
+int32_t ozaie_debug_convert(const int64_t *T_hi, const uint64_t *T_lo, const int32_t *es, int64_t n, int bits,
+                            int slices, int rounding, double *out, double *out64) {
+    if (!T_hi || !T_lo || !es || !out || !out64 || n < 1) return OZAIE_ERR_ARG;
+    if (bits < 1 || bits > kMaxBits || slices < 1 || slices > kMaxSlices || (rounding != 0 && rounding != 1))
+        return OZAIE_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc = ensure_device();
+    if (rc) return rc;
+    char *buf = nullptr;
+    const size_t w = sizeof(int64_t) * (size_t)n;
+    HIPCHK(hipMalloc(&buf, 5 * w));
+    int64_t *dh = reinterpret_cast<int64_t *>(buf);
+    uint64_t *dl = reinterpret_cast<uint64_t *>(buf + w);
+    int32_t *de = reinterpret_cast<int32_t *>(buf + 2 * w);
+    double *o1 = reinterpret_cast<double *>(buf + 3 * w), *o2 = reinterpret_cast<double *>(buf + 4 * w);
+    hipError_t e = hipMemcpy(dh, T_hi, w, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dl, T_lo, w, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(de, es, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_debug_convert(dh, dl, de, n, bits, slices, rounding, o1, o2, 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out, o1, w, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(out64, o2, w, hipMemcpyDeviceToHost);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return set_hip_error(e, "ozaie_debug_convert");
+    return OZAIE_OK;
+}
+
 int32_t ozaie_debug_split(const double *X, int64_t rows, int64_t K, int64_t sr, int64_t sk,
                           int bits, int slices, int32_t *E_out, int8_t *S_out) {
     if (!X || !E_out || !S_out) return OZAIE_ERR_ARG;

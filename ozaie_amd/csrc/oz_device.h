@@ -311,6 +311,64 @@ __device__ __forceinline__ int clz128(u128 a) {
     return 128;
 }
 
+// |T| > 0 normalised into a 64-bit head: m = the top 64 bits of |T| with its leading one at
+// bit 63, nb = the bit length of |T|, sticky = whether bits of |T| lie below the head.
+struct Head {
+    uint64_t m;
+    int nb;
+    bool sticky;
+};
+
+__device__ __forceinline__ Head head_of(uint64_t hi, uint64_t lo) {
+    Head r;
+    if (hi) {
+        const int n = __builtin_clzll(hi);
+        r.m = n ? (hi << n) | (lo >> (64 - n)) : hi;
+        r.nb = 128 - n;
+        r.sticky = (n ? (lo << n) : lo) != 0;
+    } else {
+        const int n = __builtin_clzll(lo);
+        r.m = lo << n;
+        r.nb = 64 - n;
+        r.sticky = false;
+    }
+    return r;
+}
+
+// FP64 of sign * |T| * 2^(es - 2044 - b(d+1)) from the head of |T| (sgn: the sign bit).
+// rounding 0: trunc53(|T|) with the exponent written into the bits as in
+// include/oz_aie.h:124-146 (no over/underflow handling, as in the reference); rounding 1:
+// |T| rounded to 53 bits, nearest-even, then scaled by ldexp.
+__device__ __forceinline__ double head_to_double(Head h, uint64_t sgn, int es, int b, int d, int rounding) {
+    if (rounding == 0) {
+        const uint64_t mant = (h.m >> 11) - 0x0010000000000000ull;   // top 53 bits, implicit one removed
+        const int64_t ex = h.nb - b * (d + 1) - 1;
+        return __longlong_as_double((long long)(mant + sgn + ((uint64_t)(ex + 1023) << 52) +
+                                                ((uint64_t)(int64_t)(es - 2044) << 52)));
+    }
+    uint64_t q;
+    int e2;
+    if (h.nb <= 53) {
+        q = h.m >> (64 - h.nb);   // exact
+        e2 = 0;
+    } else {
+        q = h.m >> 11;
+        const bool half = (h.m >> 10) & 1, rest = (h.m & 0x3FF) != 0 || h.sticky;
+        if (half && (rest || (q & 1))) q += 1;   // 2^53 after the carry is still exact
+        e2 = h.nb - 53;
+    }
+    const double v = ldexp((double)q, e2 + es - 2044 - b * (d + 1));
+    return sgn ? -v : v;
+}
+
+// The same for an exact T held in int64.
+__device__ __forceinline__ double t64_to_double(int64_t t, int es, int b, int d, int rounding) {
+    if (t == 0) return 0.0;
+    const uint64_t sgn = t < 0 ? (1ull << 63) : 0ull;
+    const uint64_t a = t < 0 ? (uint64_t)0 - (uint64_t)t : (uint64_t)t;
+    return head_to_double(head_of(0, a), sgn, es, b, d, rounding);
+}
+
 // Exact integer T (scaled by 2^(es - 2044 - b(d+1))) to FP64.
 // rounding 0: sign * trunc53(|T|) via the bit arithmetic of include/oz_aie.h:124-146
 // (no over/underflow handling, as in the reference); rounding 1: nearest-even.
@@ -318,31 +376,7 @@ __device__ __forceinline__ double t_to_double(i128 t, int es, int b, int d, int 
     if (t == 0) return 0.0;
     const uint64_t sgn = t < 0 ? (1ull << 63) : 0ull;
     const u128 a = t < 0 ? (u128)(-t) : (u128)t;
-    const int nb = 128 - clz128(a);
-    if (rounding == 0) {
-        const int shift = nb - 53;
-        uint64_t lu = shift > 0 ? (uint64_t)(a >> shift) : (uint64_t)(a << (-shift));
-        const int64_t ex = nb - b * (d + 1) - 1;
-        lu -= 0x0010000000000000ull;
-        lu += sgn + ((uint64_t)(ex + 1023) << 52) + ((uint64_t)(int64_t)(es - 2044) << 52);
-        return __longlong_as_double((long long)lu);
-    }
-    uint64_t m;
-    int e2;
-    if (nb <= 53) {
-        m = (uint64_t)a;
-        e2 = 0;
-    } else {
-        const int sh = nb - 53;
-        u128 q = a >> sh;
-        const u128 rem = a - (q << sh);
-        const u128 half = (u128)1 << (sh - 1);
-        if (rem > half || (rem == half && (q & 1))) q += 1;
-        m = (uint64_t)q;
-        e2 = sh;
-    }
-    const double v = ldexp((double)m, e2 + es - 2044 - b * (d + 1));
-    return sgn ? -v : v;
+    return head_to_double(head_of((uint64_t)(a >> 64), (uint64_t)a), sgn, es, b, d, rounding);
 }
 
 // Store one element: bins k0 .. k0+NB-1 of this pass (values vals[kk]); es is the sum of
@@ -398,6 +432,16 @@ template <int D, int B>
 __device__ __forceinline__ void epilogue_elem_c(const GemmLaunch &g, int row, int col, const int32_t (&vals)[D],
                                                 int es) {
     if (row >= g.M || col >= g.N) return;
+    if constexpr (D <= 31 / B + 1) {
+        // one 64-bit chunk holds every bin (|T| < 2^62): the whole product in int64
+        if (g.first && g.last) {
+            int64_t c = 0;
+#pragma unroll
+            for (int i = 0; i < D; i++) c += (int64_t)vals[D - 1 - i] << (B * i);
+            g.C[(int64_t)row * g.ldc + col] = t64_to_double(c, es, B, D, g.rounding);
+            return;
+        }
+    }
     i128 T = bins_to_T<D, B>(vals);
     i128 *tp = g.T + (int64_t)row * g.ldT + col;
     if (!g.first) T += *tp;

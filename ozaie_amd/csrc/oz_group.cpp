@@ -261,14 +261,31 @@ std::vector<std::vector<int>> xcd_parts(const std::vector<int4> &it, const std::
 // products: small tiles are latency-bound, and the epilogue is a fixed cost per tile.
 int64_t persist_tile_cost(int d, int cfg, int64_t Kp) {
     struct Fit { int d, cfg, per_block, fixed; };
-    static const Fit fits[] = {{3, GCFG_D11, 643, 2810},  {3, GCFG_D12, 630, 6890},  {3, GCFG_D22, 821, 13640},
-                               {7, GCFG_D11, 1037, 5940}, {7, GCFG_D12, 1420, 14280}, {8, GCFG_D11, 1100, 7400},
-                               {9, GCFG_D11, 1172, 8860}};
+    static const Fit fits[] = {{3, GCFG_D11, 651, 2380},  {3, GCFG_D12, 623, 6550},  {3, GCFG_D22, 826, 12440},
+                               {7, GCFG_D11, 1027, 4870}, {7, GCFG_D12, 1338, 12760}, {8, GCFG_D11, 1090, 6600},
+                               {9, GCFG_D11, 1154, 8430}};
     const int64_t blocks = Kp / kBlk;
     for (const Fit &f : fits)
         if (f.d == d && f.cfg == cfg) return f.per_block * blocks + f.fixed;
     const int64_t area = (int64_t)(group_cfg_tile_m(cfg) / 64) * (group_cfg_tile_n(cfg) / 64);
     return (blocks + 8) * area * (d * (d + 1) / 2) * 40;   // shapes without a fit
+}
+
+// Tile shape of a problem in the persistent launch: the shape the kernel has for d with
+// the least fitted cost over the problem's tiles (padding included).  Larger tiles cost
+// less per output (d = 3: 128 x 128 tiles 3x less per element than 64 x 64), so this
+// takes them wherever the padding allows.
+int persist_cfg_pick(int d, int64_t M, int64_t N, int64_t Kp) {
+    int best = GCFG_D11;
+    int64_t best_cost = INT64_MAX;
+    for (int c : {GCFG_D11, GCFG_D12, GCFG_D22}) {
+        if (!persist_cfg_supported(d, c)) continue;
+        const int64_t tiles = ((M + group_cfg_tile_m(c) - 1) / group_cfg_tile_m(c)) *
+                              ((N + group_cfg_tile_n(c) - 1) / group_cfg_tile_n(c));
+        const int64_t cost = tiles * persist_tile_cost(d, c, Kp);
+        if (cost < best_cost) { best = c; best_cost = cost; }
+    }
+    return best;
 }
 
 // Item lists of the persistent kernel.  Phase A: one item per 32-row block (of real rows)
@@ -293,8 +310,9 @@ int build_persist(ozaie_group g, const std::vector<int64_t> &Kp) {
             // tiles do not store
             for (int64_t rb = 0; rb * kBlk < o.R; rb++) {
                 itA.push_back(int4{2 * p + op, (int)rb, 0, 0});
-                // per index as A; the transposed gather of B costs ~90 indices more per item
-                costA.push_back(o.K + (op ? 110 : 40));
+                // fitted per-item timings: the same cost per index on both sides, plus a
+                // fixed part growing with d (the transposed gather of B costs ~50 more)
+                costA.push_back(o.K + (op ? 40 + 7 * g->dp[p] : 5 * g->dp[p]));
             }
         }
     std::vector<int> orderA(itA.size());
@@ -443,8 +461,9 @@ int build_persist_crt(ozaie_group g, const std::vector<int64_t> &Kp) {
             const GroupOperand &o = g->h_ops[2 * p + op];
             for (int64_t rb = 0; rb < (o.R + kBlk - 1) / kBlk; rb++) {
                 itA.push_back(int4{2 * p + op, (int)rb, 0, 0});
-                // per index as A; the transposed gather of B costs ~90 indices more per item
-                costA.push_back(o.K + (op ? 110 : 40));
+                // fitted per-item timings: the same cost per index on both sides, plus a
+                // fixed part growing with d (the transposed gather of B costs ~50 more)
+                costA.push_back(o.K + (op ? 40 + 7 * g->dp[p] : 5 * g->dp[p]));
             }
         }
     std::vector<int> porder(n);
@@ -674,7 +693,9 @@ int group_build(ozaie_group g, int force_cfg) {
         nE += Mp[p] + Np[p];
         slice_bytes += (int64_t)dd * (Mp[p] + Np[p]) * Kp[p];
         const int c = pick_cfg(dd, g->M[p], g->N[p], g->K[p], force_cfg);
-        g->cfg[p] = !g->persist ? c : (pforce >= 0 && persist_cfg_supported(dd, pforce) ? pforce : persist_cfg_for(dd, c));
+        g->cfg[p] = !g->persist ? c
+                    : (pforce >= 0 && persist_cfg_supported(dd, pforce) ? pforce
+                       : (force_cfg >= 0 ? persist_cfg_for(dd, c) : persist_cfg_pick(dd, g->M[p], g->N[p], Kp[p])));
     }
     const int64_t e_bytes = round_up(nE * 4, 1024);
     GCHK(hipMalloc(&g->ws, (size_t)(e_bytes + slice_bytes)));

@@ -267,6 +267,8 @@ int64_t max_kchunk_blocks(int b, int d);
 // Kernel launchers (oz_kernels.hip).  All enqueue on `st` and return hipError_t.
 hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
 hipError_t launch_maxexp(const OperandView &op, int32_t *E, int64_t Rp, hipStream_t st);
+hipError_t launch_debug_convert(const int64_t *hi, const uint64_t *lo, const int32_t *es, int64_t n, int b, int d,
+                                int rounding, double *out128, double *out64, hipStream_t st);
 hipError_t launch_split(const OperandView &op, const int32_t *E, int b, int d, int8_t *S,
                         int64_t Rp, int64_t Kp, hipStream_t st);
 // Fused kernel (all d bins in registers); returns hipErrorInvalidValue if d has no

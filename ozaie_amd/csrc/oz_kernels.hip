@@ -78,6 +78,29 @@ __global__ __launch_bounds__(256) void k_split(const double *__restrict__ X, int
     split_tile<KC>(X, sr, sk, R, K, E + (int64_t)blockIdx.y * 32, b, d, S, Rb, Kb, blockIdx.y, blockIdx.x, tile);
 }
 
+// Conversion check: the exact integers T[i] = hi[i] 2^64 + lo[i] (two's complement) to FP64
+// with the epilogue's conversions: out128[i] through t_to_double, out64[i] through
+// t64_to_double when T fits int64 (else 0).
+__global__ __launch_bounds__(256) void k_debug_convert(const int64_t *__restrict__ hi, const uint64_t *__restrict__ lo,
+                                                       const int32_t *__restrict__ es, int64_t n, int b, int d,
+                                                       int rounding, double *out128, double *out64) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const i128 t = (i128)(((u128)(uint64_t)hi[i] << 64) | lo[i]);
+    out128[i] = t_to_double(t, es[i], b, d, rounding);
+    const bool fits = hi[i] == ((int64_t)lo[i] >> 63);
+    out64[i] = fits ? t64_to_double((int64_t)lo[i], es[i], b, d, rounding) : 0.0;
+}
+
+hipError_t launch_debug_convert(const int64_t *hi, const uint64_t *lo, const int32_t *es, int64_t n, int b, int d,
+                                int rounding, double *out128, double *out64, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if ((n + 255) / 256 > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_debug_convert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, hi, lo, es, n, b, d,
+                       rounding, out128, out64);
+    return hipGetLastError();
+}
+
 hipError_t launch_split(const OperandView &op, const int32_t *E, int b, int d, int8_t *S,
                         int64_t Rp, int64_t Kp, hipStream_t st) {
     const int64_t Rb = Rp / kBlk, Kb = Kp / kBlk;
