@@ -1081,9 +1081,18 @@ __device__ __forceinline__ int wave_rowmax(const GroupOperand &o, int64_t rb, do
 #pragma unroll
                     for (int u = 0; u < 8; u++) m[gi * 8 + i] = max(m[gi * 8 + i], field_of(a[i][u]));
             }
-            for (; k < o.K; k += 64) {
+            // the rest (fewer than 512 indices) as one more round of 64 loads in flight, the
+            // indices clamped to K - 1 (a repeated element leaves the maximum unchanged)
+            {
+                double a[8][8];
 #pragma unroll
-                for (int i = 0; i < 8; i++) m[gi * 8 + i] = max(m[gi * 8 + i], field_of(rowp[i][k]));
+                for (int i = 0; i < 8; i++)
+#pragma unroll
+                    for (int u = 0; u < 8; u++) a[i][u] = rowp[i][min(k + 64 * u, o.K - 1)];
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+#pragma unroll
+                    for (int u = 0; u < 8; u++) m[gi * 8 + i] = max(m[gi * 8 + i], field_of(a[i][u]));
             }
         }
         // transpose-reduce through LDS: [32 rows][64 lanes]
@@ -1111,7 +1120,15 @@ __device__ __forceinline__ int wave_rowmax(const GroupOperand &o, int64_t rb, do
 #pragma unroll
             for (int u = 0; u < 64; u++) m = max(m, field_of(a[u]));
         }
-        for (; k < o.K; k += 2) m = max(m, field_of(col[k * o.sk]));
+        {
+            // the rest (fewer than 128 indices) with all its loads in flight, indices clamped
+            // to K - 1 (a repeated element leaves the maximum unchanged)
+            double a[64];
+#pragma unroll
+            for (int u = 0; u < 64; u++) a[u] = col[min(k + 2 * u, o.K - 1) * o.sk];
+#pragma unroll
+            for (int u = 0; u < 64; u++) m = max(m, field_of(a[u]));
+        }
         emax = max(m, __shfl_xor(m, 32, 64));
     }
     if (r0 + (lane & 31) >= o.R) emax = 0;
