@@ -1,8 +1,9 @@
 """Text summaries of rocprofv3 output (the files kept under profiles/).
 
-    python -m ozaie_amd.profsum kernels DIR [--top N]
+    python -m ozaie_amd.profsum kernels DIR [--top N] [--by-grid]
         per-kernel calls / average / min / max / share from the kernel trace
-        (rocprofv3 --kernel-trace --output-format csv -d DIR)
+        (rocprofv3 --kernel-trace --output-format csv -d DIR); --by-grid: one line per
+        kernel and grid size (threads in x and y), e.g. the CRT GEMM per modulus count
     python -m ozaie_amd.profsum counters DIR [--simds 1024 --xcds 8]
         per-kernel averages of one --pmc run with SQ_VALU_MFMA_BUSY_CYCLES, SQ_WAVE_CYCLES,
         SQ_WAIT_ANY, SQ_INSTS_LDS, SQ_LDS_BANK_CONFLICT, GRBM_GUI_ACTIVE (with --kernel-trace
@@ -30,11 +31,14 @@ def _csv(d, suffix):
     return list(csv.DictReader(open(f[0])))
 
 
-def kernels(d, top):
+def kernels(d, top, by_grid=False):
     rows = _csv(d, "kernel_trace.csv")
     agg = collections.defaultdict(list)
     for r in rows:
-        agg[r["Kernel_Name"]].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+        name = r["Kernel_Name"]
+        if by_grid:   # one line per launch shape: e.g. the CRT GEMM's grid.y = moduli
+            name = f"[{r['Grid_Size_X']}x{r['Grid_Size_Y']}] " + name
+        agg[name].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
     tot = sum(sum(v) for v in agg.values()) or 1.0
     out = [f"{'kernel':62s} {'calls':>6s} {'avg_us':>9s} {'min_us':>9s} {'max_us':>9s} {'pct':>6s}"]
     for name, v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:top]:
@@ -123,6 +127,7 @@ def main(argv=None):
     k = sub.add_parser("kernels")
     k.add_argument("dir")
     k.add_argument("--top", type=int, default=25)
+    k.add_argument("--by-grid", action="store_true", help="one line per kernel and grid size")
     c = sub.add_parser("counters")
     c.add_argument("dir")
     c.add_argument("--simds", type=int, default=1024)
@@ -134,7 +139,7 @@ def main(argv=None):
     t.add_argument("--n", type=int, default=None, help="square size of the profiled workload (recorded)")
     a = ap.parse_args(argv)
     if a.cmd == "kernels":
-        print(kernels(a.dir, a.top))
+        print(kernels(a.dir, a.top, a.by_grid))
         return 0
     if a.cmd == "counters":
         print(counters(a.dir, a.simds, a.xcds))
