@@ -56,7 +56,9 @@ extern "C" {
 #define OZAIE_OPT_KCHUNK 3      /* cap on contraction indices per pass (0 = automatic)      */
 #define OZAIE_OPT_KERNEL 4      /* 0 auto, 1 bin-group kernel, 2 fused with LDS copies,
                                    3 fused with direct register loads, 4 fused pipelined,
-                                   5 fused staged (4-8 LDS stages; d <= 4, else as 2)     */
+                                   5 fused staged (4-8 LDS stages; d <= 4 and d = 9,
+                                     else as 2),
+                                   6 fused with LDS copies and rolling fragment reads     */
 #define OZAIE_OPT_PHASE_TIMING 5 /* 1: plans record HIP events around the split and GEMM    */
 #define OZAIE_OPT_CRT_GROUP_MODULI 6 /* moduli per workgroup of the grouped CRT modulus GEMM
                                    (0 = automatic per tile shape); read at group creation */
@@ -244,7 +246,7 @@ int32_t ozaie_debug_split(const double *X, int64_t rows, int64_t K, int64_t sr, 
 
 /* The d int32 bins of C = A * B (row-major device A, B) as computed by the slice GEMM
  * kernels (bins_out: host int32[d][M][N]).  K must fit one pass.  kernel: as
- * OZAIE_OPT_KERNEL (0 auto, 1 bin-group, 2/3/4/5 the fused families).                 */
+ * OZAIE_OPT_KERNEL (0 auto, 1 bin-group, 2-6 the fused families).                    */
 int32_t ozaie_debug_bins(int64_t M, int64_t N, int64_t K, const double *A, const double *B,
                          int bits, int slices, int kernel, int32_t *bins_out);
 

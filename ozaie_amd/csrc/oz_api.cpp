@@ -623,7 +623,7 @@ int32_t ozaie_set_option(int option, int value) {
             if (value < 0) return OZAIE_ERR_ARG;
             g_opt_kchunk = value; return OZAIE_OK;
         case OZAIE_OPT_KERNEL:
-            if (value < 0 || value > 5) return OZAIE_ERR_ARG;
+            if (value < 0 || value > 6) return OZAIE_ERR_ARG;
             g_opt_kernel = value; return OZAIE_OK;
         case OZAIE_OPT_PHASE_TIMING: g_opt_timing = value ? 1 : 0; return OZAIE_OK;
         case OZAIE_OPT_CRT_GROUP_MODULI:

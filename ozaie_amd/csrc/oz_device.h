@@ -655,7 +655,126 @@ __device__ __forceinline__ void gemm_lds_tile(const GemmLaunch &g, int tm, int t
     wave_epilogue<D, WM, WN>(g, epi, 0, acc, scratch, lane, rbase, cbase, er_v, ec_v);
 }
 
-// Staged variant (d <= 4, workgroup tile 128 x 128): the slice blocks go HBM -> LDS
+// Rolling variant (any d): the two LDS stages and copies of the LDS-copy kernel, but a
+// single fragment set, refilled as it dies.  The LDS-copy kernel reads all d (WM + WN) fragments of a block
+// right after its barrier, so every wave waits for that burst of LDS reads before its first
+// MFMA.  Here the fragments of block b + 1 are read during block b's triangle, each into a
+// fragment of block b right after that fragment's last use:
+//   rows i = 0 .. d-1 in turn, and in row i the columns j = d-1-i down to 0.  A_i dies at the
+//   end of row i; B_j is last used at the start of row d-1-j (later rows have smaller j).
+// The next block begins with row 0 at column d-1, so the latest-read fragment (B_0, read
+// after block b's last MFMA) is needed only after the d-1 other columns of row 0: at least
+// (d-1) WM WN MFMAs of slack (14 for d = 8 on 1 x 2 wave tiles), which covers an LDS read.
+//   step b: wait for block b + 1's copies, barrier (every wave has read block b, whose
+//           fragments were read during step b - 1), copy block b + 2 into block b's stage,
+//           then the triangle of block b with the rolling reads of block b + 1.
+// The last step reads a stale stage into fragments that are never multiplied.
+template <int D, int WM, int WN>
+__device__ __forceinline__ void gemm_roll_tile(const GemmLaunch &g, int tm, int tn, int epi, int8_t *smem) {
+    using F = FusedCfg<D, WM, WN>;
+    constexpr int BA = F::BA, BB = F::BB, NBLK = F::NBLK, NLD = F::NLD, NMIN = F::NMIN;
+
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = w & 1, wn = w >> 1;
+    const int64_t rbA0 = (int64_t)tm * BA, rbB0 = (int64_t)tn * BB;
+    const int64_t planeA = g.RbA * g.Kb * kBlkBytes, planeB = g.RbB * g.Kb * kBlkBytes;
+    const int rbase = tm * 64 * WM + wm * 32 * WM, cbase = tn * 64 * WN + wn * 32 * WN;
+    int er_v, ec_v;
+    load_tile_exps(g, epi, lane, rbase, 32 * WM, cbase, 32 * WN, er_v, ec_v);
+
+    const int8_t *src[NLD];
+    int ldsoff[NLD];
+#pragma unroll
+    for (int it = 0; it < NLD; it++) {
+        const int blk = w + 4 * it;
+        const int bc = blk < NBLK ? blk : NBLK - 1;
+        if (bc < D * BA) {
+            const int j = bc / BA, mb = bc % BA;
+            src[it] = g.SA + j * planeA + (rbA0 + mb) * g.Kb * kBlkBytes;
+        } else {
+            const int bb = bc - D * BA;
+            const int j = bb / BB, nb = bb % BB;
+            src[it] = g.SB + j * planeB + (rbB0 + nb) * g.Kb * kBlkBytes;
+        }
+        src[it] += lane * 16;
+        ldsoff[it] = bc * kBlkBytes;
+    }
+    auto issue = [&](int stage, int64_t kb) {
+#pragma unroll
+        for (int it = 0; it < NLD; it++) {
+            if (it < NMIN || w + 4 * it < NBLK)
+                __builtin_amdgcn_global_load_lds((const void *)(src[it] + kb * kBlkBytes),
+                                                 (lds_void *)(smem + stage * F::STAGE + ldsoff[it]), 16, 0, 0);
+        }
+    };
+    auto readA = [&](v4i (&a)[WM], const int8_t *st, int j) {
+#pragma unroll
+        for (int mi = 0; mi < WM; mi++)
+            a[mi] = *reinterpret_cast<const v4i *>(st + (j * BA + wm * WM + mi) * kBlkBytes + lane * 16);
+    };
+    auto readB = [&](v4i (&b)[WN], const int8_t *st, int j) {
+#pragma unroll
+        for (int ni = 0; ni < WN; ni++)
+            b[ni] = *reinterpret_cast<const v4i *>(st + (D * BA + j * BB + wn * WN + ni) * kBlkBytes + lane * 16);
+    };
+
+    v16i acc[D][WM][WN];
+#pragma unroll
+    for (int k = 0; k < D; k++)
+#pragma unroll
+        for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+            for (int ni = 0; ni < WN; ni++) acc[k][mi][ni] = v16i{};
+
+    const int64_t kb0 = g.kb0, nk = g.kb1 - g.kb0;
+    v4i af[D][WM], bf[D][WN];
+    if (nk > 0) {
+        issue(0, kb0);
+        s_waitcnt_vm<0>();
+        lds_barrier();
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            readA(af[j], smem, j);
+            readB(bf[j], smem, j);
+        }
+        if (nk > 1) issue(1, kb0 + 1);
+    }
+    for (int64_t b = 0; b < nk; b++) {
+        __builtin_amdgcn_sched_barrier(0);
+        s_waitcnt_vm<0>();   // block b + 1 (the only copies in flight) is in LDS
+        lds_barrier();       // ... for every wave; every wave's reads of block b are done
+        if (b + 2 < nk) issue((int)(b & 1), kb0 + b + 2);
+        const int8_t *nx = smem + (int)((b + 1) & 1) * F::STAGE;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+#pragma unroll
+            for (int jj = 0; jj < D - i; jj++) {
+                const int j = D - 1 - i - jj;
+#pragma unroll
+                for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+                    for (int ni = 0; ni < WN; ni++)
+                        acc[i + j][mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i][mi], bf[j][ni],
+                                                                                   acc[i + j][mi][ni], 0, 0, 0);
+                if (jj == 0) {   // the last use of B_j in this block
+                    __builtin_amdgcn_sched_barrier(0);
+                    readB(bf[j], nx, j);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            readA(af[i], nx, i);   // the last use of A_i was this row
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    __syncthreads();
+
+    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * wave_scratch_words<D, WM, WN>();
+    wave_epilogue<D, WM, WN>(g, epi, 0, acc, scratch, lane, rbase, cbase, er_v, ec_v);
+}
+
+// Staged variant (d <= 4 on a 128 x 128 workgroup tile, d = 9 on 64 x 64): the slice blocks go HBM -> LDS
 // (global_load_lds_dwordx4, each wave bringing NBLK / 4 whole 1 KiB blocks) into one of
 // S stages, so S - 1 contraction blocks are in flight, and the fragments of block b + 1
 // are read from LDS while the triangle of block b runs:
@@ -668,8 +787,9 @@ __device__ __forceinline__ void gemm_lds_tile(const GemmLaunch &g, int tm, int t
 // loop has no branch; a checked tail pass issues no load past the pass's last block and
 // waits for every load, so the epilogue's LDS scratch follows the last barrier safely.
 // Any number of contraction blocks per pass.
-// Stages S: as many as fit in 160 KiB of LDS, even, at most 8 (d = 1, 2: 8; d = 3: 6; d = 4:
-// 4).  Measured against four stages: d = 1 8-22% faster, d = 2 and 3 unchanged.
+// Stages S: as many as fit in 160 KiB of LDS, even, at most 8 (d = 1, 2: 8; d = 3: 6; d = 4
+// and d = 9 on 64 x 64: 4).  Measured against four stages: d = 1 8-22% faster, d = 2 and 3
+// unchanged.
 template <int D, int WM, int WN>
 __host__ __device__ constexpr int staged_stages() {
     const int s = (160 * 1024 / FusedCfg<D, WM, WN>::STAGE) & ~1;

@@ -64,7 +64,7 @@ struct GemmLaunch {
 };
 
 // Fused-kernel families (same arithmetic, different data movement).
-enum Family { KERNEL_AUTO = 0, KERNEL_LDS = 2, KERNEL_DIRECT = 3, KERNEL_PIPE = 4, KERNEL_STAGED = 5 };
+enum Family { KERNEL_AUTO = 0, KERNEL_LDS = 2, KERNEL_DIRECT = 3, KERNEL_PIPE = 4, KERNEL_STAGED = 5, KERNEL_ROLL = 6 };
 
 // One operand of one problem of a grouped launch (R rows x K contraction indices).
 struct GroupOperand {

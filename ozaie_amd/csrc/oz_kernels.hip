@@ -211,6 +211,16 @@ void k_gemm_direct(GemmLaunch g, int epi) {
     gemm_direct_tile<D, WM, WN>(g, tm, tn, epi, smem);
 }
 
+template <int D, int WM, int WN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_gemm_roll(GemmLaunch g, int epi) {
+    const int ntm = (g.M + 64 * WM - 1) / (64 * WM), ntn = (g.N + 64 * WN - 1) / (64 * WN);
+    int tm, tn;
+    tile_map(blockIdx.x, ntm, ntn, tm, tn);
+    __shared__ __attribute__((aligned(1024))) int8_t smem[lds_tile_bytes<D, WM, WN>()];
+    gemm_roll_tile<D, WM, WN>(g, tm, tn, epi, smem);
+}
+
 // Pipelined LDS variant: the slice blocks of block t+2 are copied HBM -> LDS
 // (global_load_lds) while block t computes; fragments are double-buffered in registers
 // and the workgroup barrier sits in the middle of each block's MFMAs.  After the
@@ -356,12 +366,21 @@ void k_gemm_staged(GemmLaunch g, int epi) {
 
 bool fused_supported(int d) { return d >= 1 && d <= 16; }
 
-// Default fused-kernel family per slice count (measured on MI355X at 4096^3 and 8192^3):
-// the staged kernel for d <= 3 (12-36% faster than the others there; at d = 4 it and the
-// direct kernel are within 4% of each other, either way round by size), the
-// direct-to-register kernel for d <= 8 (its two fragment sets fit in registers), the
-// LDS-copy kernel above that.
-static int default_family(int d) { return d <= 3 ? KERNEL_STAGED : (d <= 8 ? KERNEL_DIRECT : KERNEL_LDS); }
+// Default fused-kernel family per slice count (measured on MI355X at 4096^3 and 8192^3,
+// the families interleaved in one process, medians):
+//   d <= 3: the staged kernel (12-36% faster than the others; at d = 4 it and the direct
+//           kernel are within 4% of each other, either way round by size);
+//   d = 7 and d = 9: the rolling kernel (d = 7: 1.6% faster than the direct kernel at
+//           8192^3; d = 9 on 64 x 64 tiles: 3.6% faster than the LDS-copy kernel on
+//           64 x 128 tiles at 8192^3, 2.3% at 4096^3, and ahead of the staged kernel);
+//   other d <= 8: the direct-to-register kernel (its two fragment sets fit in registers;
+//           at d = 8 the rolling kernel is 2.4% slower);
+//   above: the LDS-copy kernel.
+static int default_family(int d) {
+    if (d <= 3) return KERNEL_STAGED;
+    if (d == 7 || d == 9) return KERNEL_ROLL;
+    return d <= 8 ? KERNEL_DIRECT : KERNEL_LDS;
+}
 
 template <int D, int WM, int WN>
 static hipError_t launch_fused_t(const GemmLaunch &g, int epi, hipStream_t st) {
@@ -369,7 +388,7 @@ static hipError_t launch_fused_t(const GemmLaunch &g, int epi, hipStream_t st) {
     const int64_t tiles = (int64_t)ntm * ntn;
     if (tiles <= 0) return hipSuccess;
     int fam = g.family == KERNEL_AUTO ? default_family(D) : g.family;
-    if constexpr (D <= 4) {
+    if constexpr (D <= 4 || (D == 9 && WM == 1 && WN == 1)) {
         if (fam == KERNEL_STAGED) {
             hipLaunchKernelGGL((k_gemm_staged<D, WM, WN>), dim3((unsigned)tiles), dim3(256), 0, st, g, epi);
             return hipGetLastError();
@@ -379,6 +398,8 @@ static hipError_t launch_fused_t(const GemmLaunch &g, int epi, hipStream_t st) {
         hipLaunchKernelGGL((k_gemm_direct<D, WM, WN>), dim3((unsigned)tiles), dim3(256), 0, st, g, epi);
     else if (fam == KERNEL_PIPE)
         hipLaunchKernelGGL((k_gemm_pipe<D, WM, WN>), dim3((unsigned)tiles), dim3(256), 0, st, g, epi);
+    else if (fam == KERNEL_ROLL)
+        hipLaunchKernelGGL((k_gemm_roll<D, WM, WN>), dim3((unsigned)tiles), dim3(256), 0, st, g, epi);
     else
         hipLaunchKernelGGL((k_gemm_fused<D, WM, WN>), dim3((unsigned)tiles), dim3(256), 0, st, g, epi);
     return hipGetLastError();
@@ -396,7 +417,11 @@ hipError_t launch_gemm_fused(const GemmLaunch &g, int epi, hipStream_t st) {
         case 6: return launch_fused_t<6, 1, 2>(g, epi, st);
         case 7: return launch_fused_t<7, 1, 2>(g, epi, st);
         case 8: return launch_fused_t<8, 1, 2>(g, epi, st);
-        case 9: return launch_fused_t<9, 1, 2>(g, epi, st);
+        case 9: {   // 1 x 2 only for the families that hold one fragment set (the others spill)
+            const int fam = g.family == KERNEL_AUTO ? default_family(9) : g.family;
+            return fam == KERNEL_LDS || fam == KERNEL_PIPE ? launch_fused_t<9, 1, 2>(g, epi, st)
+                                                           : launch_fused_t<9, 1, 1>(g, epi, st);
+        }
         case 10: return launch_fused_t<10, 1, 1>(g, epi, st);
         case 11: return launch_fused_t<11, 1, 1>(g, epi, st);
         case 12: return launch_fused_t<12, 1, 1>(g, epi, st);

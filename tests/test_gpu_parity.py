@@ -53,11 +53,13 @@ def test_split_bit_exact(gpu, b, d):
 
 
 # kernel: 0 auto, 1 bin-group, 2 fused LDS copies, 3 fused direct loads, 4 fused pipelined,
-# 5 fused staged (d <= 4; larger d run the LDS-copy kernel)
+# 5 fused staged (d <= 4 and d = 9; other d run the LDS-copy kernel), 6 fused LDS copies
+# with rolling fragment reads
 BIN_CASES = [(7, 8, 0), (7, 3, 0), (7, 7, 0), (7, 9, 0), (7, 1, 0), (7, 4, 0), (7, 10, 0), (6, 16, 0),
              (3, 29, 1), (3, 26, 1), (7, 8, 1), (4, 18, 1),
              (7, 8, 2), (7, 3, 2), (7, 9, 2), (7, 8, 3), (7, 3, 3), (7, 5, 3), (7, 8, 4), (7, 3, 4),
-             (7, 7, 4), (6, 12, 4), (7, 1, 5), (7, 2, 5), (7, 3, 5), (7, 4, 5), (7, 8, 5)]
+             (7, 7, 4), (6, 12, 4), (7, 1, 5), (7, 2, 5), (7, 3, 5), (7, 4, 5), (7, 8, 5),
+             (7, 9, 3), (7, 9, 5), (7, 9, 6), (7, 8, 6), (7, 3, 6), (6, 12, 6)]
 
 
 @pytest.mark.parametrize("b,d,kernel", BIN_CASES)
@@ -161,7 +163,7 @@ def test_multi_pass_kchunk_and_groups(gpu):
     M, N, K = 70, 90, 1000
     A = gen_inputs(rng, M, K, 1.0)
     B = gen_inputs(rng, K, N, 1.0)
-    for b, d, kernel in [(7, 8, 0), (3, 29, 1), (7, 8, 1), (7, 3, 5)]:
+    for b, d, kernel in [(7, 8, 0), (3, 29, 1), (7, 8, 1), (7, 3, 5), (7, 9, 3), (7, 9, 5), (7, 9, 6)]:
         oz.set_option(oz.OPT_KCHUNK, 128)
         oz.set_option(oz.OPT_KERNEL, kernel)
         try:
