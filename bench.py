@@ -747,7 +747,7 @@ def main():
         return
     if args.probe_only:
         rec = {}
-        for pat in ("zeros", "constant", "random"):
+        for pat in ("zeros", "constant", "random", "random16x16x64", "symmetric127"):
             pops, ms = oz.probe_mfma(pat)
             rec[pat] = {"tops": pops * 1e3, "ms": ms}
         print(json.dumps({"int8_mfma_ceiling": rec, "datasheet_tops": INT8_PEAK_TOPS}), flush=True)

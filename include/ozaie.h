@@ -268,7 +268,9 @@ int32_t ozaie_exact_entries(const double *A, int64_t lda, const double *B, int64
  * registers only on every CU (one wave per SIMD, 16 independent accumulator tiles per
  * wave), for `iters` rounds of 128 MFMAs after one warm-up launch.  pattern 0: all-zero
  * operands; 1: one random fragment pair for every MFMA (constant operands); 2: eight random
- * fragment pairs in turn (operands toggle like slice or residue data).  The rate the
+ * fragment pairs in turn (operands toggle like slice or residue data); 3: pattern 2 on
+ * v_mfma_i32_16x16x64_i8 (the same int8 operations per round); 4: eight fragment pairs of
+ * bytes uniform in [-127, 127] (symmetric residues, sign-magnitude slices).  The rate the
  * matrix cores sustain depends on the operand data through power and clock; this is the
  * measured ceiling the GEMM kernels are compared with beside the datasheet peak.
  * ms_out (optional): milliseconds of the timed launch.                                  */

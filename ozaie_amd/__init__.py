@@ -550,13 +550,14 @@ def set_tune_table(slices, table, crt=False):
         _check(lib().ozaie_tune_set(int(slices), _ptr(t)), "ozaie_tune_set")
 
 
-PROBE_PATTERNS = {"zeros": 0, "constant": 1, "random": 2}
+PROBE_PATTERNS = {"zeros": 0, "constant": 1, "random": 2, "random16x16x64": 3, "symmetric127": 4}
 
 
 def probe_mfma(pattern="random", iters=20000):
     """Measured int8 MFMA ceiling (POP/s, ms of the timed launch) with register-only
-    operands: "zeros", "constant" (one random fragment pair) or "random" (operands that
-    toggle every MFMA, like slice data)."""
+    operands: "zeros", "constant" (one random fragment pair), "random" (operands that
+    toggle every MFMA, like slice data), "random16x16x64" (the same on the 16 x 16 x 64
+    MFMA) or "symmetric127" (bytes uniform in [-127, 127], like residues and slices)."""
     pops = ctypes.c_double()
     ms = ctypes.c_double()
     _check(lib().ozaie_probe_mfma(PROBE_PATTERNS[pattern], int(iters), ctypes.byref(pops), ctypes.byref(ms)),

@@ -8,15 +8,17 @@ import ozaie_amd as oz
 
 @pytest.mark.gpu
 def test_probe_rates(gpu):
-    rates = {p: oz.probe_mfma(p, iters=4000)[0] for p in ("zeros", "constant", "random")}
+    rates = {p: oz.probe_mfma(p, iters=4000)[0]
+             for p in ("zeros", "constant", "random", "random16x16x64", "symmetric127")}
     print(rates, flush=True)
     for p, r in rates.items():
         assert 0.3 < r < 5.2, (p, r)
     assert rates["random"] <= rates["zeros"] * 1.05
+    assert rates["symmetric127"] <= rates["zeros"] * 1.05
 
 
 def test_probe_rejects_bad_arguments():
     import ctypes
     v = ctypes.c_double()
-    assert oz.lib().ozaie_probe_mfma(3, 10, ctypes.byref(v), None) == oz.ERR_ARG
+    assert oz.lib().ozaie_probe_mfma(5, 10, ctypes.byref(v), None) == oz.ERR_ARG
     assert oz.lib().ozaie_probe_mfma(0, 0, ctypes.byref(v), None) == oz.ERR_ARG
