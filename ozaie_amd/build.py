@@ -8,7 +8,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libozaie.so")
 ARCH = os.environ.get("OZAIE_OFFLOAD_ARCH", "gfx950")
 SOURCES = ["oz_kernels.hip", "oz_grouped.hip", "oz_crt.hip", "oz_probe.hip", "oz_api.cpp", "oz_group.cpp"]
-HOST_SOURCES = ["oz_verify.cpp"]   # plain host C++ (g++, OpenMP)
+HOST_SOURCES = ["oz_verify.cpp", "oz_host.cpp"]   # plain host C++ (g++, OpenMP)
 
 
 def hipcc():

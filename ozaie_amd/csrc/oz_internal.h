@@ -266,6 +266,9 @@ int64_t max_kchunk_blocks(int b, int d);
 
 // Kernel launchers (oz_kernels.hip).  All enqueue on `st` and return hipError_t.
 hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
+
+// Host memcpy split over OpenMP threads (oz_host.cpp).
+void parallel_copy(void *dst, const void *src, size_t n);
 hipError_t launch_maxexp(const OperandView &op, int32_t *E, int64_t Rp, hipStream_t st);
 hipError_t launch_debug_convert(const int64_t *hi, const uint64_t *lo, const int32_t *es, int64_t n, int b, int d,
                                 int rounding, double *out128, double *out64, hipStream_t st);

@@ -156,6 +156,36 @@ def test_reference_api_device_pointers(gpu):
     assert np.array_equal(bits(dC.cpu().numpy()), bits(ref))
 
 
+@pytest.mark.parametrize("mode", ["fast", "crt"])
+def test_reference_api_host_arrays_across_staging_chunks(gpu, mode):
+    """Host arrays larger than the 32 MiB pinned staging chunks (A and B two chunks each with
+    a partial last one, C three), ragged sizes: C is bit-identical to the same plan's result
+    from device pointers, and sampled rows equal the oracle's (exponents, shifts and slices
+    of a row of A or a column of B depend on that row or column only)."""
+    import torch
+    rng = np.random.default_rng(31)
+    M, N, K = 3001, 2999, 2500
+    A = gen_inputs(rng, M, K, 1.0)
+    B = gen_inputs(rng, K, N, 1.0)
+    m = oz.Mode.fast if mode == "fast" else oz.Mode.crt
+    assert oz.ozAIE_setup(M, N, K, m) == oz.OK
+    C = np.empty((M, N), dtype=np.float64)
+    assert oz.ozAIE_DGEMM(M, N, K, A, B, C) == oz.OK
+    dC = torch.empty((M, N), dtype=torch.float64, device="cuda")
+    assert oz.ozAIE_DGEMM(M, N, K, dev(A), dev(B), dC) == oz.OK
+    assert np.array_equal(bits(C), bits(dC.cpu().numpy()))
+    rows = np.array([0, 1, 1499, 2000, M - 1])
+    cols = np.arange(0, N, 97)
+    Ar, Bc = A[rows], np.ascontiguousarray(B[:, cols])
+    if mode == "fast":
+        ref, ov = oracle.ozaki_dgemm(Ar, Bc, 7, 8)
+        assert ov == 0
+    else:
+        ref = oracle.crt_dgemm(Ar, Bc, oz.plan_info()["slices"], oz.get_option(oz.OPT_ROUNDING))
+    assert np.array_equal(bits(C[np.ix_(rows, cols)]), bits(ref))
+    oz.release()
+
+
 def test_multi_pass_kchunk_and_groups(gpu):
     """Forcing several contraction chunks (and bin groups) gives the single-pass result."""
     import torch
