@@ -58,7 +58,9 @@ extern "C" {
                                    3 fused with direct register loads, 4 fused pipelined,
                                    5 fused staged (4-8 LDS stages; d <= 4 and d = 9,
                                      else as 2),
-                                   6 fused with LDS copies and rolling fragment reads     */
+                                   6 fused with LDS copies and rolling fragment reads;
+                                   CRT plans: 2 the 256 x 256 modulus GEMM, 3 its per-wave-
+                                   load form, 4 128 x 128 tiles, others by size           */
 #define OZAIE_OPT_PHASE_TIMING 5 /* 1: plans record HIP events around the split and GEMM    */
 #define OZAIE_OPT_CRT_GROUP_MODULI 6 /* moduli per workgroup of the grouped CRT modulus GEMM
                                    (0 = automatic per tile shape); read at group creation */

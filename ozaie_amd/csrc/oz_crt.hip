@@ -216,71 +216,82 @@ __device__ __forceinline__ void crt_stats_emit(const double *tile, uint64_t *__r
     st[(rb * 32 + row) * nch + kt * 8 + q] = crt_chunk_stat(v);
 }
 
-template <bool KC>
-__global__ __launch_bounds__(256) void k_crt_stats(const double *__restrict__ X, int64_t sr, int64_t sk, int64_t R,
-                                                   int64_t K, uint64_t *__restrict__ st, int64_t nch) {
+// The split phase's kernels take both operands: grid z = 0 is A, 1 is B.  Each block's
+// operand decides the orientation (uniform over the block), and blocks beyond an operand's
+// row blocks return at once.  One launch per step instead of one per operand and step: at
+// 2048^3 and below the six launches of a split phase cost as much as the data they move.
+__global__ __launch_bounds__(256) void k_crt_stats(CrtSide a, CrtSide b, int64_t nch) {
+    const CrtSide o = blockIdx.z ? b : a;
+    if ((int64_t)blockIdx.y * kBlk >= o.Rp) return;
     __shared__ double tile[kSplitLds / 8];
     double v[16];
-    split_gather<KC>(X, sr, sk, R, K, blockIdx.y, blockIdx.x, v);
-    split_stage<KC>(v, tile);
-    __syncthreads();
-    crt_stats_emit<KC>(tile, st, nch, blockIdx.y, blockIdx.x);
+    if (o.op.sk == 1) {
+        split_gather<true>(o.op.X, o.op.sr, o.op.sk, o.op.R, o.op.K, blockIdx.y, blockIdx.x, v);
+        split_stage<true>(v, tile);
+        __syncthreads();
+        crt_stats_emit<true>(tile, o.stats, nch, blockIdx.y, blockIdx.x);
+    } else {
+        split_gather<false>(o.op.X, o.op.sr, o.op.sk, o.op.R, o.op.K, blockIdx.y, blockIdx.x, v);
+        split_stage<false>(v, tile);
+        __syncthreads();
+        crt_stats_emit<false>(tile, o.stats, nch, blockIdx.y, blockIdx.x);
+    }
 }
 
-hipError_t launch_crt_stats(const OperandView &op, uint64_t *stats, int64_t Rp, int64_t Kp, hipStream_t st) {
-    const int64_t Rb = Rp / kBlk, Kb = Kp / kBlk;
+hipError_t launch_crt_stats(const CrtSide &a, const CrtSide &b, int64_t Kp, hipStream_t st) {
+    const int64_t Rb = std::max(a.Rp, b.Rp) / kBlk, Kb = Kp / kBlk;
     const int64_t nkt = (Kb + 3) / 4;
+    if (Rb <= 0 || nkt <= 0) return hipSuccess;
     if (Rb > 65535 || nkt > 0x7fffffff) return hipErrorInvalidValue;
-    if (op.sk == 1)
-        hipLaunchKernelGGL(k_crt_stats<true>, dim3((unsigned)nkt, (unsigned)Rb), dim3(256), 0, st, op.X, op.sr,
-                           op.sk, op.R, op.K, stats, crt_chunks(Kb));
-    else
-        hipLaunchKernelGGL(k_crt_stats<false>, dim3((unsigned)nkt, (unsigned)Rb), dim3(256), 0, st, op.X, op.sr,
-                           op.sk, op.R, op.K, stats, crt_chunks(Kb));
+    hipLaunchKernelGGL(k_crt_stats, dim3((unsigned)nkt, (unsigned)Rb, 2), dim3(256), 0, st, a, b, crt_chunks(Kb));
     return hipGetLastError();
 }
 
-// Wave per row (four rows per workgroup).
-__global__ __launch_bounds__(256) void k_crt_scale(const uint64_t *__restrict__ st, int64_t Rp, int64_t nch,
-                                                   int64_t K, CrtConsts c, int32_t *__restrict__ sigma) {
+// Wave per row (four rows per workgroup): the Rp rows of A, then the Rp rows of B.
+__global__ __launch_bounds__(256) void k_crt_scale(CrtSide a, CrtSide b, int64_t nch, int64_t K, CrtConsts c) {
     const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r >= Rp) return;   // whole wave
-    const int s = crt_row_sigma(st + r * nch, nch, K, c.beta, c, threadIdx.x & 63);
-    if ((threadIdx.x & 63) == 0) sigma[r] = s;
+    const bool onb = r >= a.Rp;
+    const CrtSide o = onb ? b : a;
+    const int64_t rr = onb ? r - a.Rp : r;
+    if (rr >= o.Rp) return;   // whole wave
+    const int s = crt_row_sigma(o.stats + rr * nch, nch, K, c.beta, c, threadIdx.x & 63);
+    if ((threadIdx.x & 63) == 0) o.sigma[rr] = s;
 }
 
-hipError_t launch_crt_scale(const uint64_t *stats, int64_t Rp, int64_t nch, int64_t K, const CrtConsts &c,
-                            int32_t *sigma, hipStream_t st) {
-    const int64_t nb = (Rp + 3) / 4;
+hipError_t launch_crt_scale(const CrtSide &a, const CrtSide &b, int64_t nch, int64_t K, const CrtConsts &c,
+                            hipStream_t st) {
+    const int64_t nb = (a.Rp + b.Rp + 3) / 4;
     if (nb <= 0) return hipSuccess;
     if (nb > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_crt_scale, dim3((unsigned)nb), dim3(256), 0, st, stats, Rp, nch, K, c, sigma);
+    hipLaunchKernelGGL(k_crt_scale, dim3((unsigned)nb), dim3(256), 0, st, a, b, nch, K, c);
     return hipGetLastError();
 }
 
-template <bool KC>
-__global__ __launch_bounds__(256) void k_crt_split(const double *__restrict__ X, int64_t sr, int64_t sk,
-                                                   int64_t R, int64_t K, const int32_t *__restrict__ sigma,
-                                                   int8_t *__restrict__ S, int64_t Rb, int64_t Kb, CrtConsts c) {
+__global__ __launch_bounds__(256) void k_crt_split(CrtSide a, CrtSide b, int64_t Kb, CrtConsts c) {
+    const CrtSide o = blockIdx.z ? b : a;
+    const int64_t Rb = o.Rp / kBlk;
+    if ((int64_t)blockIdx.y >= Rb) return;
     __shared__ double tile[kSplitLds / 8];
     double v[16];
-    split_gather<KC>(X, sr, sk, R, K, blockIdx.y, blockIdx.x, v);
-    split_stage<KC>(v, tile);
-    __syncthreads();
-    crt_emit<KC>(tile, R, sigma + (int64_t)blockIdx.y * 32, c, S, Rb, Kb, blockIdx.y, blockIdx.x);
+    if (o.op.sk == 1) {
+        split_gather<true>(o.op.X, o.op.sr, o.op.sk, o.op.R, o.op.K, blockIdx.y, blockIdx.x, v);
+        split_stage<true>(v, tile);
+        __syncthreads();
+        crt_emit<true>(tile, o.op.R, o.sigma + (int64_t)blockIdx.y * 32, c, o.S, Rb, Kb, blockIdx.y, blockIdx.x);
+    } else {
+        split_gather<false>(o.op.X, o.op.sr, o.op.sk, o.op.R, o.op.K, blockIdx.y, blockIdx.x, v);
+        split_stage<false>(v, tile);
+        __syncthreads();
+        crt_emit<false>(tile, o.op.R, o.sigma + (int64_t)blockIdx.y * 32, c, o.S, Rb, Kb, blockIdx.y, blockIdx.x);
+    }
 }
 
-hipError_t launch_crt_split(const OperandView &op, const int32_t *sigma, const CrtConsts &c, int8_t *S, int64_t Rp,
-                            int64_t Kp, hipStream_t st) {
-    const int64_t Rb = Rp / kBlk, Kb = Kp / kBlk;
+hipError_t launch_crt_split(const CrtSide &a, const CrtSide &b, const CrtConsts &c, int64_t Kp, hipStream_t st) {
+    const int64_t Rb = std::max(a.Rp, b.Rp) / kBlk, Kb = Kp / kBlk;
     const int64_t nkt = (Kb + 3) / 4;
+    if (Rb <= 0 || nkt <= 0) return hipSuccess;
     if (Rb > 65535 || nkt > 0x7fffffff) return hipErrorInvalidValue;
-    if (op.sk == 1)
-        hipLaunchKernelGGL(k_crt_split<true>, dim3((unsigned)nkt, (unsigned)Rb), dim3(256), 0, st, op.X, op.sr,
-                           op.sk, op.R, op.K, sigma, S, Rb, Kb, c);
-    else
-        hipLaunchKernelGGL(k_crt_split<false>, dim3((unsigned)nkt, (unsigned)Rb), dim3(256), 0, st, op.X, op.sr,
-                           op.sk, op.R, op.K, sigma, S, Rb, Kb, c);
+    hipLaunchKernelGGL(k_crt_split, dim3((unsigned)nkt, (unsigned)Rb, 2), dim3(256), 0, st, a, b, Kb, c);
     return hipGetLastError();
 }
 
@@ -378,8 +389,13 @@ void k_crt_gemm(CrtGemm g, CrtConsts c) {
                                   g.Np);
 }
 
+// Waves per SIMD the tile shape leaves room for: the 256 x 256 tile's 16 accumulator tiles
+// take half the registers of one wave, the smaller ones fit two and four.
+constexpr int crt_occupancy(int tiles) { return tiles >= 8 ? 1 : (tiles >= 2 ? 2 : 4); }
+
 template <int WM, int WN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(crt_occupancy(WM * WN), crt_occupancy(WM * WN))))
 void k_crt_gemm_lds(CrtGemm g, CrtConsts c) {
     __shared__ __attribute__((aligned(1024))) int8_t smem[kCrtStages * (2 * WM + 2 * WN) * kBlkBytes];
     const int j = blockIdx.y;
@@ -399,16 +415,19 @@ void k_crt_gemm_lds(CrtGemm g, CrtConsts c) {
 }
 
 hipError_t launch_crt_gemm(const CrtGemm &g, const CrtConsts &c, hipStream_t st) {
-    constexpr int WM = 4, WN = 4;
+    const int WM = g.variant == 2 ? 2 : 4, WN = WM;   // workgroup tile 128 x 128 or 256 x 256
     if (g.Mp % (64 * WM) || g.Np % (64 * WN) || g.Kb % kCrtPF || g.Kb <= 0 || c.n < 1 || c.n > kCrtMaxMod)
         return hipErrorInvalidValue;
     const int64_t tiles = (g.Mp / (64 * WM)) * (g.Np / (64 * WN));
     if (tiles <= 0) return hipSuccess;
     if (tiles > 0x7fffffff) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)tiles, (unsigned)c.n);
     if (g.variant == 1)
-        hipLaunchKernelGGL((k_crt_gemm<WM, WN>), dim3((unsigned)tiles, (unsigned)c.n), dim3(256), 0, st, g, c);
+        hipLaunchKernelGGL((k_crt_gemm<4, 4>), grid, dim3(256), 0, st, g, c);
+    else if (g.variant == 2)
+        hipLaunchKernelGGL((k_crt_gemm_lds<2, 2>), grid, dim3(256), 0, st, g, c);
     else
-        hipLaunchKernelGGL((k_crt_gemm_lds<WM, WN>), dim3((unsigned)tiles, (unsigned)c.n), dim3(256), 0, st, g, c);
+        hipLaunchKernelGGL((k_crt_gemm_lds<4, 4>), grid, dim3(256), 0, st, g, c);
     return hipGetLastError();
 }
 
@@ -710,10 +729,6 @@ hipError_t launch_group_crt_split(const GroupOperand *ops, const CrtGroupProb *p
     hipLaunchKernelGGL(k_grp_crt_split, dim3((unsigned)n), dim3(256), 0, st, ops, items, c);
     return hipGetLastError();
 }
-
-// Waves per SIMD the tile shape leaves room for: the 256 x 256 tile's 16 accumulator tiles
-// take half the registers of one wave, the smaller ones fit two and four.
-constexpr int crt_occupancy(int tiles) { return tiles >= 8 ? 1 : (tiles >= 2 ? 2 : 4); }
 
 // One workgroup per tile item and range of moduli [j0, j1) (grid y): the contraction loop
 // of each modulus in turn, the next modulus's first blocks issued before the epilogue of

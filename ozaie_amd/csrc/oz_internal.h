@@ -187,21 +187,30 @@ void crt_consts(int n, int beta, CrtConsts &c);
 // S_c the sum of squared 16-bit ceilings of the chunk's magnitudes; [Rp][crt_chunks(Kb)].
 constexpr int kCrtChunk = 16;
 __host__ __device__ inline int64_t crt_chunks(int64_t Kb) { return (Kb + 3) / 4 * 8; }   // 8 per split tile of 128
-hipError_t launch_crt_stats(const OperandView &op, uint64_t *stats, int64_t Rp, int64_t Kp, hipStream_t st);
+// One operand of the split phase: its view, its Rp rows' chunk statistics and shifts, and
+// its residue planes.  Each launch below handles both operands (A and B) at once.
+struct CrtSide {
+    OperandView op;
+    uint64_t *stats;   // [Rp][crt_chunks(Kb)]
+    int32_t *sigma;    // [Rp]
+    int8_t *S;         // residues [n][Rp / 32][Kb][1024]
+    int64_t Rp;
+};
+hipError_t launch_crt_stats(const CrtSide &a, const CrtSide &b, int64_t Kp, hipStream_t st);
 // sigma[r] for r < Rp from the statistics: the norm rule for c.beta = 0 (contraction
 // length K), else sigma = 53 + E - c.beta.
-hipError_t launch_crt_scale(const uint64_t *stats, int64_t Rp, int64_t nch, int64_t K, const CrtConsts &c,
-                            int32_t *sigma, hipStream_t st);
-// Residues S[j][rb][kb][1024] (int8, slice-block layout) of one operand with shifts sigma.
-hipError_t launch_crt_split(const OperandView &op, const int32_t *sigma, const CrtConsts &c, int8_t *S, int64_t Rp,
-                            int64_t Kp, hipStream_t st);
+hipError_t launch_crt_scale(const CrtSide &a, const CrtSide &b, int64_t nch, int64_t K, const CrtConsts &c,
+                            hipStream_t st);
+// Residues S[j][rb][kb][1024] (int8, slice-block layout) with shifts sigma.
+hipError_t launch_crt_split(const CrtSide &a, const CrtSide &b, const CrtConsts &c, int64_t Kp, hipStream_t st);
 // g_j = A_j B_j^T mod p_j for every modulus: R[j][Mp/4][Np] words of four rows, one signed
 // byte each (congruent to g_j, |byte| <= p_j / 2).
 struct CrtGemm {
     const int8_t *SA, *SB;
     int64_t Mp, Np, Kb;
     uint32_t *R;
-    int variant;   // 0: fragments shared through LDS, 1: per-wave loads (OZAIE_OPT_KERNEL 3)
+    int variant;   // 0: fragments shared through LDS (256 x 256 tiles), 1: per-wave loads
+                   // (OZAIE_OPT_KERNEL 3), 2: as 0 on 128 x 128 tiles, two workgroups per CU
 };
 hipError_t launch_crt_gemm(const CrtGemm &g, const CrtConsts &c, hipStream_t st);
 // C = rebuild(R) * 2^(sigma_r + sigma_c - 2150), truncated (rounding 0) or nearest (1).
