@@ -291,7 +291,9 @@ hipError_t launch_crt_split(const CrtSide &a, const CrtSide &b, const CrtConsts 
     const int64_t nkt = (Kb + 3) / 4;
     if (Rb <= 0 || nkt <= 0) return hipSuccess;
     if (Rb > 65535 || nkt > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_crt_split, dim3((unsigned)nkt, (unsigned)Rb, 2), dim3(256), 0, st, a, b, Kb, c);
+    // B first (grid z = 0): the statistics launch read B last, so part of it may still be in
+    // the memory-side cache
+    hipLaunchKernelGGL(k_crt_split, dim3((unsigned)nkt, (unsigned)Rb, 2), dim3(256), 0, st, b, a, Kb, c);
     return hipGetLastError();
 }
 
