@@ -54,3 +54,32 @@ def test_gpu_conversion_matches_oracle(gpu, b, d, rounding):
     fits = np.array([-(1 << 63) <= v < (1 << 63) for v in vals])
     bad64 = np.nonzero(fits & (out64.view(np.int64) != ref.view(np.int64)))[0]
     assert bad64.size == 0, [(vals[i], int(es[i]), out64[i], ref[i]) for i in bad64[:5]]
+
+
+@pytest.mark.parametrize("b,d", [(7, 8), (3, 29)])
+@pytest.mark.parametrize("rounding", [0, 1])
+def test_gpu_conversion_out_of_range_matches_oracle(gpu, b, d, rounding):
+    """Exponent sums (within the 0..4094 that two exponent fields reach) whose results lie
+    beyond the double range or below it: truncation writes the exponent into the bits with
+    no over- or underflow handling, as the reference's host conversion does, and nearest
+    rounding scales with ldexp (infinity, subnormals, zero); the GPU matches the oracle bit
+    for bit either way."""
+    rng = np.random.default_rng(7 * b + d + 3 * rounding)
+    vals = [v for v in cases(rng) if v != 0]
+    nbs = np.array([abs(v).bit_length() for v in vals])
+    target = np.concatenate([rng.integers(-1130, -1000, len(vals) // 2),
+                             rng.integers(1000, 1100, len(vals) - len(vals) // 2)])
+    es = np.clip(target - nbs + 2045 + b * (d + 1), 0, 4094).astype(np.int32)
+    hi = np.array([(v >> 64) for v in vals], dtype=np.int64)
+    lo = np.array([v & ((1 << 64) - 1) for v in vals], dtype=np.uint64)
+    out, out64 = oz.debug_convert(hi, lo, es, b, d, rounding)
+    ref = np.array([oracle.T_to_double(int(h), int(l.astype(np.int64)), int(e), b, d, rounding)
+                    for h, l, e in zip(hi, lo, es)])
+    if rounding:   # the range really is left: infinities, subnormals and zeros occur
+        assert np.isinf(ref).any() and (ref == 0).any()
+        assert ((ref != 0) & (np.abs(ref) < np.finfo(np.float64).tiny)).any()
+    bad = np.nonzero(out.view(np.int64) != ref.view(np.int64))[0]
+    assert bad.size == 0, [(vals[i], int(es[i]), out[i], ref[i]) for i in bad[:5]]
+    fits = np.array([-(1 << 63) <= v < (1 << 63) for v in vals])
+    bad64 = np.nonzero(fits & (out64.view(np.int64) != ref.view(np.int64)))[0]
+    assert bad64.size == 0, [(vals[i], int(es[i]), out64[i], ref[i]) for i in bad64[:5]]
