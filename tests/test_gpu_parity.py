@@ -75,6 +75,40 @@ def test_bins_bit_exact(gpu, b, d, kernel):
     assert np.array_equal(got.astype(np.int64), ref)
 
 
+def special_matrix(rng, m, n):
+    X = gen_inputs(rng, m, n, 1.0)
+    X[0, 3], X[1, 5], X[2, 7], X[3, 1] = np.inf, -np.inf, np.nan, np.finfo(np.float64).max
+    X[4, :] = np.inf
+    X[6, 2] = -np.nan
+    return X
+
+
+@pytest.mark.parametrize("b,d", [(7, 8), (7, 9), (3, 29)])
+def test_special_values_follow_the_reference_rule(gpu, b, d):
+    """Infinities and NaNs carry exponent field 2047 and go through the reference's split
+    and conversion like any other value (the reference has no special-value handling): the
+    GPU's exponents, slices and C equal the oracle's, bit for bit."""
+    import torch
+    rng = np.random.default_rng(40 + d)
+    A = special_matrix(rng, 70, 160)
+    B = np.ascontiguousarray(special_matrix(rng, 90, 160).T)   # K x N with specials in columns
+    E, S = oz.debug_split(dev(A), b, d, 1)
+    Eo = oracle.max_exp(A, 1)
+    assert np.array_equal(E, Eo.astype(np.int32))
+    assert np.array_equal(S, oracle.split(A, Eo, b, d, 1))
+    E, S = oz.debug_split(dev(B), b, d, 0)
+    Eo = oracle.max_exp(B, 0)
+    assert np.array_equal(E, Eo.astype(np.int32))
+    assert np.array_equal(S, oracle.split(B, Eo, b, d, 0))
+    p = oz.Plan(70, 90, 160, b, d)
+    C = p(dev(A), dev(B))
+    torch.cuda.synchronize()
+    ref, ov = oracle.ozaki_dgemm(A, B, b, d)
+    assert ov == 0
+    assert np.array_equal(bits(C.cpu().numpy()), bits(ref))
+    p.close()
+
+
 C_CASES = [(7, 8), (7, 3), (7, 7), (7, 9), (3, 29), (3, 26), (4, 18), (7, 12), (5, 19), (7, 1), (7, 2), (6, 4)]
 
 
