@@ -74,6 +74,8 @@ def parse():
                     help="only the reference's Mode.accurate section (prints its record as JSON)")
     ap.add_argument("--probe-only", action="store_true",
                     help="only the measured int8 MFMA ceiling (ozaie_probe_mfma), as one JSON line")
+    ap.add_argument("--power-only", action="store_true",
+                    help="only board power and GFX clocks (amd-smi) under sustained loads, as one JSON line")
     ap.add_argument("--profile-run", action="store_true",
                     help="only run each preset's GEMM twice and one 1 GiB device copy (for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -126,6 +128,87 @@ def vendor_int8_gemm(torch, n, steps=10):
         return out
     except Exception as e:  # reported beside value, never required
         return {"error": repr(e)}
+
+
+def amd_smi_sample():
+    """(socket power W, [GFX clocks MHz]) from one `amd-smi metric -p -c`, or None."""
+    import re
+    import subprocess
+    try:
+        out = subprocess.run(["amd-smi", "metric", "-p", "-c"], capture_output=True, text=True,
+                             timeout=30).stdout
+    except Exception:
+        return None
+    pw = re.findall(r"SOCKET_POWER:\s*([\d.]+)\s*W", out)
+    clk = re.findall(r"GFX_\d+:\s*\n\s*CLK:\s*(\d+)\s*MHz", out)
+    if not pw:
+        return None
+    return float(pw[0]), [int(c) for c in clk]
+
+
+def power_section(torch, np, oz, n, seconds=8.0):
+    """Board power against the card's cap while each kind of work runs back to back for
+    `seconds`: native DGEMM, the CRT mode (15 moduli), the split scheme (d = 8, 9) at n^3,
+    and the int8 MFMA probe with all-zero and with random operands (the same instructions).
+    amd-smi is sampled once a second after 3 s of warm-up; medians of socket power and of the
+    mean GFX clock.  Shows whether a kernel runs at the power limit."""
+    import statistics
+    import threading
+    A, B = gen_inputs(torch, n, 1.0, 12445, "cuda:0")
+    C = torch.empty(n, n, dtype=torch.float64, device="cuda:0")
+    loads = [("native_dgemm", lambda: torch.matmul(A, B, out=C))]
+    plans = []
+    for name, bits, d, crt in (("crt15", 0, 15, True), ("split_d8", 7, 8, False), ("split_d9", 7, 9, False)):
+        pl = oz.Plan(n, n, n, bits, d, rounding=oz.ROUND_NEAREST if crt else oz.ROUND_TRUNCATE, crt=crt)
+        plans.append(pl)
+        loads.append((name, (lambda pl=pl: pl(A, B, C))))
+    loads.append(("probe_zeros", lambda: oz.probe_mfma("zeros", iters=4000)))
+    loads.append(("probe_random", lambda: oz.probe_mfma("random", iters=4000)))
+    cap = None
+    try:
+        import re
+        import subprocess
+        out = subprocess.run(["rocm-smi", "--showmaxpower"], capture_output=True, text=True, timeout=30).stdout
+        m = re.search(r"Max Graphics Package Power \(W\):\s*([\d.]+)", out)
+        cap = float(m.group(1)) if m else None
+    except Exception:
+        cap = None
+    rec = {"power_cap_w": cap, "workload": f"{n}^3, phi=1", "loads": {}}
+    for name, fn in loads:
+        samples, stop = [], threading.Event()
+
+        def sampler():
+            if stop.wait(3.0):
+                return
+            while not stop.is_set():
+                s_ = amd_smi_sample()
+                if s_:
+                    samples.append(s_)
+                stop.wait(1.0)
+
+        th = threading.Thread(target=sampler)
+        th.start()
+        t0 = time.time()
+        calls = 0
+        while time.time() - t0 < seconds:
+            fn()
+            calls += 1
+            if calls % 8 == 0:
+                torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        stop.set()
+        th.join()
+        if samples:
+            rec["loads"][name] = {
+                "socket_power_w": statistics.median(p for p, _ in samples),
+                "gfx_clock_mhz": statistics.median(sum(c) / len(c) for _, c in samples if c) if any(c for _, c in samples) else None,
+                "samples": len(samples), "calls": calls}
+        else:
+            rec["loads"][name] = {"error": "amd-smi gave no power reading", "calls": calls}
+        print(f"  power {name}: {rec['loads'][name]}", file=sys.stderr, flush=True)
+    for pl in plans:
+        pl.close()
+    return rec
 
 
 def scaling_name(bits):
@@ -744,6 +827,9 @@ def main():
         return
     if args.accurate_only:
         print(json.dumps(accurate_section(args, torch, np, oz)), flush=True)
+        return
+    if args.power_only:
+        print(json.dumps(power_section(torch, np, oz, n)), flush=True)
         return
     if args.probe_only:
         rec = {}
