@@ -5,8 +5,8 @@ native FP64 DGEMM on the same GPU, at matched maximum relative error.
 Workload (one "step" = one C = A B over device-resident FP64 inputs):
   square DGEMM M = N = K = 8192 (--size), entries (U(0,1) - 0.5) exp(phi N(0,1)), phi = 1,
   seeded synthetic data; b = 7 split-scheme presets (the reference rule) with 3 (fast),
-  7 (accurate), 8 (the reference's fast mode) and 9 slices, and the CRT mode with 16
-  moduli (exact integer product of the scaled operands; no reference counterpart).
+  7 (accurate), 8 (the reference's fast mode) and 9 slices, and the CRT mode with 14, 15
+  and 16 moduli (exact integer product of the scaled operands; no reference counterpart).
 Errors are max / median |C - C*| / |C*| on a sampled 128 x 128 block of C against the
 correctly rounded exact product (libozaie's exact-entry meter).  `value` is the
 effective TFLOP/s (2 M N K / time) of the fastest preset whose maximum error is no
@@ -22,7 +22,9 @@ The JSON line also carries:
                 zero, constant and toggling random operands) and beside the vendor's int8
                 GEMM (hipBLASLt through torch._int_mm) on the same card and size;
                 traffic = HBM bytes per launch from the rocprofv3 FETCH/WRITE summaries in
-                profiles/ when present
+                profiles/ when present; board_power = socket power (amd-smi) and GFX clock
+                during 5 s of back-to-back calls of the matched preset and of native DGEMM,
+                beside the card's power cap
   cpu_baseline  the oracle's OpenMP restatement of the reference rule (b = 7, the matched
                 split preset) on a row sample of the same workload, on this host's cores;
                 the CRT-mode port, the host BLAS DGEMM and the reference's single-threaded
@@ -146,24 +148,29 @@ def amd_smi_sample():
     return float(pw[0]), [int(c) for c in clk]
 
 
-def power_section(torch, np, oz, n, seconds=8.0):
+POWER_PRESETS = (("crt15", 0, 15, True), ("split_d8", 7, 8, False), ("split_d9", 7, 9, False))
+
+
+def power_section(torch, np, oz, n, seconds=8.0, warm=3.0, presets=POWER_PRESETS, probes=True):
     """Board power against the card's cap while each kind of work runs back to back for
-    `seconds`: native DGEMM, the CRT mode (15 moduli), the split scheme (d = 8, 9) at n^3,
-    and the int8 MFMA probe with all-zero and with random operands (the same instructions).
-    amd-smi is sampled once a second after 3 s of warm-up; medians of socket power and of the
-    mean GFX clock.  Shows whether a kernel runs at the power limit."""
+    `seconds`: native DGEMM and the given emulated presets (name, bits, slices or moduli,
+    CRT) at n^3, and (probes) the int8 MFMA probe with all-zero and with random operands
+    (the same instructions).  amd-smi is sampled once a second after `warm` seconds; medians
+    of socket power and of the mean GFX clock.  Shows whether a kernel runs at the power
+    limit."""
     import statistics
     import threading
     A, B = gen_inputs(torch, n, 1.0, 12445, "cuda:0")
     C = torch.empty(n, n, dtype=torch.float64, device="cuda:0")
     loads = [("native_dgemm", lambda: torch.matmul(A, B, out=C))]
     plans = []
-    for name, bits, d, crt in (("crt15", 0, 15, True), ("split_d8", 7, 8, False), ("split_d9", 7, 9, False)):
+    for name, bits, d, crt in presets:
         pl = oz.Plan(n, n, n, bits, d, rounding=oz.ROUND_NEAREST if crt else oz.ROUND_TRUNCATE, crt=crt)
         plans.append(pl)
         loads.append((name, (lambda pl=pl: pl(A, B, C))))
-    loads.append(("probe_zeros", lambda: oz.probe_mfma("zeros", iters=4000)))
-    loads.append(("probe_random", lambda: oz.probe_mfma("random", iters=4000)))
+    if probes:
+        loads.append(("probe_zeros", lambda: oz.probe_mfma("zeros", iters=4000)))
+        loads.append(("probe_random", lambda: oz.probe_mfma("random", iters=4000)))
     cap = None
     try:
         import re
@@ -178,7 +185,7 @@ def power_section(torch, np, oz, n, seconds=8.0):
         samples, stop = [], threading.Event()
 
         def sampler():
-            if stop.wait(3.0):
+            if stop.wait(warm):
                 return
             while not stop.is_set():
                 s_ = amd_smi_sample()
@@ -208,6 +215,8 @@ def power_section(torch, np, oz, n, seconds=8.0):
         print(f"  power {name}: {rec['loads'][name]}", file=sys.stderr, flush=True)
     for pl in plans:
         pl.close()
+    del A, B, C
+    torch.cuda.empty_cache()
     return rec
 
 
@@ -857,6 +866,13 @@ def main():
     log(args, f"  int8 MFMA ceiling (TOP/s): {ceiling}")
     vendor = vendor_int8_gemm(torch, n)
     log(args, f"  vendor int8 GEMM (hipBLASLt, torch._int_mm) at {n}^3: {vendor}")
+    # board power of the matched preset and of native DGEMM under 5 s of back-to-back calls
+    try:
+        pname = f"crt{d}" if crt else f"split_d{d}"
+        power = power_section(torch, np, oz, n, seconds=5.0, warm=2.0,
+                              presets=((pname, m["bits"] if crt else 7, d, crt),), probes=False)
+    except Exception as e:  # reported beside value, never required
+        power = {"error": repr(e)}
     kname = (f"CRT modulus GEMM (k_crt_gemm_lds<4, 4>), {d} moduli" if crt else f"fused slice GEMM, d={d}")
     cfg = ({"scheme": "crt", "moduli": d, "bits": m["bits"], "scaling": m["scaling"], "rounding": "nearest"} if crt
            else {"scheme": "split", "bits": 7, "slices": d})
@@ -907,7 +923,7 @@ def main():
                      "measured_ceiling_constant": ceiling.get("constant"),
                      "measured_ceiling_zeros": ceiling.get("zeros"),
                      "frac_of_measured_ceiling": (achieved / ceiling["random"]) if ceiling.get("random") else None,
-                     "vendor_int8_gemm": vendor},
+                     "vendor_int8_gemm": vendor, "board_power": power},
     }
     try:
         line["host_pointer"] = host_pointer(np, oz, Ah, Bh, n, crt, d)
