@@ -85,6 +85,34 @@ def test_crt_C_bit_exact(gpu, M, N, K, n, phi, rounding):
     p.close()
 
 
+@pytest.mark.parametrize("n", [14, 17])
+def test_crt_modulus_gemm_forms_agree(gpu, n):
+    """Every form of the modulus GEMM -- 256 x 256 tiles (OZAIE_OPT_KERNEL 2), per-wave loads
+    (3), 128 x 128 tiles (4) and the size rule (0) -- gives the same C on a shape that is no
+    multiple of either tile, equal to the oracle on sampled rows and columns."""
+    import torch
+    rng = np.random.default_rng(60 + n)
+    M, N, K = 700, 900, 1000
+    A = gen_inputs(rng, M, K, 1.0)
+    B = gen_inputs(rng, K, N, 1.0)
+    dA, dB = dev(A), dev(B)
+    out = {}
+    for kernel in (0, 2, 3, 4):
+        oz.set_option(oz.OPT_KERNEL, kernel)
+        try:
+            p = oz.Plan(M, N, K, 0, n, rounding=oz.ROUND_NEAREST, crt=True)
+        finally:
+            oz.set_option(oz.OPT_KERNEL, 0)
+        out[kernel] = p(dA, dB).cpu().numpy()
+        p.close()
+    for kernel in (2, 3, 4):
+        assert np.array_equal(bits(out[kernel]), bits(out[0])), kernel
+    rows = np.array([0, 1, 255, 256, 511, M - 1])
+    cols = np.array([0, 127, 128, 255, 256, 640, N - 1])
+    ref = oracle.crt_dgemm(A[rows], np.ascontiguousarray(B[:, cols]), n, oz.ROUND_NEAREST)
+    assert np.array_equal(bits(out[0][np.ix_(rows, cols)]), bits(ref))
+
+
 @pytest.mark.parametrize("M,N,K,n", [(130, 67, 300, 16), (64, 128, 100, 16), (33, 36, 90, 20)])
 def test_crt_C_into_strided_view(gpu, M, N, K, n):
     """C written through a view that starts 8 bytes past a 16-byte boundary, with a row
