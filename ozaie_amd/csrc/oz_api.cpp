@@ -803,6 +803,7 @@ int32_t ozaie_dgemm_noalloc(int64_t M, int64_t N, int64_t K, int bits, int slice
         return OZAIE_ERR_NOMEM;
     }
     HIPCHK(hipGetDevice(&p.device));
+    HIPCHK(hipDeviceGetAttribute(&p.cus, hipDeviceAttributeMultiprocessorCount, p.device));
     char *ws = nullptr;
     rc = noalloc_workspace(p.device, &ws);
     if (rc) return rc;
