@@ -259,6 +259,26 @@ def test_long_K_needs_chunks(gpu):
     assert np.array_equal(bits(C.cpu().numpy()), bits(ref))
 
 
+@pytest.mark.parametrize("d", [8, 9])
+def test_very_long_K_many_passes(gpu, d):
+    """K = 100000: six (d = 8) or seven (d = 9) contraction passes accumulated into the exact
+    128-bit T, over a wide exponent range; C equals the oracle's at the planner's chunk."""
+    import torch
+    rng = np.random.default_rng(30 + d)
+    M, N, K = 12, 10, 100000
+    A = gen_inputs(rng, M, K, 2.0)
+    B = gen_inputs(rng, K, N, 2.0)
+    p = oz.Plan(M, N, K, 7, d)
+    info = p.info()
+    assert info["passes"] >= 6
+    C = p(dev(A), dev(B))
+    torch.cuda.synchronize()
+    ref, ov = oracle.ozaki_dgemm(A, B, 7, d, kchunk=info["kchunk"])
+    assert ov == 0
+    assert np.array_equal(bits(C.cpu().numpy()), bits(ref))
+    p.close()
+
+
 def test_transposed_operands(gpu):
     import torch
     rng = np.random.default_rng(23)
