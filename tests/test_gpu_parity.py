@@ -296,6 +296,42 @@ def test_transposed_operands(gpu):
             assert np.array_equal(bits(C.cpu().numpy()), bits(ref)), (ta, tb)
 
 
+@pytest.mark.parametrize("scheme", ["split", "crt"])
+def test_padded_leading_dimensions(gpu, scheme):
+    """A, B and C as views into wider buffers (leading dimensions above their row lengths,
+    not multiples of 16 bytes), plain and transposed: the same C as from contiguous operands,
+    equal to the oracle, and nothing outside C's view written."""
+    import torch
+    rng = np.random.default_rng(25)
+    M, N, K = 70, 90, 333
+    A = gen_inputs(rng, M, K, 1.0)
+    B = gen_inputs(rng, K, N, 1.0)
+    if scheme == "split":
+        p = oz.Plan(M, N, K, 7, 9)
+        ref, _ = oracle.ozaki_dgemm(A, B, 7, 9)
+    else:
+        p = oz.Plan(M, N, K, 0, 15, rounding=oz.ROUND_NEAREST, crt=True)
+        ref = oracle.crt_dgemm(A, B, 15, oz.ROUND_NEAREST)
+
+    def padded(X, extra):
+        buf = torch.full((X.shape[0], X.shape[1] + extra), float("nan"), dtype=torch.float64, device="cuda")
+        buf[:, :X.shape[1]] = torch.from_numpy(np.ascontiguousarray(X)).to("cuda")
+        return buf[:, :X.shape[1]]
+
+    for ta in (False, True):
+        for tb in (False, True):
+            a = padded(A.T, 3) if ta else padded(A, 5)
+            b = padded(B.T, 7) if tb else padded(B, 1)
+            cbuf = torch.full((M, N + 9), -7.0, dtype=torch.float64, device="cuda")
+            C = cbuf[:, :N]
+            assert a.stride(0) != a.shape[1] and C.stride(0) == N + 9
+            p(a, b, C, transA=ta, transB=tb)
+            torch.cuda.synchronize()
+            assert np.array_equal(bits(C.cpu().numpy()), bits(ref)), (ta, tb)
+            assert bool((cbuf[:, N:] == -7.0).all()), (ta, tb)
+    p.close()
+
+
 def test_paired_wire_format(gpu):
     rng = np.random.default_rng(24)
     for K, b, d in [(128, 3, 29), (256, 7, 8), (2048, 3, 26)]:
