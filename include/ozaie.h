@@ -80,7 +80,9 @@ int32_t ozAIE_setup_slices(int M, int N, int K, int bits, int slices);
 
 /* Replaces oz_aie::ozAIE_DGEMM (include/oz_aie.h:311-349 of the reference).
  * C = A * B, row-major, contiguous, dimensions must equal the last setup.  A, B and C
- * may be host or device pointers.  Blocking.                                          */
+ * may be host or device pointers.  Blocking.  Host arrays are copied through two pinned
+ * 32 MiB chunks that the global plan allocates at its first host-pointer call and holds
+ * until the next setup or ozaie_release.                                              */
 int32_t ozAIE_DGEMM(int M, int N, int K, const double *A, const double *B, double *C);
 
 /* Set / get a library option (OZAIE_OPT_*).  Applies to plans set up afterwards.       */
