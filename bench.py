@@ -770,12 +770,14 @@ def autograd_section(args, torch, np, oz, n=2048, slices=9):
 
 
 def accurate_section(args, torch, np, oz):
-    """The reference's own default mode, Mode.accurate (b = 3, d = 29 .. 25 by log2 K,
-    include/oz_aie.h:60-70), on the repository example's shape 256 x 512 x 128 (phi = 0.5)
-    and on squares 2048^3 and 4096^3 (phi = 1): time, effective TFLOP/s, speed against
-    native DGEMM, error on a sampled block against the exact product."""
+    """The reference's own default mode, Mode.accurate (b = 3, d = 29 .. 25 by log2 K, and
+    b = 4, d = 18 from K = 2^14 on, include/oz_aie.h:60-70), on the repository example's
+    shape 256 x 512 x 128 (phi = 0.5), on squares 2048^3 and 4096^3 and on 4096 x 4096 x
+    16384 (phi = 1): time, effective TFLOP/s, speed against native DGEMM, error on a sampled
+    block against the exact product."""
     out = []
-    for (M, N, K, phi) in ((256, 512, 128, 0.5), (2048, 2048, 2048, 1.0), (4096, 4096, 4096, 1.0)):
+    for (M, N, K, phi) in ((256, 512, 128, 0.5), (2048, 2048, 2048, 1.0), (4096, 4096, 4096, 1.0),
+                           (4096, 4096, 16384, 1.0)):
         g = torch.Generator(device="cuda:0")
         g.manual_seed(args.seed)
         A = (torch.rand(M, K, dtype=torch.float64, device="cuda:0", generator=g) - 0.5) * torch.exp(

@@ -43,7 +43,7 @@ ROUND_NEAREST = 1
 
 
 # kernel ids of ozaie_plan_info / ozaie_plan_query (out[4])
-KERNEL_NAMES = ("fused", "group", "crt", "pairs")
+KERNEL_NAMES = ("fused", "group", "crt", "pairs", "window")
 
 
 class Mode(enum.IntEnum):

@@ -48,6 +48,9 @@ int set_hip_error(hipError_t e, const char *where) {
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
+// Bins per pass of the multi-pass kernels: bin windows (4) and bin groups (1).
+constexpr int kPassBins(int kernel) { return kernel == 4 ? kWindowBins : kGroupBins; }
+
 // Makes `dev` current for the guard's lifetime and restores the caller's device on every
 // return path.
 struct DeviceGuard {
@@ -290,9 +293,11 @@ int plan_configure(ozaie_plan p, int64_t M, int64_t N, int64_t K, int b, int d, 
     if (g_opt_kchunk > 0) kc = std::min<int64_t>(kc, std::max<int64_t>(2 * kBlk, g_opt_kchunk / (2 * kBlk) * (2 * kBlk)));
     p->kcb = std::max<int64_t>(2, kc / kBlk / 2 * 2);
     p->nchunks = (int)((p->Kb + p->kcb - 1) / p->kcb);
-    p->kernel = (fused_supported(d) && g_opt_kernel != 1) ? 0 : 1;
+    // 0: fused (all bins in registers), 4: bin windows of the rolling kernel (d = 18),
+    // 1: bin groups of 8 (any d; also on request)
+    p->kernel = g_opt_kernel == 1 ? 1 : (fused_supported(d) ? 0 : (window_supported(d) ? 4 : 1));
     p->family = g_opt_kernel >= 2 ? g_opt_kernel : KERNEL_AUTO;
-    p->ngroups = p->kernel == 0 ? 1 : (d + kGroupBins - 1) / kGroupBins;
+    p->ngroups = p->kernel == 0 ? 1 : (d + kPassBins(p->kernel) - 1) / kPassBins(p->kernel);
     if (allow_pairs && g_opt_kernel != 1 && b <= 3 && d >= kPairMinSlices) {
         const PairCfg pc = pair_cfg(b, d);
         const int64_t m1 = (1 << b) - 1, m2 = (1 << (2 * b)) - 1;
@@ -479,10 +484,11 @@ int run_passes(ozaie_plan p, double *C, int64_t ldc, int epi, int32_t *bins, hip
         g.kb0 = (int64_t)c * p->kcb;
         g.kb1 = std::min<int64_t>(p->Kb, g.kb0 + p->kcb);
         for (int gi = 0; gi < p->ngroups; gi++, pass++) {
-            g.k0 = gi * kGroupBins;
+            g.k0 = gi * kPassBins(p->kernel);
             g.first = pass == 0;
             g.last = pass == npass - 1;
             if (p->kernel == 1) HIPCHK(launch_gemm_group(g, epi, st));
+            else if (p->kernel == 4) HIPCHK(launch_gemm_window(g, epi, st));
             else HIPCHK(launch_gemm_fused(g, epi, st));   // 0 fused, 3 paired digits (fused over P pair planes)
         }
     }

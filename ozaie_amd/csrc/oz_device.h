@@ -656,23 +656,42 @@ __device__ __forceinline__ void gemm_lds_tile(const GemmLaunch &g, int tm, int t
 }
 
 // Rolling variant (any d): the two LDS stages and copies of the LDS-copy kernel, but a
-// single fragment set, refilled as it dies.  The LDS-copy kernel reads all d (WM + WN) fragments of a block
-// right after its barrier, so every wave waits for that burst of LDS reads before its first
-// MFMA.  Here the fragments of block b + 1 are read during block b's triangle, each into a
-// fragment of block b right after that fragment's last use:
-//   rows i = 0 .. d-1 in turn, and in row i the columns j = d-1-i down to 0.  A_i dies at the
-//   end of row i; B_j is last used at the start of row d-1-j (later rows have smaller j).
-// The next block begins with row 0 at column d-1, so the latest-read fragment (B_0, read
-// after block b's last MFMA) is needed only after the d-1 other columns of row 0: at least
-// (d-1) WM WN MFMAs of slack (14 for d = 8 on 1 x 2 wave tiles), which covers an LDS read.
+// single fragment set, refilled as it dies.  The LDS-copy kernel reads all d (WM + WN)
+// fragments of a block right after its barrier, so every wave waits for that burst of LDS
+// reads before its first MFMA.  Here the fragments of block b + 1 are read during block b's
+// products, each into a fragment of block b right after that fragment's last use.
+// Bin window [K0, K0 + NB): the products s_i * s_j with K0 <= i + j < K0 + NB (i, j < S =
+// min(D, K0 + NB), the slices that reach the window) into NB bins; the whole triangle is
+// the window [0, D).  Products of a d > 16 split, whose d bins do not fit in registers, run
+// as several windows, each a pass that adds its bins into the exact T.
+//   Order: rows i = 0 .. S-1 in turn, and in row i the columns j from the window's top down.
+//   A_i dies at the end of row i; B_j at its use in row r_j = min(S-1, K0+NB-1-j), the last
+//   row that reaches it (the first column of that row unless r_j = S-1).  Both are refilled
+//   right there.  The next block begins with row 0 at its top column, whose B was refilled
+//   at the start of the previous block; the latest-read fragment is needed only after the
+//   other columns of row 0 (or, in a window that starts above 0, several rows later): at
+//   least (d-1) WM WN MFMAs of slack for a whole triangle (14 for d = 8 on 1 x 2 wave
+//   tiles), which covers an LDS read.
 //   step b: wait for block b + 1's copies, barrier (every wave has read block b, whose
 //           fragments were read during step b - 1), copy block b + 2 into block b's stage,
-//           then the triangle of block b with the rolling reads of block b + 1.
+//           then the products of block b with the rolling reads of block b + 1.
 // The last step reads a stale stage into fragments that are never multiplied.
-template <int D, int WM, int WN>
-__device__ __forceinline__ void gemm_roll_tile(const GemmLaunch &g, int tm, int tn, int epi, int8_t *smem) {
-    using F = FusedCfg<D, WM, WN>;
+template <int D, int K0, int NB, int WM, int WN>
+__host__ __device__ constexpr int window_slices() { return D < K0 + NB ? D : K0 + NB; }
+
+template <int D, int K0, int NB, int WM, int WN>
+__host__ __device__ constexpr int window_tile_bytes() {
+    return 2 * FusedCfg<window_slices<D, K0, NB, WM, WN>(), WM, WN>::STAGE;
+}
+
+template <int D, int K0, int NB, int WM, int WN>
+__device__ __forceinline__ void gemm_window_tile(const GemmLaunch &g, int tm, int tn, int epi, int8_t *smem) {
+    constexpr int S = window_slices<D, K0, NB, WM, WN>();
+    static_assert(K0 < D && NB >= 1, "the window starts at a bin of the product");
+    using F = FusedCfg<S, WM, WN>;
     constexpr int BA = F::BA, BB = F::BB, NBLK = F::NBLK, NLD = F::NLD, NMIN = F::NMIN;
+    static_assert(window_tile_bytes<D, K0, NB, WM, WN>() >= 4 * 4 * wave_scratch_words<NB, WM, WN>(),
+                  "epilogue scratch");
 
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -689,11 +708,11 @@ __device__ __forceinline__ void gemm_roll_tile(const GemmLaunch &g, int tm, int 
     for (int it = 0; it < NLD; it++) {
         const int blk = w + 4 * it;
         const int bc = blk < NBLK ? blk : NBLK - 1;
-        if (bc < D * BA) {
+        if (bc < S * BA) {
             const int j = bc / BA, mb = bc % BA;
             src[it] = g.SA + j * planeA + (rbA0 + mb) * g.Kb * kBlkBytes;
         } else {
-            const int bb = bc - D * BA;
+            const int bb = bc - S * BA;
             const int j = bb / BB, nb = bb % BB;
             src[it] = g.SB + j * planeB + (rbB0 + nb) * g.Kb * kBlkBytes;
         }
@@ -716,25 +735,25 @@ __device__ __forceinline__ void gemm_roll_tile(const GemmLaunch &g, int tm, int 
     auto readB = [&](v4i (&b)[WN], const int8_t *st, int j) {
 #pragma unroll
         for (int ni = 0; ni < WN; ni++)
-            b[ni] = *reinterpret_cast<const v4i *>(st + (D * BA + j * BB + wn * WN + ni) * kBlkBytes + lane * 16);
+            b[ni] = *reinterpret_cast<const v4i *>(st + (S * BA + j * BB + wn * WN + ni) * kBlkBytes + lane * 16);
     };
 
-    v16i acc[D][WM][WN];
+    v16i acc[NB][WM][WN];
 #pragma unroll
-    for (int k = 0; k < D; k++)
+    for (int k = 0; k < NB; k++)
 #pragma unroll
         for (int mi = 0; mi < WM; mi++)
 #pragma unroll
             for (int ni = 0; ni < WN; ni++) acc[k][mi][ni] = v16i{};
 
     const int64_t kb0 = g.kb0, nk = g.kb1 - g.kb0;
-    v4i af[D][WM], bf[D][WN];
+    v4i af[S][WM], bf[S][WN];
     if (nk > 0) {
         issue(0, kb0);
         s_waitcnt_vm<0>();
         lds_barrier();
 #pragma unroll
-        for (int j = 0; j < D; j++) {
+        for (int j = 0; j < S; j++) {
             readA(af[j], smem, j);
             readB(bf[j], smem, j);
         }
@@ -747,17 +766,18 @@ __device__ __forceinline__ void gemm_roll_tile(const GemmLaunch &g, int tm, int 
         if (b + 2 < nk) issue((int)(b & 1), kb0 + b + 2);
         const int8_t *nx = smem + (int)((b + 1) & 1) * F::STAGE;
 #pragma unroll
-        for (int i = 0; i < D; i++) {
+        for (int i = 0; i < S; i++) {
+            constexpr int kTop = K0 + NB - 1;
 #pragma unroll
-            for (int jj = 0; jj < D - i; jj++) {
-                const int j = D - 1 - i - jj;
+            for (int j = S - 1; j >= 0; j--) {
+                if (i + j < K0 || i + j > kTop) continue;   // outside the window (compile time)
 #pragma unroll
                 for (int mi = 0; mi < WM; mi++)
 #pragma unroll
                     for (int ni = 0; ni < WN; ni++)
-                        acc[i + j][mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i][mi], bf[j][ni],
-                                                                                   acc[i + j][mi][ni], 0, 0, 0);
-                if (jj == 0) {   // the last use of B_j in this block
+                        acc[i + j - K0][mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i][mi], bf[j][ni],
+                                                                                        acc[i + j - K0][mi][ni], 0, 0, 0);
+                if (i == (S - 1 < kTop - j ? S - 1 : kTop - j)) {   // the last use of B_j in this block
                     __builtin_amdgcn_sched_barrier(0);
                     readB(bf[j], nx, j);
                     __builtin_amdgcn_sched_barrier(0);
@@ -770,8 +790,13 @@ __device__ __forceinline__ void gemm_roll_tile(const GemmLaunch &g, int tm, int 
     }
     __syncthreads();
 
-    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * wave_scratch_words<D, WM, WN>();
-    wave_epilogue<D, WM, WN>(g, epi, 0, acc, scratch, lane, rbase, cbase, er_v, ec_v);
+    int32_t *scratch = reinterpret_cast<int32_t *>(smem) + w * wave_scratch_words<NB, WM, WN>();
+    wave_epilogue<NB, WM, WN>(g, epi, K0, acc, scratch, lane, rbase, cbase, er_v, ec_v);
+}
+
+template <int D, int WM, int WN>
+__device__ __forceinline__ void gemm_roll_tile(const GemmLaunch &g, int tm, int tn, int epi, int8_t *smem) {
+    gemm_window_tile<D, 0, D, WM, WN>(g, tm, tn, epi, smem);
 }
 
 // Staged variant (d <= 4 on a 128 x 128 workgroup tile, d = 9 on 64 x 64): the slice blocks go HBM -> LDS

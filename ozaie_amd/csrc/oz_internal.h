@@ -290,6 +290,11 @@ hipError_t launch_gemm_fused(const GemmLaunch &g, int epi, hipStream_t st);
 // Bin-group kernel for any d <= kMaxSlices: bins [g.k0, g.k0 + 8).
 constexpr int kGroupBins = 8;
 hipError_t launch_gemm_group(const GemmLaunch &g, int epi, hipStream_t st);
+// Bin-window kernel for the d > 16 splits it is built for (d = 18): bins [g.k0, g.k0 + 9)
+// per pass, g.k0 a multiple of kWindowBins.
+bool window_supported(int d);
+constexpr int kWindowBins = 9;
+hipError_t launch_gemm_window(const GemmLaunch &g, int epi, hipStream_t st);
 
 enum Epi { EPI_C = 0, EPI_BINS = 1 };
 

@@ -221,6 +221,16 @@ void k_gemm_roll(GemmLaunch g, int epi) {
     gemm_roll_tile<D, WM, WN>(g, tm, tn, epi, smem);
 }
 
+template <int D, int K0, int NB, int WM, int WN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_gemm_window(GemmLaunch g, int epi) {
+    const int ntm = (g.M + 64 * WM - 1) / (64 * WM), ntn = (g.N + 64 * WN - 1) / (64 * WN);
+    int tm, tn;
+    tile_map(blockIdx.x, ntm, ntn, tm, tn);
+    __shared__ __attribute__((aligned(1024))) int8_t smem[window_tile_bytes<D, K0, NB, WM, WN>()];
+    gemm_window_tile<D, K0, NB, WM, WN>(g, tm, tn, epi, smem);
+}
+
 // Pipelined LDS variant: the slice blocks of block t+2 are copied HBM -> LDS
 // (global_load_lds) while block t computes; fragments are double-buffered in registers
 // and the workgroup barrier sits in the middle of each block's MFMAs.  After the
@@ -431,6 +441,26 @@ hipError_t launch_gemm_fused(const GemmLaunch &g, int epi, hipStream_t st) {
         case 16: return launch_fused_t<16, 1, 1>(g, epi, st);
         default: return hipErrorInvalidValue;
     }
+}
+
+// The products of a d = 18 split (the reference's accurate preset for K >= 2^14: b = 4,
+// whose 4-bit digits do not pair into an int8 operand) as two bin windows of 9, each a
+// pass of the rolling kernel on 32 x 32 wave tiles: 9 bins (144 accumulator registers) and
+// one fragment set of up to 18 A and 18 B fragments (144 registers), two LDS stages of
+// 18 x 4 KiB.  The bin-group kernel needs three passes of 8 bins.
+bool window_supported(int d) { return d == 18; }
+
+hipError_t launch_gemm_window(const GemmLaunch &g, int epi, hipStream_t st) {
+    if (g.d != 18 || (g.k0 != 0 && g.k0 != kWindowBins)) return hipErrorInvalidValue;
+    const int64_t tiles = (int64_t)((g.M + 63) / 64) * ((g.N + 63) / 64);
+    if (tiles <= 0) return hipSuccess;
+    if (tiles > 0x7fffffff) return hipErrorInvalidValue;
+    if (g.k0 == 0)
+        hipLaunchKernelGGL((k_gemm_window<18, 0, kWindowBins, 1, 1>), dim3((unsigned)tiles), dim3(256), 0, st, g, epi);
+    else
+        hipLaunchKernelGGL((k_gemm_window<18, kWindowBins, kWindowBins, 1, 1>), dim3((unsigned)tiles), dim3(256), 0,
+                           st, g, epi);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ bin-group GEMM

@@ -59,7 +59,7 @@ BIN_CASES = [(7, 8, 0), (7, 3, 0), (7, 7, 0), (7, 9, 0), (7, 1, 0), (7, 4, 0), (
              (3, 29, 1), (3, 26, 1), (7, 8, 1), (4, 18, 1),
              (7, 8, 2), (7, 3, 2), (7, 9, 2), (7, 8, 3), (7, 3, 3), (7, 5, 3), (7, 8, 4), (7, 3, 4),
              (7, 7, 4), (6, 12, 4), (7, 1, 5), (7, 2, 5), (7, 3, 5), (7, 4, 5), (7, 8, 5),
-             (7, 9, 3), (7, 9, 5), (7, 9, 6), (7, 8, 6), (7, 3, 6), (6, 12, 6)]
+             (7, 9, 3), (7, 9, 5), (7, 9, 6), (7, 8, 6), (7, 3, 6), (6, 12, 6), (4, 18, 0)]
 
 
 @pytest.mark.parametrize("b,d,kernel", BIN_CASES)
@@ -227,7 +227,7 @@ def test_multi_pass_kchunk_and_groups(gpu):
     M, N, K = 70, 90, 1000
     A = gen_inputs(rng, M, K, 1.0)
     B = gen_inputs(rng, K, N, 1.0)
-    for b, d, kernel in [(7, 8, 0), (3, 29, 1), (7, 8, 1), (7, 3, 5), (7, 9, 3), (7, 9, 5), (7, 9, 6)]:
+    for b, d, kernel in [(7, 8, 0), (3, 29, 1), (7, 8, 1), (7, 3, 5), (7, 9, 3), (7, 9, 5), (7, 9, 6), (4, 18, 0)]:
         oz.set_option(oz.OPT_KCHUNK, 128)
         oz.set_option(oz.OPT_KERNEL, kernel)
         try:
@@ -241,6 +241,39 @@ def test_multi_pass_kchunk_and_groups(gpu):
             oz.set_option(oz.OPT_KERNEL, 0)
         ref, _ = oracle.ozaki_dgemm(A, B, b, d)
         assert np.array_equal(bits(C.cpu().numpy()), bits(ref))
+
+
+def test_window_kernel_matches_bin_groups(gpu):
+    """The reference's accurate preset for K >= 2^14 (b = 4, d = 18: 4-bit digits do not pair)
+    runs as two bin windows of the rolling kernel; C equals the three-pass bin-group kernel's
+    and the oracle's, bit for bit, in both roundings."""
+    import torch
+    rng = np.random.default_rng(26)
+    M, N, K = 130, 140, 2000
+    A = gen_inputs(rng, M, K, 1.0)
+    B = gen_inputs(rng, K, N, 1.0)
+    for rounding in (oz.ROUND_TRUNCATE, oz.ROUND_NEAREST):
+        p = oz.Plan(M, N, K, 4, 18, rounding=rounding)
+        info = p.info()
+        assert info["kernel"] == "window" and info["passes"] == 2
+        C = p(dev(A), dev(B)).cpu().numpy()
+        p.close()
+        oz.set_option(oz.OPT_KERNEL, 1)
+        try:
+            q = oz.Plan(M, N, K, 4, 18, rounding=rounding)
+        finally:
+            oz.set_option(oz.OPT_KERNEL, 0)
+        assert q.info()["kernel"] == "group" and q.info()["passes"] == 3
+        G = q(dev(A), dev(B)).cpu().numpy()
+        q.close()
+        assert np.array_equal(bits(C), bits(G))
+        if rounding == oz.ROUND_TRUNCATE:
+            ref, ov = oracle.ozaki_dgemm(A, B, 4, 18)
+            assert ov == 0
+            assert np.array_equal(bits(C), bits(ref))
+    assert oz.ozAIE_setup(64, 64, 1 << 14, oz.Mode.accurate) == oz.OK
+    assert oz.plan_info()["kernel"] == "window"
+    oz.release()
 
 
 def test_long_K_needs_chunks(gpu):
