@@ -93,7 +93,8 @@ int32_t ozaie_get_option(int option);
  * out[0]=b out[1]=d out[2]=k-chunk out[3]=passes out[4]=kernel (0 fused, 1 group,
  * 2 CRT mode: then b = the max rule's bits per operand (0: the norm rule) and d = number
  * of moduli, 3 paired digits: b <= 3
- * with d >= 8, adjacent digits paired into one int8 digit, the same C)                  */
+ * with d >= 8, adjacent digits paired into one int8 digit, the same C, 4 bin windows:
+ * d = 18 as two passes of 9 bins on the rolling fused kernel)                          */
 int32_t ozaie_plan_info(int32_t *out5);
 
 /* Release every device buffer and plan (reference: cleanup_xrt, include/oz_aie.h:153). */
@@ -127,7 +128,7 @@ int32_t ozaie_dgemm(ozaie_plan plan, int transA, int transB, const double *A, in
 
 /* Milliseconds of the last ozaie_dgemm on `plan` (needs OZAIE_OPT_PHASE_TIMING = 1 when
  * the plan was created): out[0] exponent max + split, out[1] slice GEMM passes (the
- * fused GEMM kernel, or all bin-group passes; in CRT mode the modulus GEMM kernel
+ * fused GEMM kernel, or all bin-group or bin-window passes; in CRT mode the modulus GEMM kernel
  * alone), out[2] whole call (CRT: including the rebuild).  Synchronizes on the events
  * of that call.                                                                        */
 int32_t ozaie_plan_phase_times(ozaie_plan plan, float *out3);
