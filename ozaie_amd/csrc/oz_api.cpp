@@ -469,7 +469,7 @@ int run_passes(ozaie_plan p, double *C, int64_t ldc, int epi, int32_t *bins, hip
         const PairCfg pc = pair_cfg(p->b, p->d);
         const int64_t planeA = p->Mp / kBlk * p->Kb * kBlkBytes, planeB = p->Np / kBlk * p->Kb * kBlkBytes;
         HIPCHK(launch_gemm_antidiag(p->SA + pc.P * planeA, p->SB + pc.P * planeB, p->Mp, p->Np, pc.Q * p->Kb, p->X,
-                                    p->Np, st));
+                                    p->Np, p->cus, st));
         g.b = 2 * p->b;
         g.d = pc.P;
         g.cb = p->b;

@@ -391,9 +391,6 @@ void k_crt_gemm(CrtGemm g, CrtConsts c) {
                                   g.Np);
 }
 
-// Waves per SIMD the tile shape leaves room for: the 256 x 256 tile's 16 accumulator tiles
-// take half the registers of one wave, the smaller ones fit two and four.
-constexpr int crt_occupancy(int tiles) { return tiles >= 8 ? 1 : (tiles >= 2 ? 2 : 4); }
 
 template <int WM, int WN>
 __global__ __launch_bounds__(256)

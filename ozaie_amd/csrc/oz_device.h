@@ -70,9 +70,11 @@ constexpr int kSplitLds = 128 * 33 * 8;  // bytes of the FP64 staging tile (eith
 // digit j is the b-bit field of Q at bit b(d-1-j): one bit-field extract per digit.  Four
 // digits share a word; the negative elements' bytes are negated without carries between
 // bytes (every digit is below 128, so 0x80 - x fits a byte: -x = (0x80 - x) ^ 0x80).
-// Wider strings shift the mantissa per digit.
+// Wider strings shift the mantissa per digit.  Only planes j0 <= j < j1 are written.
 __device__ __forceinline__ void emit_digits16(uint64_t (&mx)[16], const int (&bias)[16], uint32_t negm, int b,
-                                              int d, int8_t *__restrict__ dst, int64_t plane) {
+                                              int d, int8_t *__restrict__ dst, int64_t plane, int j0 = 0,
+                                              int j1 = 1 << 30) {
+    if (j1 > d) j1 = d;
     uint32_t nbm[4];
 #pragma unroll
     for (int gi = 0; gi < 4; gi++) {
@@ -102,19 +104,19 @@ __device__ __forceinline__ void emit_digits16(uint64_t (&mx)[16], const int (&bi
             }
             *reinterpret_cast<uint4 *>(dst + (int64_t)j * plane) = make_uint4(w[0], w[1], w[2], w[3]);
         };
-        int j = 0;
-        for (; j < d && b * (d - 1 - j) >= 32; j++) {   // field inside the high word
+        int j = j0;
+        for (; j < j1 && b * (d - 1 - j) >= 32; j++) {   // field inside the high word
             const uint32_t off = (uint32_t)(b * (d - 1 - j) - 32);
             plane_out(j, [&](int i) { return __builtin_amdgcn_ubfe(qhi[i], off, (uint32_t)b); });
         }
-        for (; j < d; j++) {                              // field at bit < 32 (may straddle)
+        for (; j < j1; j++) {                             // field at bit < 32 (may straddle)
             const uint32_t sj = (uint32_t)(b * (d - 1 - j));
             plane_out(j, [&](int i) { return __builtin_amdgcn_alignbit(qhi[i], qlo[i], sj) & mask; });
         }
         return;
     }
     const uint64_t mask = (1ull << b) - 1ull;
-    for (int j = 0; j < d; j++) {
+    for (int j = j0; j < j1; j++) {
         uint32_t w[4];
 #pragma unroll
         for (int gi = 0; gi < 4; gi++) {
@@ -144,10 +146,12 @@ __device__ __forceinline__ void emit_digits16(uint64_t (&mx)[16], const int (&bi
 // (d even: its low digit s_2I+1).  dstS: the lane's chunk in pair plane 0 (planes planeS
 // apart); dstX: its chunk in single-digit slot 0 (slots planeX apart, written in reverse
 // order when reverse_q).  Every field is below 64, so the carry-free byte negation of
-// emit_digits16 applies.
+// emit_digits16 applies.  Only planes g0 <= g < g1 of the P pair planes followed by the Q
+// single-digit slots are written.
 __device__ __forceinline__ void emit_pairs16(const uint64_t (&mx)[16], const int (&bias)[16], uint32_t negm,
                                              int b, int d, int8_t *__restrict__ dstS, int64_t planeS,
-                                             int8_t *__restrict__ dstX, int64_t planeX, bool reverse_q) {
+                                             int8_t *__restrict__ dstX, int64_t planeX, bool reverse_q,
+                                             int g0 = 0, int g1 = 1 << 30) {
     const PairCfg pc = pair_cfg(b, d);
     const int D = (d + 1) / 2;
     uint32_t nbm[4];
@@ -181,8 +185,10 @@ __device__ __forceinline__ void emit_pairs16(const uint64_t (&mx)[16], const int
         }
         *reinterpret_cast<uint4 *>(dst) = make_uint4(w[0], w[1], w[2], w[3]);
     };
-    for (int j = 0; j < pc.P; j++) plane_out(dstS + (int64_t)j * planeS, 2 * b * (D - 1 - j), (1u << (2 * b)) - 1u);
-    for (int I = 0; I < pc.Q; I++) {
+    const int jend = g1 < pc.P ? g1 : pc.P;
+    for (int j = g0; j < jend; j++) plane_out(dstS + (int64_t)j * planeS, 2 * b * (D - 1 - j), (1u << (2 * b)) - 1u);
+    const int iend = g1 - pc.P < pc.Q ? g1 - pc.P : pc.Q;
+    for (int I = g0 > pc.P ? g0 - pc.P : 0; I < iend; I++) {
         const int slot = reverse_q ? pc.Q - 1 - I : I;
         plane_out(dstX + (int64_t)slot * planeX, 2 * b * (D - 1 - I) + (pc.odd ? b : 0), (1u << b) - 1u);
     }
@@ -227,7 +233,7 @@ __device__ __forceinline__ void split_stage(const double (&v)[TK / 8], double *t
 template <bool KC, int TK = 128>
 __device__ __forceinline__ void split_emit(const double *tile, int64_t R, const int32_t *__restrict__ Erow, int b,
                                            int d, int8_t *__restrict__ S, int64_t Rb, int64_t Kb, int64_t rb,
-                                           int64_t kt) {
+                                           int64_t kt, int j0 = 0, int j1 = 1 << 30) {
     constexpr int LDW = KC ? (TK + 1) : (32 + 1);
     const int t = threadIdx.x;
     const int64_t r0 = rb * 32;
@@ -252,16 +258,17 @@ __device__ __forceinline__ void split_emit(const double *tile, int64_t R, const 
             negm |= (uint32_t)(u >> 63) << i;
         }
         int8_t *dst = S + (rb * Kb + kb) * (int64_t)kBlkBytes + (q & 1) * 512 + row * 16;
-        emit_digits16(mx, bias, negm, b, d, dst, plane);
+        emit_digits16(mx, bias, negm, b, d, dst, plane, j0, j1);
     }
 }
 
 // split_emit for the paired-digit layout: pair planes at S (standard layout, Rb x Kb blocks
-// per plane), single-digit slots at S + P planes ([rb][slot][kb] blocks).
+// per plane), single-digit slots at S + P planes ([rb][slot][kb] blocks); planes g0 .. g1 - 1
+// of the P + Q.
 template <bool KC>
 __device__ __forceinline__ void split_emit_pairs(const double *tile, int64_t R, const int32_t *__restrict__ Erow,
                                                  int b, int d, int8_t *__restrict__ S, int64_t Rb, int64_t Kb,
-                                                 int64_t rb, int64_t kt, bool reverse_q) {
+                                                 int64_t rb, int64_t kt, bool reverse_q, int g0, int g1) {
     constexpr int TK = 128;
     constexpr int LDW = KC ? (TK + 1) : (32 + 1);
     const PairCfg pc = pair_cfg(b, d);
@@ -288,19 +295,19 @@ __device__ __forceinline__ void split_emit_pairs(const double *tile, int64_t R, 
     const int64_t intra = (q & 1) * 512 + row * 16;
     int8_t *dstS = S + (rb * Kb + kb) * (int64_t)kBlkBytes + intra;
     int8_t *dstX = S + pc.P * plane + ((rb * pc.Q) * Kb + kb) * (int64_t)kBlkBytes + intra;
-    emit_pairs16(mx, bias, negm, b, d, dstS, plane, dstX, Kb * (int64_t)kBlkBytes, reverse_q);
+    emit_pairs16(mx, bias, negm, b, d, dstS, plane, dstX, Kb * (int64_t)kBlkBytes, reverse_q, g0, g1);
 }
 
 template <bool KC>
 __device__ __forceinline__ void split_tile(const double *__restrict__ X, int64_t sr, int64_t sk, int64_t R,
                                            int64_t K, const int32_t *__restrict__ Erow, int b, int d,
                                            int8_t *__restrict__ S, int64_t Rb, int64_t Kb, int64_t rb,
-                                           int64_t kt, double *tile) {
+                                           int64_t kt, double *tile, int j0 = 0, int j1 = 1 << 30) {
     double v[16];
     split_gather<KC>(X, sr, sk, R, K, rb, kt, v);
     split_stage<KC>(v, tile);
     __syncthreads();
-    split_emit<KC>(tile, R, Erow, b, d, S, Rb, Kb, rb, kt);
+    split_emit<KC>(tile, R, Erow, b, d, S, Rb, Kb, rb, kt, j0, j1);
 }
 
 // ------------------------------------------------------------------ epilogue math
@@ -941,6 +948,11 @@ __device__ __forceinline__ void gemm_staged_tile(const GemmLaunch &g, int tm, in
 }
 
 constexpr int kCrtStages = 4;   // LDS stages of the modulus GEMM's contraction loop
+
+// Waves per SIMD a single-product tile shape of WM x WN accumulator tiles per wave leaves
+// room for: the 256 x 256 tile's 16 take half the registers of one wave, the smaller ones
+// fit two and four.
+constexpr int crt_occupancy(int tiles) { return tiles >= 8 ? 1 : (tiles >= 2 ? 2 : 4); }
 
 // ------------------------------------------------------------------ single-product LDS loop
 // (the CRT modulus GEMM and the anti-diagonal GEMM of the paired-digit path)

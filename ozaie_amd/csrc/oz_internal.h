@@ -111,9 +111,10 @@ __host__ __device__ inline PairCfg pair_cfg(int b, int d) {
 hipError_t launch_split_pairs(const OperandView &op, const int32_t *E, int b, int d, int8_t *S, int64_t Rp,
                               int64_t Kp, bool reverse_q, hipStream_t st);
 // X[r][c] (int32, row stride ldx) = sum over Q*Kb contraction blocks of the single-digit
-// planes (tile 256 x 256: Mp and Np multiples of 256).
+// planes (Mp and Np multiples of 256; workgroup tiles of 256, 128 or 64 square by the grid
+// they give on `cus` CUs).
 hipError_t launch_gemm_antidiag(const int8_t *SxA, const int8_t *SxB, int64_t Mp, int64_t Np, int64_t nblk,
-                                int32_t *X, int64_t ldx, hipStream_t st);
+                                int32_t *X, int64_t ldx, int cus, hipStream_t st);
 constexpr int kPairRowPad = 256;
 
 // Grouped launchers: items are int4 {operand, a, b, c} for the split phase and

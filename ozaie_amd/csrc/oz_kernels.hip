@@ -69,13 +69,28 @@ hipError_t launch_maxexp(const OperandView &op, int32_t *E, int64_t Rp, hipStrea
     return hipGetLastError();
 }
 
+// grid (k tiles, row blocks, plane groups of `per` planes)
 template <bool KC>
 __global__ __launch_bounds__(256) void k_split(const double *__restrict__ X, int64_t sr,
                                                int64_t sk, int64_t R, int64_t K,
                                                const int32_t *__restrict__ E, int b, int d,
-                                               int8_t *__restrict__ S, int64_t Rb, int64_t Kb) {
+                                               int8_t *__restrict__ S, int64_t Rb, int64_t Kb, int per) {
     __shared__ double tile[kSplitLds / 8];
-    split_tile<KC>(X, sr, sk, R, K, E + (int64_t)blockIdx.y * 32, b, d, S, Rb, Kb, blockIdx.y, blockIdx.x, tile);
+    const int j0 = (int)blockIdx.z * per;
+    split_tile<KC>(X, sr, sk, R, K, E + (int64_t)blockIdx.y * 32, b, d, S, Rb, Kb, blockIdx.y, blockIdx.x, tile,
+                   j0, j0 + per);
+}
+
+// Plane groups of a split launch: a grid of fewer tiles than about two per CU (512 on the
+// MI355X's 256 CUs) spreads the planes of each tile over several workgroups, each of which
+// reads the tile again (from L2 at such sizes) and writes `per` of the planes.  Returns the
+// group count and sets per.
+static int split_plane_groups(int64_t tiles, int planes, int &per) {
+    int64_t g = (tiles >= 512 || tiles < 1) ? 1 : (512 + tiles - 1) / tiles;
+    if (g > planes) g = planes;
+    if (g < 1) g = 1;
+    per = (int)((planes + g - 1) / g);
+    return (planes + per - 1) / per;
 }
 
 // Conversion check: the exact integers T[i] = hi[i] 2^64 + lo[i] (two's complement) to FP64
@@ -105,30 +120,34 @@ hipError_t launch_split(const OperandView &op, const int32_t *E, int b, int d, i
                         int64_t Rp, int64_t Kp, hipStream_t st) {
     const int64_t Rb = Rp / kBlk, Kb = Kp / kBlk;
     const int64_t nkt = (Kb + 3) / 4;
-    if (Rb > 65535 || nkt > 0x7fffffff) return hipErrorInvalidValue;
+    if (Rb > 65535 || nkt > 0x7fffffff || d < 1) return hipErrorInvalidValue;
+    int per;
+    const int groups = split_plane_groups(nkt * Rb, d, per);
+    const dim3 grid((unsigned)nkt, (unsigned)Rb, (unsigned)groups);
     if (op.sk == 1)
-        hipLaunchKernelGGL(k_split<true>, dim3((unsigned)nkt, (unsigned)Rb), dim3(256), 0, st, op.X,
-                           op.sr, op.sk, op.R, op.K, E, b, d, S, Rb, Kb);
+        hipLaunchKernelGGL(k_split<true>, grid, dim3(256), 0, st, op.X, op.sr, op.sk, op.R, op.K, E, b, d, S, Rb,
+                           Kb, per);
     else
-        hipLaunchKernelGGL(k_split<false>, dim3((unsigned)nkt, (unsigned)Rb), dim3(256), 0, st,
-                           op.X, op.sr, op.sk, op.R, op.K, E, b, d, S, Rb, Kb);
+        hipLaunchKernelGGL(k_split<false>, grid, dim3(256), 0, st, op.X, op.sr, op.sk, op.R, op.K, E, b, d, S, Rb,
+                           Kb, per);
     return hipGetLastError();
 }
 
 // Paired-digit split (b <= 3 presets): the same grid and staging as k_split, pair planes
-// and single-digit slots written by emit_pairs16.
+// and single-digit slots written by emit_pairs16 (grid z: groups of `per` of the P + Q).
 template <bool KC>
 __global__ __launch_bounds__(256) void k_split_pairs(const double *__restrict__ X, int64_t sr, int64_t sk,
                                                      int64_t R, int64_t K, const int32_t *__restrict__ E, int b,
                                                      int d, int8_t *__restrict__ S, int64_t Rb, int64_t Kb,
-                                                     int reverse_q) {
+                                                     int reverse_q, int per) {
     __shared__ double tile[kSplitLds / 8];
     double v[16];
     split_gather<KC>(X, sr, sk, R, K, blockIdx.y, blockIdx.x, v);
     split_stage<KC>(v, tile);
     __syncthreads();
+    const int g0 = (int)blockIdx.z * per;
     split_emit_pairs<KC>(tile, R, E + (int64_t)blockIdx.y * 32, b, d, S, Rb, Kb, blockIdx.y, blockIdx.x,
-                         reverse_q != 0);
+                         reverse_q != 0, g0, g0 + per);
 }
 
 hipError_t launch_split_pairs(const OperandView &op, const int32_t *E, int b, int d, int8_t *S, int64_t Rp,
@@ -136,23 +155,28 @@ hipError_t launch_split_pairs(const OperandView &op, const int32_t *E, int b, in
     const int64_t Rb = Rp / kBlk, Kb = Kp / kBlk;
     const int64_t nkt = (Kb + 3) / 4;
     if (Rb > 65535 || nkt > 0x7fffffff || b < 1 || b > 3 || d < 2 || b * d > 96) return hipErrorInvalidValue;
+    const PairCfg pc = pair_cfg(b, d);
+    int per;
+    const int groups = split_plane_groups(nkt * Rb, pc.P + pc.Q, per);
+    const dim3 grid((unsigned)nkt, (unsigned)Rb, (unsigned)groups);
     if (op.sk == 1)
-        hipLaunchKernelGGL(k_split_pairs<true>, dim3((unsigned)nkt, (unsigned)Rb), dim3(256), 0, st, op.X, op.sr,
-                           op.sk, op.R, op.K, E, b, d, S, Rb, Kb, (int)reverse_q);
+        hipLaunchKernelGGL(k_split_pairs<true>, grid, dim3(256), 0, st, op.X, op.sr, op.sk, op.R, op.K, E, b, d, S,
+                           Rb, Kb, (int)reverse_q, per);
     else
-        hipLaunchKernelGGL(k_split_pairs<false>, dim3((unsigned)nkt, (unsigned)Rb), dim3(256), 0, st, op.X, op.sr,
-                           op.sk, op.R, op.K, E, b, d, S, Rb, Kb, (int)reverse_q);
+        hipLaunchKernelGGL(k_split_pairs<false>, grid, dim3(256), 0, st, op.X, op.sr, op.sk, op.R, op.K, E, b, d, S,
+                           Rb, Kb, (int)reverse_q, per);
     return hipGetLastError();
 }
 
 // The anti-diagonal of the paired-digit path: one int8 GEMM over nblk contraction blocks
-// (Q single-digit slots x Kb blocks, contiguous per 32-row block), workgroup tile 256 x 256
-// through the CRT modulus GEMM's LDS loop, int32 result stored whole (the sums stay below
-// Q K (2^b - 1)^2 < 2^31, which the planner checks).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+// (Q single-digit slots x Kb blocks, contiguous per 32-row block), workgroup tile (64 WM) x
+// (64 WN) through the CRT modulus GEMM's LDS loop, int32 result stored whole (the sums stay
+// below Q K (2^b - 1)^2 < 2^31, which the planner checks).
+template <int WM, int WN>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(crt_occupancy(WM * WN), crt_occupancy(WM * WN))))
 void k_gemm_antidiag(const int8_t *__restrict__ SxA, const int8_t *__restrict__ SxB, int64_t Mp, int64_t Np,
                      int64_t nblk, int32_t *__restrict__ X, int64_t ldx) {
-    constexpr int WM = 4, WN = 4;
     __shared__ __attribute__((aligned(1024))) int8_t smem[kCrtStages * (2 * WM + 2 * WN) * kBlkBytes];
     const int ntm = (int)(Mp / (64 * WM)), ntn = (int)(Np / (64 * WN));
     int tm, tn;
@@ -181,13 +205,25 @@ void k_gemm_antidiag(const int8_t *__restrict__ SxA, const int8_t *__restrict__ 
                 }
 }
 
+// The workgroup tile is the largest of 256, 128 and 64 square whose grid still gives every
+// CU a workgroup (cus of them), so that small products are not left to a few CUs.
 hipError_t launch_gemm_antidiag(const int8_t *SxA, const int8_t *SxB, int64_t Mp, int64_t Np, int64_t nblk,
-                                int32_t *X, int64_t ldx, hipStream_t st) {
+                                int32_t *X, int64_t ldx, int cus, hipStream_t st) {
     if (Mp % 256 || Np % 256 || nblk < 1 || ldx < Np) return hipErrorInvalidValue;
-    const int64_t tiles = (Mp / 256) * (Np / 256);
-    if (tiles <= 0) return hipSuccess;
+    if (Mp == 0 || Np == 0) return hipSuccess;
+    int t = 256;
+    while (t > 64 && (Mp / t) * (Np / t) < (int64_t)cus) t /= 2;
+    const int64_t tiles = (Mp / t) * (Np / t);
     if (tiles > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_gemm_antidiag, dim3((unsigned)tiles), dim3(256), 0, st, SxA, SxB, Mp, Np, nblk, X, ldx);
+    if (t == 256)
+        hipLaunchKernelGGL((k_gemm_antidiag<4, 4>), dim3((unsigned)tiles), dim3(256), 0, st, SxA, SxB, Mp, Np, nblk,
+                           X, ldx);
+    else if (t == 128)
+        hipLaunchKernelGGL((k_gemm_antidiag<2, 2>), dim3((unsigned)tiles), dim3(256), 0, st, SxA, SxB, Mp, Np, nblk,
+                           X, ldx);
+    else
+        hipLaunchKernelGGL((k_gemm_antidiag<1, 1>), dim3((unsigned)tiles), dim3(256), 0, st, SxA, SxB, Mp, Np, nblk,
+                           X, ldx);
     return hipGetLastError();
 }
 
