@@ -415,6 +415,7 @@ GemmLaunch base_launch(const ozaie_plan p) {
     g.T = p->T; g.ldT = p->N;
     g.K_true = (int)std::min<int64_t>(p->K, INT32_MAX);
     g.family = p->family;
+    g.cus = p->cus;
     return g;
 }
 

@@ -434,28 +434,35 @@ __device__ __forceinline__ i128 bins_to_T(const int32_t (&v)[D]) {
     return T;
 }
 
-// The same store for EPI_C when the pass holds every bin (k0 = 0, NB = d) and b = B.
+// The same store for EPI_C when the pass holds every bin (k0 = 0, NB = d) and b = B, with
+// the paired-digit path's anti-diagonal term and conversion (cb, cd) when g.cd is set.
 template <int D, int B>
 __device__ __forceinline__ void epilogue_elem_c(const GemmLaunch &g, int row, int col, const int32_t (&vals)[D],
                                                 int es) {
     if (row >= g.M || col >= g.N) return;
+    const int cb = g.cd ? g.cb : B, cd = g.cd ? g.cd : D;
     if constexpr (D <= 31 / B + 1) {
-        // one 64-bit chunk holds every bin (|T| < 2^62): the whole product in int64
+        // one 64-bit chunk holds every bin: each bin is below 2^31 in magnitude, so |T| <
+        // 2^31 * 2^(B (D - 1)) / (1 - 2^-B) <= 2^62.5 for B >= 2, and the int32 anti-diagonal
+        // term keeps it inside int64: the whole product in int64
         if (g.first && g.last) {
             int64_t c = 0;
 #pragma unroll
             for (int i = 0; i < D; i++) c += (int64_t)vals[D - 1 - i] << (B * i);
-            g.C[(int64_t)row * g.ldc + col] = t64_to_double(c, es, B, D, g.rounding);
+            if (g.xadd) c += g.xsign * (int64_t)g.xadd[(int64_t)row * g.ldx + col];
+            g.C[(int64_t)row * g.ldc + col] = t64_to_double(c, es, cb, cd, g.rounding);
             return;
         }
     }
     i128 T = bins_to_T<D, B>(vals);
     i128 *tp = g.T + (int64_t)row * g.ldT + col;
     if (!g.first) T += *tp;
-    if (g.last)
-        g.C[(int64_t)row * g.ldc + col] = t_to_double(T, es, B, D, g.rounding);
-    else
+    if (g.last) {
+        if (g.xadd) T += (i128)(g.xsign * (int64_t)g.xadd[(int64_t)row * g.ldx + col]);
+        g.C[(int64_t)row * g.ldc + col] = t_to_double(T, es, cb, cd, g.rounding);
+    } else {
         *tp = T;
+    }
 }
 
 // Block -> tile mapping.  Workgroups are dispatched round-robin over the 8 XCDs; give
@@ -559,10 +566,17 @@ __device__ __forceinline__ void wave_epilogue(const GemmLaunch &g, int epi, int 
                                               int ec_v) {
     scratch[NB * 256 + lane] = er_v;
     scratch[NB * 256 + 64 + lane] = ec_v;
-    if (epi == EPI_C && k0 == 0 && NB == g.d && g.b == 7 && !g.xadd && !g.cd)
+    if (epi == EPI_C && k0 == 0 && NB == g.d && g.b == 7 && !g.xadd && !g.cd) {
         wave_epilogue_t<NB, WM, WN, 7>(g, epi, k0, acc, scratch, lane, rbase, cbase);
-    else
-        wave_epilogue_t<NB, WM, WN, 0>(g, epi, k0, acc, scratch, lane, rbase, cbase);
+        return;
+    }
+    if constexpr (NB >= 4) {   // pair planes of 3-bit digits (the reference's accurate presets)
+        if (epi == EPI_C && k0 == 0 && NB == g.d && g.b == 6 && g.cd) {
+            wave_epilogue_t<NB, WM, WN, 6>(g, epi, k0, acc, scratch, lane, rbase, cbase);
+            return;
+        }
+    }
+    wave_epilogue_t<NB, WM, WN, 0>(g, epi, k0, acc, scratch, lane, rbase, cbase);
 }
 
 // Exponent fields this lane hands to the epilogue: row rbase + lane of A (lane < rows)

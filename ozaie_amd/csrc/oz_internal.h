@@ -61,6 +61,7 @@ struct GemmLaunch {
     const int32_t *xadd;
     int64_t ldx;
     int xsign;
+    int cus;                // compute units of the device (0: unknown), for the tile choice
 };
 
 // Fused-kernel families (same arithmetic, different data movement).

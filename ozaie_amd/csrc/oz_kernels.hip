@@ -422,10 +422,8 @@ bool fused_supported(int d) { return d >= 1 && d <= 16; }
 //   other d <= 8: the direct-to-register kernel (its two fragment sets fit in registers;
 //           at d = 8 the rolling kernel is 2.4% slower);
 //   above: the LDS-copy kernel.
-static int default_family(int d) {
-    if (d <= 3) return KERNEL_STAGED;
-    if (d == 7 || d == 9) return KERNEL_ROLL;
-    return d <= 8 ? KERNEL_DIRECT : KERNEL_LDS;
+static constexpr int default_family(int d) {
+    return d <= 3 ? KERNEL_STAGED : ((d == 7 || d == 9) ? KERNEL_ROLL : (d <= 8 ? KERNEL_DIRECT : KERNEL_LDS));
 }
 
 template <int D, int WM, int WN>
@@ -451,9 +449,44 @@ static hipError_t launch_fused_t(const GemmLaunch &g, int epi, hipStream_t st) {
     return hipGetLastError();
 }
 
+// Small products: the default family of D on 32 x 32 wave tiles (64 x 64 workgroup tiles),
+// four times the workgroups of the 64 x 64 wave tiles of d <= 4 and twice those of the
+// 32 x 64 ones of d = 5 .. 8.  Where the default grid leaves CUs idle, the smaller tile
+// spreads the work (and the epilogue, a fixed cost per element that dominates at small K)
+// over more of them.
+template <int D>
+static hipError_t launch_fused_small(const GemmLaunch &g, int epi, hipStream_t st) {
+    const int64_t tiles = (int64_t)((g.M + 63) / 64) * ((g.N + 63) / 64);
+    if (tiles <= 0) return hipSuccess;
+    if constexpr (default_family(D) == KERNEL_STAGED)
+        hipLaunchKernelGGL((k_gemm_staged<D, 1, 1>), dim3((unsigned)tiles), dim3(256), 0, st, g, epi);
+    else if constexpr (default_family(D) == KERNEL_ROLL)
+        hipLaunchKernelGGL((k_gemm_roll<D, 1, 1>), dim3((unsigned)tiles), dim3(256), 0, st, g, epi);
+    else
+        hipLaunchKernelGGL((k_gemm_direct<D, 1, 1>), dim3((unsigned)tiles), dim3(256), 0, st, g, epi);
+    return hipGetLastError();
+}
+
 // Wave tile per slice count: the accumulators (d bins x 16 registers per 32 x 32 block)
-// must leave room for the fragments in the 512 registers of a wave.
+// must leave room for the fragments in the 512 registers of a wave.  With the default
+// family, a d <= 8 grid of fewer workgroups than the device has CUs runs on 32 x 32 wave
+// tiles instead (launch_fused_small).
 hipError_t launch_gemm_fused(const GemmLaunch &g, int epi, hipStream_t st) {
+    if (g.d >= 1 && g.d <= 8 && g.family == KERNEL_AUTO && g.cus > 0) {
+        const int tm = g.d <= 4 ? 128 : 64, tn = 128;   // the default workgroup tile
+        if ((int64_t)((g.M + tm - 1) / tm) * ((g.N + tn - 1) / tn) < g.cus) {
+            switch (g.d) {
+                case 1: return launch_fused_small<1>(g, epi, st);
+                case 2: return launch_fused_small<2>(g, epi, st);
+                case 3: return launch_fused_small<3>(g, epi, st);
+                case 4: return launch_fused_small<4>(g, epi, st);
+                case 5: return launch_fused_small<5>(g, epi, st);
+                case 6: return launch_fused_small<6>(g, epi, st);
+                case 7: return launch_fused_small<7>(g, epi, st);
+                default: return launch_fused_small<8>(g, epi, st);
+            }
+        }
+    }
     switch (g.d) {
         case 1: return launch_fused_t<1, 2, 2>(g, epi, st);
         case 2: return launch_fused_t<2, 2, 2>(g, epi, st);
