@@ -32,7 +32,8 @@ The JSON line also carries:
   host_pointer  the matched preset through the reference-style entry point with host arrays
                 (PCIe copies of A, B and C inside the timed call); not `value`
   square_4096   the same measurements at 4096^3 (256 x 256 error block)
-  accurate_mode the reference's Mode.accurate (b = 3) at 256 x 512 x 128, 2048^3, 4096^3
+  accurate_mode the reference's Mode.accurate (b = 3) at 256 x 512 x 128, 2048^3, 4096^3, and
+                (b = 4, d = 18) at 4096 x 4096 x 16384
   long_k, grouped_512, autograd_2048: the other benchmark configurations
 """
 import argparse
@@ -264,10 +265,22 @@ def measure(args, torch, np, oz, n, steps, warmup, block, slices):
                       "median_rel_err": native_err[1]}, "presets": []}
     log(args, f"  n={n}: native {t_nat:.3f} ms {flop / t_nat / 1e9:.1f} TF/s err {native_err}")
 
-    oz.set_option(oz.OPT_PHASE_TIMING, 1)
+    def timed_plan(*a, **kw):
+        # a second plan with the library's phase events, for the per-phase split only: the
+        # timed calls above run the plan as a caller gets it, with no event records
+        oz.set_option(oz.OPT_PHASE_TIMING, 1)
+        try:
+            return oz.Plan(*a, **kw)
+        finally:
+            oz.set_option(oz.OPT_PHASE_TIMING, 0)
+
     for d in slices:
         plan = oz.Plan(n, n, n, 7, d)
         t = time_steps(torch, lambda: plan(A, B, C), steps, warmup)
+        err = rel_errors(np, C[blk].cpu().numpy().ravel(), exact)
+        info = plan.info()
+        plan.close()
+        plan = timed_plan(n, n, n, 7, d)
         gemm = []
         split = []
         for _ in range(max(3, steps)):
@@ -275,8 +288,6 @@ def measure(args, torch, np, oz, n, steps, warmup, block, slices):
             s_ms, g_ms, _ = plan.phase_times()
             split.append(s_ms)
             gemm.append(g_ms)
-        err = rel_errors(np, C[blk].cpu().numpy().ravel(), exact)
-        info = plan.info()
         prods = d * (d + 1) // 2
         rec = {"scheme": "split", "bits": 7, "slices": d, "ms": t, "tflops": flop / t / 1e9, "speedup": t_nat / t,
                "max_rel_err": err[0], "median_rel_err": err[1], "split_ms": float(np.mean(split)),
@@ -290,6 +301,10 @@ def measure(args, torch, np, oz, n, steps, warmup, block, slices):
         # CRT mode: exact integer product from `mods` int8 residue GEMMs (nearest rounding)
         plan = oz.Plan(n, n, n, 0, mods, rounding=oz.ROUND_NEAREST, crt=True)
         t = time_steps(torch, lambda: plan(A, B, C), steps, warmup)
+        err = rel_errors(np, C[blk].cpu().numpy().ravel(), exact)
+        info = plan.info()
+        plan.close()
+        plan = timed_plan(n, n, n, 0, mods, rounding=oz.ROUND_NEAREST, crt=True)
         gemm, split, tot = [], [], []
         for _ in range(max(3, steps)):
             plan(A, B, C)
@@ -297,8 +312,6 @@ def measure(args, torch, np, oz, n, steps, warmup, block, slices):
             split.append(s_ms)
             gemm.append(g_ms)
             tot.append(a_ms)
-        err = rel_errors(np, C[blk].cpu().numpy().ravel(), exact)
-        info = plan.info()
         g_ms = float(np.mean(gemm))
         rec = {"scheme": "crt", "moduli": mods, "bits": info["bits"], "scaling": scaling_name(info["bits"]),
                "ms": t, "tflops": flop / t / 1e9,
@@ -311,7 +324,6 @@ def measure(args, torch, np, oz, n, steps, warmup, block, slices):
         log(args, f"  n={n}: crt {mods} moduli ({info['bits']} bits) {t:.3f} ms {rec['tflops']:.1f} TF/s "
                   f"x{rec['speedup']:.2f} err {err} gemm {g_ms:.3f} ms {rec['gemm_int8_tops']:.0f} TOPS")
         plan.close()
-    oz.set_option(oz.OPT_PHASE_TIMING, 0)
     # matched: the fastest preset whose max error is no worse than native's
     ok = [p for p in out["presets"] if p["max_rel_err"] <= native_err[0]]
     if ok:
@@ -793,16 +805,20 @@ def accurate_section(args, torch, np, oz):
         exact = oz.exact_entries(A[:blk].cpu().numpy(), B[:, :blk].cpu().numpy(), rows, cols)
         nat_err = rel_errors(np, C[:blk, :blk].cpu().numpy().ravel(), exact)
         b, d = oz.mode_slices(oz.Mode.accurate, K)
+        # the time from a plan without phase events (their records cost a few microseconds a
+        # call, which shows at the small shape), the GEMM phase from a second plan with them
+        p = oz.Plan(M, N, K, b, d)
+        t = time_steps(torch, lambda: p(A, B, C), 10, 5)
+        err = rel_errors(np, C[:blk, :blk].cpu().numpy().ravel(), exact)
+        info = p.info()
+        p.close()
         oz.set_option(oz.OPT_PHASE_TIMING, 1)
         p = oz.Plan(M, N, K, b, d)
         oz.set_option(oz.OPT_PHASE_TIMING, 0)
-        t = time_steps(torch, lambda: p(A, B, C), 10, 5)
         gemm = []
         for _ in range(5):
             p(A, B, C)
             gemm.append(p.phase_times()[1])
-        err = rel_errors(np, C[:blk, :blk].cpu().numpy().ravel(), exact)
-        info = p.info()
         p.close()
         rec = {"M": M, "N": N, "K": K, "phi": phi, "bits": b, "slices": d, "ms": t, "tflops": flop / t / 1e9,
                "native_ms": t_nat, "speedup": t_nat / t, "max_rel_err": err[0], "median_rel_err": err[1],
