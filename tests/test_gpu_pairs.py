@@ -77,6 +77,32 @@ def test_pairs_equal_bin_group_kernel_and_config1(gpu):
     assert np.array_equal(bits(C), bits(got))
 
 
+@pytest.mark.parametrize("n,b,d", [(2048, 3, 27), (4096, 3, 25), (2048, 3, 9), (2048, 3, 16)])
+def test_pairs_large_grids_match_bin_groups_and_oracle(gpu, n, b, d):
+    """Grids large enough for the anti-diagonal GEMM's 128 and 256 square workgroup tiles and
+    the fused kernel's default wave tiles (4 and 8 pair planes: 64 x 64 and 32 x 64; small
+    grids run 64 square and 32 x 32 tiles): C equal, bit for bit, to the bin-group kernel's
+    over the whole of C, and to the oracle on a 64 x 64 block (the split scales each row of A
+    and each column of B on its own, so the block's rows and columns alone give its C)."""
+    K = 256
+    rng = np.random.default_rng(n + 7 * d)
+    print(f"pairs large grid b={b} d={d} {n}x{n}x{K}", flush=True)
+    A = gen_inputs(rng, n, K, 1.0)
+    B = gen_inputs(rng, K, n, 1.0)
+    got, info = run(n, n, K, b, d, A, B)
+    assert info["kernel"] == "pairs"
+    oz.set_option(oz.OPT_KERNEL, 1)
+    try:
+        grp, ginfo = run(n, n, K, b, d, A, B)
+    finally:
+        oz.set_option(oz.OPT_KERNEL, 0)
+    assert ginfo["kernel"] == "group"
+    assert np.array_equal(bits(got), bits(grp))
+    r0, c0 = (int(x) for x in rng.integers(0, n - 64, 2))
+    ref, _ = oracle.ozaki_dgemm(A[r0:r0 + 64], B[:, c0:c0 + 64], b, d)
+    assert np.array_equal(bits(got[r0:r0 + 64, c0:c0 + 64]), bits(ref))
+
+
 @pytest.mark.parametrize("b,d", [(3, 29), (3, 26)])
 def test_pairs_several_contraction_chunks(gpu, b, d):
     rng = np.random.default_rng(40 + d)

@@ -127,6 +127,37 @@ def test_C_bit_exact_truncating(gpu, b, d):
     assert np.array_equal(bits(C.cpu().numpy()), bits(ref))
 
 
+# the default fused-kernel family per slice count (OZAIE_OPT_KERNEL values)
+DEFAULT_FAMILY = {1: 5, 2: 5, 3: 5, 4: 3, 5: 3, 6: 3, 7: 6, 8: 3}
+
+
+@pytest.mark.parametrize("d", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_small_grid_tiles_match_default_tiles(gpu, d):
+    """A d <= 8 product whose default grid has fewer workgroups than CUs runs on 32 x 32
+    wave tiles; naming the same family through the option keeps the default tile.  Both give
+    the same C, bit for bit, equal to the oracle's, with ragged edges in both dimensions."""
+    import torch
+    rng = np.random.default_rng(40 + d)
+    M, N, K = 450, 390, 300
+    A = gen_inputs(rng, M, K, 1.0)
+    B = gen_inputs(rng, K, N, 1.0)
+    print(f"small-grid tiles b=7 d={d} {M}x{N}x{K}", flush=True)
+    out = []
+    for fam in (0, DEFAULT_FAMILY[d]):
+        oz.set_option(oz.OPT_KERNEL, fam)
+        try:
+            plan = oz.Plan(M, N, K, 7, d)
+        finally:
+            oz.set_option(oz.OPT_KERNEL, 0)
+        C = plan(dev(A), dev(B))
+        torch.cuda.synchronize()
+        plan.close()
+        out.append(C.cpu().numpy())
+    ref, _ = oracle.ozaki_dgemm(A, B, 7, d)
+    assert np.array_equal(bits(out[0]), bits(ref))
+    assert np.array_equal(bits(out[1]), bits(ref))
+
+
 def test_C_nearest_even(gpu):
     import torch
     rng = np.random.default_rng(3)
