@@ -79,6 +79,11 @@ def parse():
                     help="only the measured int8 MFMA ceiling (ozaie_probe_mfma), as one JSON line")
     ap.add_argument("--power-only", action="store_true",
                     help="only board power and GFX clocks (amd-smi) under sustained loads, as one JSON line")
+    ap.add_argument("--power-presets", default="crt15,split_d8,split_d9",
+                    help="--power-only: emulated loads, crtN (N moduli) or split_dN (b = 7, N slices)")
+    ap.add_argument("--power-sizes", default="",
+                    help="--power-only: comma-separated n of the n^3 workloads (default --size)")
+    ap.add_argument("--no-probes", action="store_true", help="--power-only: leave out the MFMA probe loads")
     ap.add_argument("--profile-run", action="store_true",
                     help="only run each preset's GEMM twice and one 1 GiB device copy (for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -856,7 +861,17 @@ def main():
         print(json.dumps(accurate_section(args, torch, np, oz)), flush=True)
         return
     if args.power_only:
-        print(json.dumps(power_section(torch, np, oz, n)), flush=True)
+        presets = []
+        for name in (x.strip() for x in args.power_presets.split(",") if x.strip()):
+            if name.startswith("crt"):
+                presets.append((name, 0, int(name[3:]), True))
+            elif name.startswith("split_d"):
+                presets.append((name, 7, int(name[7:]), False))
+            else:
+                raise SystemExit(f"unknown power preset {name!r} (crtN or split_dN)")
+        sizes = [int(x) for x in args.power_sizes.split(",") if x.strip()] or [n]
+        recs = [power_section(torch, np, oz, m, presets=tuple(presets), probes=not args.no_probes) for m in sizes]
+        print(json.dumps(recs[0] if len(recs) == 1 else recs), flush=True)
         return
     if args.probe_only:
         rec = {}
