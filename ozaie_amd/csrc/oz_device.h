@@ -833,25 +833,27 @@ __device__ __forceinline__ void gemm_roll_tile(const GemmLaunch &g, int tm, int 
 // loop has no branch; a checked tail pass issues no load past the pass's last block and
 // waits for every load, so the epilogue's LDS scratch follows the last barrier safely.
 // Any number of contraction blocks per pass.
-// Stages S: as many as fit in 160 KiB of LDS, even, at most 8 (d = 1, 2: 8; d = 3: 6; d = 4
-// and d = 9 on 64 x 64: 4).  Measured against four stages: d = 1 8-22% faster, d = 2 and 3
-// unchanged.
-template <int D, int WM, int WN>
+// Stages S: as many as fit in 160 KiB of LDS, even, at most MAXS (8 by default: d = 1, 2: 8;
+// d = 3: 6; d = 4 and d = 9 on 64 x 64: 4).  Measured against four stages: d = 1 8-22%
+// faster, d = 2 and 3 unchanged.  A smaller MAXS leaves LDS for a second workgroup per CU.
+template <int D, int WM, int WN, int MAXS = 8>
 __host__ __device__ constexpr int staged_stages() {
     const int s = (160 * 1024 / FusedCfg<D, WM, WN>::STAGE) & ~1;
-    return s > 8 ? 8 : s;
+    return s > MAXS ? MAXS : s;
 }
 
-template <int D, int WM, int WN>
-__host__ __device__ constexpr int staged_tile_bytes() { return staged_stages<D, WM, WN>() * FusedCfg<D, WM, WN>::STAGE; }
+template <int D, int WM, int WN, int MAXS = 8>
+__host__ __device__ constexpr int staged_tile_bytes() {
+    return staged_stages<D, WM, WN, MAXS>() * FusedCfg<D, WM, WN>::STAGE;
+}
 
-template <int D, int WM, int WN>
+template <int D, int WM, int WN, int MAXS = 8>
 __device__ __forceinline__ void gemm_staged_tile(const GemmLaunch &g, int tm, int tn, int epi, int8_t *smem) {
     using F = FusedCfg<D, WM, WN>;
-    constexpr int BA = F::BA, BB = F::BB, NBLK = F::NBLK, S = staged_stages<D, WM, WN>();
+    constexpr int BA = F::BA, BB = F::BB, NBLK = F::NBLK, S = staged_stages<D, WM, WN, MAXS>();
     static_assert(S >= 4, "at least three blocks in flight");
     static_assert(NBLK % 4 == 0, "every wave brings the same number of blocks");
-    static_assert(staged_tile_bytes<D, WM, WN>() >= 4 * 4 * wave_scratch_words<D, WM, WN>(), "epilogue scratch");
+    static_assert(staged_tile_bytes<D, WM, WN, MAXS>() >= 4 * 4 * wave_scratch_words<D, WM, WN>(), "epilogue scratch");
     constexpr int PER = NBLK / 4;
     constexpr int VM = PER * (S - 2);
     static_assert(VM < 64, "vmcnt range");

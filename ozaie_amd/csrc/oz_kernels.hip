@@ -410,6 +410,21 @@ void k_gemm_staged(GemmLaunch g, int epi) {
     gemm_staged_tile<D, WM, WN>(g, tm, tn, epi, smem);
 }
 
+// d = 3 on 32 x 32 wave tiles with four LDS stages (48 KiB) and two waves per SIMD: two
+// workgroups share a CU, so one's epilogue and first loads overlap the other's MFMAs.  It
+// moves twice the bytes per int8 op through L2 and LDS of the 64 x 64 wave tile, which
+// costs clock where the card runs at its power limit (measured on MI355X against the
+// default tile: 2048^3 0.106 / 0.113 ms, 4096^3 0.517 / 0.546 ms, 8192^3 3.54 / 3.20 ms).
+template <int D>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_gemm_staged_x2(GemmLaunch g, int epi) {
+    const int ntm = (g.M + 63) / 64, ntn = (g.N + 63) / 64;
+    int tm, tn;
+    tile_map(blockIdx.x, ntm, ntn, tm, tn);
+    __shared__ __attribute__((aligned(1024))) int8_t smem[staged_tile_bytes<D, 1, 1, 4>()];
+    gemm_staged_tile<D, 1, 1, 4>(g, tm, tn, epi, smem);
+}
+
 bool fused_supported(int d) { return d >= 1 && d <= 16; }
 
 // Default fused-kernel family per slice count (measured on MI355X at 4096^3 and 8192^3,
@@ -469,12 +484,20 @@ static hipError_t launch_fused_small(const GemmLaunch &g, int epi, hipStream_t s
 
 // Wave tile per slice count: the accumulators (d bins x 16 registers per 32 x 32 block)
 // must leave room for the fragments in the 512 registers of a wave.  With the default
-// family, a d <= 8 grid of fewer workgroups than the device has CUs runs on 32 x 32 wave
-// tiles instead (launch_fused_small).
+// family, d = 3 runs k_gemm_staged_x2 up to four default workgroups per CU, and another
+// d <= 8 grid of fewer workgroups than the device has CUs runs on 32 x 32 wave tiles
+// (launch_fused_small).
 hipError_t launch_gemm_fused(const GemmLaunch &g, int epi, hipStream_t st) {
     if (g.d >= 1 && g.d <= 8 && g.family == KERNEL_AUTO && g.cus > 0) {
         const int tm = g.d <= 4 ? 128 : 64, tn = 128;   // the default workgroup tile
-        if ((int64_t)((g.M + tm - 1) / tm) * ((g.N + tn - 1) / tn) < g.cus) {
+        const int64_t wgs = (int64_t)((g.M + tm - 1) / tm) * ((g.N + tn - 1) / tn);
+        if (g.d == 3 && wgs <= 4 * (int64_t)g.cus) {   // up to four default workgroups per CU
+            const int64_t tiles = (int64_t)((g.M + 63) / 64) * ((g.N + 63) / 64);
+            if (tiles > 0x7fffffff) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_gemm_staged_x2<3>), dim3((unsigned)tiles), dim3(256), 0, st, g, epi);
+            return hipGetLastError();
+        }
+        if (wgs < g.cus) {
             switch (g.d) {
                 case 1: return launch_fused_small<1>(g, epi, st);
                 case 2: return launch_fused_small<2>(g, epi, st);
