@@ -271,6 +271,17 @@ def release():
 
 
 # ---------------------------------------------------------------- explicit plans (torch)
+def _ld(x):
+    """Leading dimension of a row-major 2-D tensor for the C ABI.  An axis of length 1 has
+    no stride to speak of (numpy and torch keep any value there): a single row passes its
+    own length, and a single column any row stride."""
+    return int(x.stride(0)) if x.shape[0] > 1 else int(x.shape[1])
+
+
+def _unit_last_axis(x):
+    return (x.shape[1] <= 1 or x.stride(1) == 1) and _ld(x) >= x.shape[1]
+
+
 def _stream_handle(stream):
     import torch
     if stream is None:
@@ -305,15 +316,15 @@ class Plan:
         for x in (A, B, C):
             if x.dtype != torch.float64 or not x.is_cuda:
                 raise ValueError("operands must be float64 device tensors")
-            if x.stride(1) != 1 or x.stride(0) < x.shape[1]:
+            if not _unit_last_axis(x):
                 raise ValueError("operands must be row-major with unit stride along their last axis")
         ea = (self.K, self.M) if transA else (self.M, self.K)
         eb = (self.N, self.K) if transB else (self.K, self.N)
         if tuple(A.shape) != ea or tuple(B.shape) != eb or tuple(C.shape) != (self.M, self.N):
             raise ValueError(f"shape mismatch: A {tuple(A.shape)} (want {ea}), B {tuple(B.shape)} "
                              f"(want {eb}), C {tuple(C.shape)}")
-        _check(lib().ozaie_dgemm(self._h, int(transA), int(transB), _ptr(A), A.stride(0), _ptr(B),
-                                 B.stride(0), _ptr(C), C.stride(0), _stream_handle(stream)),
+        _check(lib().ozaie_dgemm(self._h, int(transA), int(transB), _ptr(A), _ld(A), _ptr(B),
+                                 _ld(B), _ptr(C), _ld(C), _stream_handle(stream)),
                "ozaie_dgemm")
         return C
 
@@ -345,12 +356,12 @@ def dgemm_noalloc(A, B, C, bits=7, slices=8, rounding=ROUND_TRUNCATE, crt=False,
     M, N = C.shape
     K = A.shape[0] if transA else A.shape[1]
     for x in (A, B, C):
-        if str(x.dtype) != "torch.float64" or not x.is_cuda or x.stride(1) != 1:
+        if str(x.dtype) != "torch.float64" or not x.is_cuda or not _unit_last_axis(x):
             raise ValueError("operands must be float64 device tensors with unit stride along their last axis")
     flags = (1 if rounding == ROUND_NEAREST else 0) | (4 if crt else 0)
     _check(lib().ozaie_dgemm_noalloc(int(M), int(N), int(K), int(bits), int(slices), flags, int(transA),
-                                     int(transB), _ptr(A), A.stride(0), _ptr(B), B.stride(0), _ptr(C),
-                                     C.stride(0), _stream_handle(stream)), "ozaie_dgemm_noalloc")
+                                     int(transB), _ptr(A), _ld(A), _ptr(B), _ld(B), _ptr(C),
+                                     _ld(C), _stream_handle(stream)), "ozaie_dgemm_noalloc")
     return C
 
 

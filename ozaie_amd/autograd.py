@@ -14,13 +14,13 @@ mode (``slices`` moduli, ``bits`` = 0 for the largest exact bit count).
 """
 import torch
 
-from . import Plan, ROUND_NEAREST, ROUND_TRUNCATE
+from . import Plan, ROUND_NEAREST, ROUND_TRUNCATE, _unit_last_axis
 
 _plans = {}
 
 
 def _row_major(x):
-    return x.stride(1) == 1 and x.stride(0) >= x.shape[1]
+    return _unit_last_axis(x)
 
 
 def _plan(M, N, K, bits, slices, rounding, crt=False):
