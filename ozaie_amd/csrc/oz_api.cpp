@@ -432,16 +432,19 @@ int run_split(ozaie_plan p, const OperandView &va, const OperandView &vb, hipStr
         HIPCHK(launch_crt_split(a, b, p->cc, p->Kp, st));
         return OZAIE_OK;
     }
-    HIPCHK(launch_maxexp(va, p->Er, p->Mp, st));
+    // K <= 128: each split workgroup holds a whole row's contraction axis and forms the
+    // row's exponent maximum itself (two launches fewer per operand)
+    const bool one_tile = p->Kp <= 128;
+    if (!one_tile) HIPCHK(launch_maxexp(va, p->Er, p->Mp, st));
     if (p->pairs)
-        HIPCHK(launch_split_pairs(va, p->Er, p->b, p->d, p->SA, p->Mp, p->Kp, false, st));
+        HIPCHK(launch_split_pairs(va, p->Er, p->b, p->d, p->SA, p->Mp, p->Kp, false, st, one_tile ? p->Er : nullptr));
     else
-        HIPCHK(launch_split(va, p->Er, p->b, p->d, p->SA, p->Mp, p->Kp, st));
-    HIPCHK(launch_maxexp(vb, p->Ec, p->Np, st));
+        HIPCHK(launch_split(va, p->Er, p->b, p->d, p->SA, p->Mp, p->Kp, st, one_tile ? p->Er : nullptr));
+    if (!one_tile) HIPCHK(launch_maxexp(vb, p->Ec, p->Np, st));
     if (p->pairs)
-        HIPCHK(launch_split_pairs(vb, p->Ec, p->b, p->d, p->SB, p->Np, p->Kp, true, st));
+        HIPCHK(launch_split_pairs(vb, p->Ec, p->b, p->d, p->SB, p->Np, p->Kp, true, st, one_tile ? p->Ec : nullptr));
     else
-        HIPCHK(launch_split(vb, p->Ec, p->b, p->d, p->SB, p->Np, p->Kp, st));
+        HIPCHK(launch_split(vb, p->Ec, p->b, p->d, p->SB, p->Np, p->Kp, st, one_tile ? p->Ec : nullptr));
     return OZAIE_OK;
 }
 

@@ -228,6 +228,37 @@ __device__ __forceinline__ void split_stage(const double (&v)[TK / 8], double *t
     }
 }
 
+// Exponent maxima of the staged tile's 32 rows, for a tile that holds the whole contraction
+// axis (K <= 128): the exponent pass folded into the split.  Thread (row, q) takes the
+// maximum field of its 16 elements, and the eight q of a row meet in red[32 ..]; red[row]
+// then holds the row's maximum (zeros, padding included, count as field 0, the exponent
+// pass's starting value), also stored to E[row] when `write`.  Two barriers, every thread.
+template <bool KC>
+__device__ __forceinline__ void tile_row_max(const double *tile, int32_t *red, int32_t *__restrict__ E, bool write) {
+    constexpr int LDW = KC ? (128 + 1) : (32 + 1);
+    const int t = threadIdx.x, row = t & 31, q = t >> 5;
+    int m = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const double v = KC ? tile[row * LDW + 16 * q + i] : tile[(16 * q + i) * LDW + row];
+        const int e = (int)(((uint64_t)__double_as_longlong(v) >> 52) & 0x7FF);
+        m = e > m ? e : m;
+    }
+    red[32 + q * 32 + row] = m;
+    __syncthreads();
+    if (t < 32) {
+        int mm = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int x = red[32 + k * 32 + t];
+            mm = x > mm ? x : mm;
+        }
+        red[t] = mm;
+        if (write) E[t] = mm;
+    }
+    __syncthreads();
+}
+
 // Digits of the staged tile: thread (row, q) owns 16 consecutive k of one row and writes
 // one 16-byte lane chunk per slice plane (q = t / 32 + 8 p for the TK / 128 passes p).
 template <bool KC, int TK = 128>

@@ -109,8 +109,10 @@ __host__ __device__ inline PairCfg pair_cfg(int b, int d) {
     c.Q = c.H + 1;
     return c;
 }
+// Eout (non-null only when Kp <= 128): the split forms the exponent maxima itself and
+// stores them there, with no exponent pass before it.
 hipError_t launch_split_pairs(const OperandView &op, const int32_t *E, int b, int d, int8_t *S, int64_t Rp,
-                              int64_t Kp, bool reverse_q, hipStream_t st);
+                              int64_t Kp, bool reverse_q, hipStream_t st, int32_t *Eout = nullptr);
 // X[r][c] (int32, row stride ldx) = sum over Q*Kb contraction blocks of the single-digit
 // planes (Mp and Np multiples of 256; workgroup tiles of 256, 128 or 64 square by the grid
 // they give on `cus` CUs).
@@ -284,7 +286,7 @@ hipError_t launch_maxexp(const OperandView &op, int32_t *E, int64_t Rp, hipStrea
 hipError_t launch_debug_convert(const int64_t *hi, const uint64_t *lo, const int32_t *es, int64_t n, int b, int d,
                                 int rounding, double *out128, double *out64, hipStream_t st);
 hipError_t launch_split(const OperandView &op, const int32_t *E, int b, int d, int8_t *S,
-                        int64_t Rp, int64_t Kp, hipStream_t st);
+                        int64_t Rp, int64_t Kp, hipStream_t st, int32_t *Eout = nullptr);
 // Fused kernel (all d bins in registers); returns hipErrorInvalidValue if d has no
 // instantiation.  `wn` picks the wave tile (1: 32x32, 2: 32x64 per wave).
 bool fused_supported(int d);

@@ -69,14 +69,27 @@ hipError_t launch_maxexp(const OperandView &op, int32_t *E, int64_t Rp, hipStrea
     return hipGetLastError();
 }
 
-// grid (k tiles, row blocks, plane groups of `per` planes)
+// grid (k tiles, row blocks, plane groups of `per` planes).  Eout: the grid has one k tile,
+// which holds the whole contraction axis, so the exponent maxima come from the tile itself
+// (the workgroups of plane group 0 store them to Eout); otherwise E holds them.
 template <bool KC>
 __global__ __launch_bounds__(256) void k_split(const double *__restrict__ X, int64_t sr,
                                                int64_t sk, int64_t R, int64_t K,
                                                const int32_t *__restrict__ E, int b, int d,
-                                               int8_t *__restrict__ S, int64_t Rb, int64_t Kb, int per) {
+                                               int8_t *__restrict__ S, int64_t Rb, int64_t Kb, int per,
+                                               int32_t *__restrict__ Eout) {
     __shared__ double tile[kSplitLds / 8];
     const int j0 = (int)blockIdx.z * per;
+    if (Eout) {
+        __shared__ int32_t red[32 + 256];
+        double v[16];
+        split_gather<KC>(X, sr, sk, R, K, blockIdx.y, blockIdx.x, v);
+        split_stage<KC>(v, tile);
+        __syncthreads();
+        tile_row_max<KC>(tile, red, Eout + (int64_t)blockIdx.y * 32, blockIdx.z == 0);
+        split_emit<KC>(tile, R, red, b, d, S, Rb, Kb, blockIdx.y, blockIdx.x, j0, j0 + per);
+        return;
+    }
     split_tile<KC>(X, sr, sk, R, K, E + (int64_t)blockIdx.y * 32, b, d, S, Rb, Kb, blockIdx.y, blockIdx.x, tile,
                    j0, j0 + per);
 }
@@ -117,54 +130,61 @@ hipError_t launch_debug_convert(const int64_t *hi, const uint64_t *lo, const int
 }
 
 hipError_t launch_split(const OperandView &op, const int32_t *E, int b, int d, int8_t *S,
-                        int64_t Rp, int64_t Kp, hipStream_t st) {
+                        int64_t Rp, int64_t Kp, hipStream_t st, int32_t *Eout) {
     const int64_t Rb = Rp / kBlk, Kb = Kp / kBlk;
     const int64_t nkt = (Kb + 3) / 4;
-    if (Rb > 65535 || nkt > 0x7fffffff || d < 1) return hipErrorInvalidValue;
+    if (Rb > 65535 || nkt > 0x7fffffff || d < 1 || (Eout && nkt != 1)) return hipErrorInvalidValue;
     int per;
     const int groups = split_plane_groups(nkt * Rb, d, per);
     const dim3 grid((unsigned)nkt, (unsigned)Rb, (unsigned)groups);
     if (op.sk == 1)
         hipLaunchKernelGGL(k_split<true>, grid, dim3(256), 0, st, op.X, op.sr, op.sk, op.R, op.K, E, b, d, S, Rb,
-                           Kb, per);
+                           Kb, per, Eout);
     else
         hipLaunchKernelGGL(k_split<false>, grid, dim3(256), 0, st, op.X, op.sr, op.sk, op.R, op.K, E, b, d, S, Rb,
-                           Kb, per);
+                           Kb, per, Eout);
     return hipGetLastError();
 }
 
 // Paired-digit split (b <= 3 presets): the same grid and staging as k_split, pair planes
-// and single-digit slots written by emit_pairs16 (grid z: groups of `per` of the P + Q).
+// and single-digit slots written by emit_pairs16 (grid z: groups of `per` of the P + Q);
+// Eout as for k_split.
 template <bool KC>
 __global__ __launch_bounds__(256) void k_split_pairs(const double *__restrict__ X, int64_t sr, int64_t sk,
                                                      int64_t R, int64_t K, const int32_t *__restrict__ E, int b,
                                                      int d, int8_t *__restrict__ S, int64_t Rb, int64_t Kb,
-                                                     int reverse_q, int per) {
+                                                     int reverse_q, int per, int32_t *__restrict__ Eout) {
     __shared__ double tile[kSplitLds / 8];
+    __shared__ int32_t red[32 + 256];
     double v[16];
     split_gather<KC>(X, sr, sk, R, K, blockIdx.y, blockIdx.x, v);
     split_stage<KC>(v, tile);
     __syncthreads();
+    const int32_t *Erow = E + (int64_t)blockIdx.y * 32;
+    if (Eout) {
+        tile_row_max<KC>(tile, red, Eout + (int64_t)blockIdx.y * 32, blockIdx.z == 0);
+        Erow = red;
+    }
     const int g0 = (int)blockIdx.z * per;
-    split_emit_pairs<KC>(tile, R, E + (int64_t)blockIdx.y * 32, b, d, S, Rb, Kb, blockIdx.y, blockIdx.x,
-                         reverse_q != 0, g0, g0 + per);
+    split_emit_pairs<KC>(tile, R, Erow, b, d, S, Rb, Kb, blockIdx.y, blockIdx.x, reverse_q != 0, g0, g0 + per);
 }
 
 hipError_t launch_split_pairs(const OperandView &op, const int32_t *E, int b, int d, int8_t *S, int64_t Rp,
-                              int64_t Kp, bool reverse_q, hipStream_t st) {
+                              int64_t Kp, bool reverse_q, hipStream_t st, int32_t *Eout) {
     const int64_t Rb = Rp / kBlk, Kb = Kp / kBlk;
     const int64_t nkt = (Kb + 3) / 4;
-    if (Rb > 65535 || nkt > 0x7fffffff || b < 1 || b > 3 || d < 2 || b * d > 96) return hipErrorInvalidValue;
+    if (Rb > 65535 || nkt > 0x7fffffff || b < 1 || b > 3 || d < 2 || b * d > 96 || (Eout && nkt != 1))
+        return hipErrorInvalidValue;
     const PairCfg pc = pair_cfg(b, d);
     int per;
     const int groups = split_plane_groups(nkt * Rb, pc.P + pc.Q, per);
     const dim3 grid((unsigned)nkt, (unsigned)Rb, (unsigned)groups);
     if (op.sk == 1)
         hipLaunchKernelGGL(k_split_pairs<true>, grid, dim3(256), 0, st, op.X, op.sr, op.sk, op.R, op.K, E, b, d, S,
-                           Rb, Kb, (int)reverse_q, per);
+                           Rb, Kb, (int)reverse_q, per, Eout);
     else
         hipLaunchKernelGGL(k_split_pairs<false>, grid, dim3(256), 0, st, op.X, op.sr, op.sk, op.R, op.K, E, b, d, S,
-                           Rb, Kb, (int)reverse_q, per);
+                           Rb, Kb, (int)reverse_q, per, Eout);
     return hipGetLastError();
 }
 

@@ -158,6 +158,34 @@ def test_small_grid_tiles_match_default_tiles(gpu, d):
     assert np.array_equal(bits(out[1]), bits(ref))
 
 
+@pytest.mark.parametrize("K", [1, 64, 100, 128])
+@pytest.mark.parametrize("b,d", [(7, 8), (7, 3), (3, 29)])
+def test_short_K_split_forms_exponents(gpu, K, b, d):
+    """K <= 128: each split workgroup holds a row's whole contraction axis and forms the
+    row and column exponent maxima itself, with no exponent pass.  C equals the oracle's, bit
+    for bit, for plain and transposed operands, with zero rows, subnormals and mixed
+    magnitudes."""
+    import torch
+    rng = np.random.default_rng(K * 13 + d)
+    M, N = 70, 90
+    if K >= 6:
+        A = edge_matrix(rng, M, K)
+        B = np.ascontiguousarray(edge_matrix(rng, N, K).T)
+    else:
+        A = gen_inputs(rng, M, K, 2.0)
+        B = gen_inputs(rng, K, N, 2.0)
+        A[1, :] = 0.0
+        B[:, 2] = 5e-324
+    print(f"short K b={b} d={d} {M}x{N}x{K}", flush=True)
+    ref, _ = oracle.ozaki_dgemm(A, B, b, d)
+    for ta, tb in ((False, False), (True, True)):
+        plan = oz.Plan(M, N, K, b, d)
+        C = plan(dev(A.T) if ta else dev(A), dev(B.T) if tb else dev(B), transA=ta, transB=tb)
+        torch.cuda.synchronize()
+        plan.close()
+        assert np.array_equal(bits(C.cpu().numpy()), bits(ref)), (ta, tb)
+
+
 def test_C_nearest_even(gpu):
     import torch
     rng = np.random.default_rng(3)
