@@ -286,7 +286,8 @@ def test_multi_pass_kchunk_and_groups(gpu):
     M, N, K = 70, 90, 1000
     A = gen_inputs(rng, M, K, 1.0)
     B = gen_inputs(rng, K, N, 1.0)
-    for b, d, kernel in [(7, 8, 0), (3, 29, 1), (7, 8, 1), (7, 3, 5), (7, 9, 3), (7, 9, 5), (7, 9, 6), (4, 18, 0)]:
+    for b, d, kernel in [(7, 8, 0), (7, 3, 0), (7, 5, 0), (3, 29, 1), (7, 8, 1), (7, 3, 5), (7, 9, 3), (7, 9, 5),
+                         (7, 9, 6), (4, 18, 0)]:
         oz.set_option(oz.OPT_KCHUNK, 128)
         oz.set_option(oz.OPT_KERNEL, kernel)
         try:
